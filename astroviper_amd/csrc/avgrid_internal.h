@@ -1,0 +1,167 @@
+// avgrid_internal.h -- shared device helpers and the plan layout of
+// libavgrid.so.  Compiled for gfx950 only (wave64, 160 KB LDS per CU).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/avgrid.h"
+
+namespace av {
+
+// ------------------------------------------------------------ errors ----
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define AV_HIP(call)                                            \
+  do {                                                          \
+    hipError_t _e = (call);                                     \
+    if (_e != hipSuccess) return ::av::hip_fail(_e, #call);     \
+  } while (0)
+
+#define AV_LAUNCH_CHECK(name)                                   \
+  do {                                                          \
+    hipError_t _e = hipGetLastError();                          \
+    if (_e != hipSuccess) return ::av::hip_fail(_e, name);      \
+  } while (0)
+
+constexpr double kLightSpeed = 299792458.0;  // standard_grid.py:296
+constexpr uint32_t kInvalidGeo = 0xFFFFFFFFu;
+constexpr int kPosBias = 16;   // tile-relative positions are stored + 16
+constexpr int kOffBias = 128;  // kernel offsets are stored + 128
+constexpr int kMaxSupport = 16;
+constexpr int kMaxPol = 4;
+
+// Packed geometry of one sample relative to a tile origin:
+// bits 0-7 u position + 16, 8-15 u offset + 128, 16-23 v position + 16,
+// 24-31 v offset + 128.
+__host__ __device__ inline uint32_t pack_geo(int au, int ou, int av_, int ov) {
+  return (uint32_t)(au + kPosBias) | ((uint32_t)(ou + kOffBias) << 8) |
+         ((uint32_t)(av_ + kPosBias) << 16) | ((uint32_t)(ov + kOffBias) << 24);
+}
+__device__ inline void unpack_geo(uint32_t g, int &au, int &ou, int &av_, int &ov) {
+  au = (int)(g & 0xFFu) - kPosBias;
+  ou = (int)((g >> 8) & 0xFFu) - kOffBias;
+  av_ = (int)((g >> 16) & 0xFFu) - kPosBias;
+  ov = (int)((g >> 24) & 0xFFu) - kOffBias;
+}
+
+// --------------------------------------------------- FP64 geometry ----
+// Library-wide -ffp-contract=off keeps every product and sum below a
+// separately rounded IEEE operation, as in the reference's numba code.
+
+// standard_grid.py:322-352.  u = uvw_u * scale_u (scale precomputed per
+// channel as -(f*dl*N)/c), centre int(u + N//2 + 0.5) truncated toward zero,
+// bounds iu-sc >= 0 and iu+sc < N, offset floor((iu - pos)*ov + 0.5).
+__device__ inline bool grid_geometry(double uvw_u, double uvw_v, double scale_u,
+                                     double scale_v, int n_u, int n_v, int sc,
+                                     int ov, int &iu, int &iv, int &ou, int &ovv) {
+  double u = uvw_u * scale_u;
+  double v = uvw_v * scale_v;
+  if (isnan(u) || isnan(v)) return false;
+  double u_pos = u + (double)(n_u / 2);
+  double v_pos = v + (double)(n_v / 2);
+  double xu = u_pos + 0.5, xv = v_pos + 0.5;
+  // Anything outside +-1e9 fails the bounds test; reject before the cast.
+  if (!(xu > -1.0e9 && xu < 1.0e9 && xv > -1.0e9 && xv < 1.0e9)) return false;
+  int cu = (int)xu;  // int(): truncation toward zero
+  int cv = (int)xv;
+  if (!((cu + sc < n_u) && (cv + sc < n_v) && (cu - sc >= 0) && (cv - sc >= 0)))
+    return false;
+  double uo = (double)cu - u_pos;
+  double vo = (double)cv - v_pos;
+  iu = cu;
+  iv = cv;
+  ou = (int)floor(uo * (double)ov + 0.5);
+  ovv = (int)floor(vo * (double)ov + 0.5);
+  return true;
+}
+
+// standard_degrid.py:435-462 with scale = -dl*N (standard_degrid.py:572),
+// offset N/2 (float), pos = scale*uvw*freq/c + offset evaluated left to right,
+// loc = round-half-even(pos), off = round-half-even((loc - pos)*ov).
+// The reference bounds the u index by grid.shape[3] and v by grid.shape[2]
+// (standard_degrid.py:407,457-462); both those and the true axis bounds
+// are applied (they differ only for non-square grids).
+__device__ inline bool degrid_geometry(double uvw_u, double uvw_v, double freq,
+                                       double dscale_u, double dscale_v,
+                                       double doff_u, double doff_v, int n_u,
+                                       int n_v, int support, int ov, int &lu,
+                                       int &lv, int &fu, int &fv) {
+  double pu = dscale_u * uvw_u * freq / kLightSpeed + doff_u;
+  double pv = dscale_v * uvw_v * freq / kLightSpeed + doff_v;
+  if (isnan(pu) || isnan(pv)) return false;
+  double ru = rint(pu), rv = rint(pv);
+  if (!(ru > -1.0e9 && ru < 1.0e9 && rv > -1.0e9 && rv < 1.0e9)) return false;
+  int x0 = (int)ru, y0 = (int)rv;
+  int sb = -(support / 2);
+  int se = sb + support;
+  int bu = n_u < n_v ? n_u : n_v;  // min(shape[3] bound, true bound)
+  int bv = bu;
+  if (!((x0 + sb >= 0) && (x0 + se - 1 < bu) && (y0 + sb >= 0) && (y0 + se - 1 < bv)))
+    return false;
+  lu = x0;
+  lv = y0;
+  fu = (int)rint(((double)x0 - pu) * (double)ov);
+  fv = (int)rint(((double)y0 - pv) * (double)ov);
+  return true;
+}
+
+// ------------------------------------------------------------- plan ----
+struct PlanDev {
+  // sizes
+  int64_t n_rows;
+  int32_t n_chan, n_img_chan, n_u, n_v;
+  double cell_u, cell_v;
+  double dscale_u, dscale_v, doff_u, doff_v;  // degrid pixel transform
+  int32_t support, ov, tile_u, tile_v, geometry, unit_cap;
+  int32_t ntu, ntv;      // tiles per axis
+  int64_t n_bins;        // ntu * ntv * n_img_chan
+  int64_t n_items;       // samples (row, chan)
+  int64_t max_records;   // capacity of the record arrays
+  int64_t max_gunits, max_dunits;
+  // per-channel data
+  double *gscale_u, *gscale_v;  // [n_chan] grid scale -(f*dl*N)/c
+  double *freq;                 // [n_chan] copy of the frequencies
+  int32_t *chan_map;            // [n_chan]
+  int32_t *row_chan;            // [n_rows] or nullptr (not owned)
+  const double *uvw;            // [n_rows][3] (not owned; valid until next set)
+  // bins
+  uint32_t *cnt_all, *cnt_prim;     // [n_bins]
+  uint32_t *bin_start;              // [n_bins + 1]
+  uint32_t *cur_prim, *cur_dup;     // [n_bins]
+  uint32_t *gunit_start, *dunit_start;  // [n_bins + 1]
+  uint32_t *scan_tmp;               // block sums
+  // records (SoA)
+  uint32_t *rec_idx, *rec_g, *rec_d;  // [max_records]
+  // work units: x = bin, y = first record, z = count, w = flags
+  uint4 *gunits, *dunits;
+  // zero list: samples no degrid visits
+  uint32_t *zero_idx;               // [n_items]
+  // counters: [0] valid-grid, [1] valid-degrid, [2] zero count,
+  // [3] grid work cursor, [4] degrid work cursor, [5] zero cursor
+  uint32_t *counters;
+};
+
+constexpr uint32_t kUnitMulti = 1u;  // bin split over several units
+constexpr uint32_t kUnitFirst = 2u;  // first unit of its bin
+
+struct av_plan_host;  // defined in plan.hip
+
+// launch helpers shared by the translation units
+int launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n,
+                       uint32_t *tmp, hipStream_t s);
+
+}  // namespace av
+
+struct av_plan {
+  av::PlanDev d;
+  void *alloc_base;   // single device allocation holding every plan array
+  size_t alloc_bytes;
+  int device;
+};

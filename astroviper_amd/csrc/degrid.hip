@@ -1,0 +1,280 @@
+// degrid.hip -- tiled degridding (replaces dgrid_numba,
+// core/imaging/imaging_utils/standard_degrid.py:388-485, zero dphase).
+//
+// A persistent kernel pulls degrid units (primary records of one bin).  The
+// unit copies its tile plus a halo of support/2 + 1 pixels of every grid
+// polarisation plane into LDS with coalesced reads, then each wavefront
+// evaluates samples: lanes = (component, v-tap) read one LDS row per u-tap
+// (bank-conflict free), multiply by the u and v kernel values, and a DPP
+// reduction over the v-taps gives the sample, divided by the kernel-weight
+// sum as in the reference.  Each sample is written exactly once.
+#include <algorithm>
+
+#include "tile_common.h"
+
+namespace av {
+
+constexpr int kDegridThreads = 256;
+constexpr int kDegridWaves = kDegridThreads / 64;
+
+struct DegridLds {
+  int tab, reg, st_cu, st_cv, st_pos, st_item, s_unit, total;
+  int ntab, halo, ru, rv, rps;
+};
+
+template <typename R, int DVW>
+__host__ __device__ inline DegridLds degrid_lds_layout(int ov, int support, int tu, int tv,
+                                                       int npol_g) {
+  DegridLds L;
+  int sc = support / 2;
+  L.ntab = ov * (sc + 1);
+  L.halo = sc + 1;
+  L.ru = tu + 2 * L.halo;
+  L.rv = tv + 2 * L.halo;
+  L.rps = plane_stride(L.ru * L.rv, DVW);
+  int o = 0;
+  auto take = [&](int bytes) {
+    int r = o;
+    o += (bytes + 15) & ~15;
+    return r;
+  };
+  L.tab = take(L.ntab * (int)sizeof(R));
+  L.reg = take(2 * npol_g * L.rps * (int)sizeof(R));
+  L.st_cu = take(kDegridWaves * 64 * DVW * (int)sizeof(R));
+  L.st_cv = take(kDegridWaves * 64 * DVW * (int)sizeof(R));
+  L.st_pos = take(kDegridWaves * 64 * (int)sizeof(int));
+  L.st_item = take(kDegridWaves * 64 * (int)sizeof(uint32_t));
+  L.s_unit = take(16);
+  L.total = o;
+  return L;
+}
+
+template <typename R, int NPOL, int DVW>
+__global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
+    PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
+    const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
+    int mode) {
+  using LM = LaneMap<NPOL, DVW>;
+  extern __shared__ __align__(16) char smem[];
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const int Tu = P.tile_u, Tv = P.tile_v;
+  const DegridLds L = degrid_lds_layout<R, DVW>(ov, S, Tu, Tv, npol_g);
+  R *tab = (R *)(smem + L.tab);
+  R *reg = (R *)(smem + L.reg);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  R *st_cu = (R *)(smem + L.st_cu) + wave * 64 * DVW;
+  R *st_cv = (R *)(smem + L.st_cv) + wave * 64 * DVW;
+  int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
+  uint32_t *st_item = (uint32_t *)(smem + L.st_item) + wave * 64;
+  uint32_t *s_unit = (uint32_t *)(smem + L.s_unit);
+  const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const int H = L.halo;
+
+  for (int i = tid; i < L.ntab; i += kDegridThreads) tab[i] = (R)cgk[i];
+  const uint32_t n_units = P.dunit_start[P.n_bins];
+
+  const int slot = lane / LM::kLanesPerItem;
+  const int lq = (lane % LM::kLanesPerItem) / DVW;
+  const int dv = lane % DVW;
+  int gq[LM::kQPerLane];  // LDS plane of each component this lane handles
+#pragma unroll
+  for (int k = 0; k < LM::kQPerLane; ++k) {
+    int q = lq + k * LM::kQStride;
+    int p = q >> 1;
+    int pg = pol_map ? pol_map[p] : p % npol_g;
+    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);  // stay inside the LDS region
+    gq[k] = 2 * pg + (q & 1);
+  }
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) *s_unit = atomicAdd(&P.counters[4], 1u);
+    __syncthreads();
+    const uint32_t u = *s_unit;
+    if (u >= n_units) break;
+    const uint4 unit = P.dunits[u];
+    const int64_t b = unit.x;
+    const uint32_t first = unit.y, cnt = unit.z;
+    const int ic = (int)(b % P.n_img_chan);
+    const int64_t tile = b / P.n_img_chan;
+    const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+    const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
+    const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+    // ---- tile + halo of every grid polarisation into LDS (zero outside)
+    const int rcells = L.ru * L.rv;
+    for (int idx = tid; idx < npol_g * rcells; idx += kDegridThreads) {
+      const int p = idx / rcells, c = idx - p * rcells;
+      const int ru = c / L.rv, rv = c - ru * L.rv;
+      const int gu = u0 - H + ru, gv = v0 - H + rv;
+      R re = (R)0, im = (R)0;
+      if (gu >= 0 && gu < P.n_u && gv >= 0 && gv < P.n_v) {
+        const R *src = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+        re = src[0];
+        im = src[1];
+      }
+      reg[(2 * p) * L.rps + c] = re;
+      reg[(2 * p + 1) * L.rps + c] = im;
+    }
+    __syncthreads();
+
+    const uint32_t end = first + cnt;
+    for (uint32_t r0 = first; r0 < end; r0 += kDegridThreads) {
+      const uint32_t r = r0 + wave * 64 + lane;
+      int pos = -1;
+      if (r < end) {
+        const uint32_t g = P.rec_d[r];
+        if (g != kInvalidGeo) {
+          int au, ou, avv, ovv;
+          unpack_geo(g, au, ou, avv, ovv);
+          pos = (au + kPosBias) | ((avv + kPosBias) << 16);
+          st_item[lane] = P.rec_idx[r];
+#pragma unroll
+          for (int t = 0; t < DVW; ++t) {
+            st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
+            st_cv[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
+          }
+        }
+      }
+      st_pos[lane] = pos;
+      wave_lds_sync();
+      const int nst = (int)min((uint32_t)64, end > r0 + wave * 64 ? end - (r0 + wave * 64) : 0u);
+      for (int j0 = 0; j0 < nst; j0 += LM::kItemsPerIter) {
+        const int j = j0 + slot;
+        const int pj = j < nst ? st_pos[j] : -1;
+        // The kernel-weight sum is formed with exactly the operations applied
+        // to the data, so a constant grid degrids to that constant exactly
+        // (the reference sums wt*g and wt in the same loop).
+        R accv[LM::kQPerLane];
+        R nacc = (R)0;
+#pragma unroll
+        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = (R)0;
+        int au = 0, avv = 0;
+        if (pj >= 0) {
+          au = (pj & 0xFFFF) - kPosBias;
+          avv = (pj >> 16) - kPosBias;
+        }
+        if (pj >= 0 && dv < S) {
+          const int col = avv + H + dv - sc;
+          for (int du = 0; du < S; ++du) {
+            const R cu = st_cu[j * DVW + du];
+            const int row = (au + H + du - sc) * L.rv + col;
+            nacc += cu;
+#pragma unroll
+            for (int k = 0; k < LM::kQPerLane; ++k) accv[k] += cu * reg[gq[k] * L.rps + row];
+          }
+          const R cv = st_cv[j * DVW + dv];
+          nacc *= cv;
+#pragma unroll
+          for (int k = 0; k < LM::kQPerLane; ++k) accv[k] *= cv;
+        }
+        // every lane takes part in the DPP reductions
+#pragma unroll
+        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = group_sum_to_last<DVW>(accv[k]);
+        nacc = group_sum_to_last<DVW>(nacc);
+        if (pj >= 0 && dv == DVW - 1) {
+          const uint32_t item = st_item[j];
+          const R nv = nacc;
+#pragma unroll
+          for (int k = 0; k < LM::kQPerLane; ++k) {
+            const int q = lq + k * LM::kQStride;
+            const int p = q >> 1;
+            const size_t o = ((size_t)item * NPOL + p) * 2 + (q & 1);
+            const bool flagged = flag && flag[(size_t)item * NPOL + p];
+            if (incremental) {
+              if (!flagged) vis_out[o] += accv[k] / nv;
+            } else {
+              vis_out[o] = flagged ? (R)0 : accv[k] / nv;
+            }
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+// non-incremental mode: samples no unit visits (off-grid, NaN, bad channel)
+template <typename R>
+__global__ void k_zero_samples(PlanDev P, R *vis_out, int npol) {
+  const int64_t n = (int64_t)P.counters[2] * npol * 2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t z = i / (2 * npol), k = i - z * 2 * npol;
+    vis_out[(size_t)P.zero_idx[z] * npol * 2 + k] = (R)0;
+  }
+}
+
+template <typename R, int NPOL, int DVW>
+static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
+                         const uint8_t *flag, const double *cgk, void *vis_out, int mode,
+                         hipStream_t s) {
+  PlanDev &d = plan->d;
+  DegridLds L = degrid_lds_layout<R, DVW>(d.ov, d.support, d.tile_u, d.tile_v, npol_g);
+  auto kfn = k_degrid_tiles<R, NPOL, DVW>;
+  int dev = 0;
+  AV_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  AV_HIP(hipGetDeviceProperties(&prop, dev));
+  if ((size_t)L.total > prop.sharedMemPerBlock)
+    return fail(AV_ERR_UNSUPPORTED, "av_degrid: tile too large for LDS (" +
+                                        std::to_string(L.total) + " bytes)");
+  AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             L.total));
+  int per_cu = 0;
+  AV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kDegridThreads, L.total));
+  if (per_cu < 1) per_cu = 1;
+  int blocks = per_cu * prop.multiProcessorCount;
+  int rc = launch_fill_u32(d.counters + 4, 0u, 1, s);
+  if (rc) return rc;
+  if (!(mode & AV_DEGRID_INCREMENTAL)) {
+    hipLaunchKernelGGL(k_zero_samples<R>, dim3(2048), dim3(256), 0, s, d, (R *)vis_out, NPOL);
+    AV_LAUNCH_CHECK("k_zero_samples");
+  }
+  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kDegridThreads), L.total, s, d, (const R *)grid,
+                     npol_g, pol_map, flag, cgk, (R *)vis_out, mode);
+  AV_LAUNCH_CHECK("k_degrid_tiles");
+  return AV_OK;
+}
+
+template <typename R, int NPOL>
+static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
+                           const uint8_t *flag, const double *cgk, void *vis_out, int mode,
+                           hipStream_t s) {
+  if (plan->d.support <= 8)
+    return launch_degrid<R, NPOL, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+  return launch_degrid<R, NPOL, 16>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+}
+
+template <typename R>
+static int degrid_by_pol(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
+                         const uint8_t *flag, int npol, const double *cgk, void *vis_out,
+                         int mode, hipStream_t s) {
+  switch (npol) {
+    case 1: return dispatch_degrid<R, 1>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+    case 2: return dispatch_degrid<R, 2>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+    case 4: return dispatch_degrid<R, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+    default: return fail(AV_ERR_UNSUPPORTED, "av_degrid: npol must be 1, 2 or 4");
+  }
+}
+
+}  // namespace av
+
+using namespace av;
+
+extern "C" int av_degrid(av_plan *plan, const void *grid, int32_t n_grid_pol,
+                         const int32_t *pol_map, const uint8_t *flag, int32_t n_pol,
+                         const double *cgk, void *vis_out, int32_t dtype, int32_t mode,
+                         void *stream) {
+  if (!plan || !grid || !cgk || !vis_out) return fail(AV_ERR_INVALID, "av_degrid: NULL argument");
+  if (n_grid_pol < 1 || n_grid_pol > kMaxPol)
+    return fail(AV_ERR_UNSUPPORTED, "av_degrid: n_grid_pol must be in [1, 4]");
+  if (!plan->d.uvw) return fail(AV_ERR_INVALID, "av_degrid: plan has no uvw (av_plan_set_uvw)");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == AV_F32)
+    return degrid_by_pol<float>(plan, grid, n_grid_pol, pol_map, flag, n_pol, cgk, vis_out, mode, s);
+  if (dtype == AV_F64)
+    return degrid_by_pol<double>(plan, grid, n_grid_pol, pol_map, flag, n_pol, cgk, vis_out, mode,
+                                 s);
+  return fail(AV_ERR_INVALID, "av_degrid: dtype must be AV_F32 or AV_F64");
+}

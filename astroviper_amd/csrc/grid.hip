@@ -1,0 +1,433 @@
+// grid.hip -- tiled convolutional gridding (replaces standard_grid_jit,
+// core/imaging/imaging_utils/standard_grid.py:246-432).
+//
+// One persistent kernel pulls work units (a bin, or a slice of a large bin)
+// from a device counter.  A unit owns a tile_u x tile_v tile of every
+// polarisation plane of one image channel.  The tile is accumulated in LDS
+// with LDS atomics (one wavefront per sample, lanes = (component, v-tap),
+// loop over u-taps; only taps inside the tile are added, which is why each
+// footprint is recorded in every tile it overlaps) and written to HBM once,
+// row by row, coalesced.  The kernel table sits in LDS.  A bin split over
+// several units adds its partial tiles with global atomics into a tile that
+// a pre-pass zeroed.  Sums of weights are FP64.
+#include <algorithm>
+
+#include "tile_common.h"
+
+namespace av {
+
+constexpr int kGridThreads = 256;
+constexpr int kGridWaves = kGridThreads / 64;
+constexpr int kCompChunkRounds = 4;  // compensated mode: fold every 4*256 records
+
+template <typename R>
+struct Cplx {
+  R re, im;
+};
+
+struct GridLds {
+  // offsets in bytes inside dynamic LDS
+  int tab, nrm, acc, st_d, st_cu, st_cv, st_pos, red, total;
+  int ntab, nnrm, ps;
+};
+
+template <typename R, int NPOL, int DVW>
+__host__ __device__ inline GridLds grid_lds_layout(int ov, int support, int tu, int tv) {
+  GridLds L;
+  int sc = support / 2;
+  L.ntab = ov * (sc + 1);
+  L.nnrm = 2 * (ov / 2 + 2) + 1;  // offsets in [-(ov/2+2), ov/2+2]
+  L.ps = plane_stride(tu * tv, DVW);
+  int o = 0;
+  auto take = [&](int bytes) {
+    int r = o;
+    o += (bytes + 15) & ~15;
+    return r;
+  };
+  L.tab = take(L.ntab * (int)sizeof(R));
+  L.nrm = take(L.nnrm * (int)sizeof(double));
+  L.acc = take(2 * NPOL * L.ps * (int)sizeof(R));
+  L.st_d = take(kGridWaves * 64 * 2 * NPOL * (int)sizeof(R));
+  L.st_cu = take(kGridWaves * 64 * DVW * (int)sizeof(R));
+  L.st_cv = take(kGridWaves * 64 * DVW * (int)sizeof(R));
+  L.st_pos = take(kGridWaves * 64 * (int)sizeof(int));
+  L.red = take(kGridWaves * NPOL * (int)sizeof(double) + 16);
+  L.total = o;
+  return L;
+}
+
+// norm of one axis: sum over taps of the table, FP64, for offset `off`
+__device__ inline double axis_norm(const double *cgk, int ov, int support, int off) {
+  int sc = support / 2;
+  double s = 0.0;
+  for (int t = 0; t < support; ++t) s += cgk[tab_index(ov, t, sc, off)];
+  return s;
+}
+
+template <typename R, int NPOL, int DVW, bool COMP>
+__global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R *__restrict__ vis,
+                                                             const R *__restrict__ weight,
+                                                             const uint8_t *__restrict__ flag,
+                                                             const double *__restrict__ cgk,
+                                                             R *__restrict__ grid,
+                                                             double *__restrict__ sumwt, int mode) {
+  using LM = LaneMap<NPOL, DVW>;
+  extern __shared__ __align__(16) char smem[];
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const int Tu = P.tile_u, Tv = P.tile_v;
+  const GridLds L = grid_lds_layout<R, NPOL, DVW>(ov, S, Tu, Tv);
+  R *tab = (R *)(smem + L.tab);
+  double *nrm = (double *)(smem + L.nrm);
+  R *acc = (R *)(smem + L.acc);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  R *st_d = (R *)(smem + L.st_d) + wave * 64 * 2 * NPOL;
+  R *st_cu = (R *)(smem + L.st_cu) + wave * 64 * DVW;
+  R *st_cv = (R *)(smem + L.st_cv) + wave * 64 * DVW;
+  int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
+  double *red = (double *)(smem + L.red);
+  uint32_t *s_unit = (uint32_t *)(red + kGridWaves * NPOL);
+
+  const bool psf = (mode & AV_GRID_PSF) != 0;
+  const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
+  const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
+  const int half = ov / 2 + 2;
+
+  for (int i = tid; i < L.ntab; i += kGridThreads) tab[i] = (R)cgk[i];
+  for (int i = tid; i < L.nnrm; i += kGridThreads) nrm[i] = axis_norm(cgk, ov, S, i - half);
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  const int cells = Tu * Tv;
+
+  // lane -> (slot, component, v-tap)
+  const int slot = lane / LM::kLanesPerItem;
+  const int lq = (lane % LM::kLanesPerItem) / DVW;
+  const int dv = lane % DVW;
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) *s_unit = atomicAdd(&P.counters[3], 1u);
+    __syncthreads();
+    const uint32_t u = *s_unit;
+    if (u >= n_units) break;
+    const uint4 unit = P.gunits[u];
+    const int64_t b = unit.x;
+    const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
+    const int ic = (int)(b % P.n_img_chan);
+    const int64_t tile = b / P.n_img_chan;
+    const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+    const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
+    R *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
+    const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+    if (cnt == 0) {
+      if (!accumulate) {
+        for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+          int p = idx / cells, c = idx - p * cells;
+          int gu = u0 + c / Tv, gv = v0 + c % Tv;
+          if (gu < P.n_u && gv < P.n_v) {
+            Cplx<R> z{(R)0, (R)0};
+            *(Cplx<R> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) = z;
+          }
+        }
+      }
+      continue;
+    }
+
+    for (int i = tid; i < 2 * NPOL * L.ps; i += kGridThreads) acc[i] = (R)0;
+    // compensated mode: per-thread running sums of the cells it flushes
+    constexpr int kCpt = COMP ? 32 : 1;
+    R csum[kCpt], ccmp[kCpt];
+    if (COMP) {
+#pragma unroll
+      for (int k = 0; k < kCpt; ++k) csum[k] = ccmp[k] = (R)0;
+    }
+    double sw[NPOL];
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+    __syncthreads();
+
+    const uint32_t end = first + cnt;
+    int round = 0;
+    for (uint32_t r0 = first; r0 < end; r0 += kGridThreads, ++round) {
+      // ---- stage: one record per lane
+      const uint32_t r = r0 + wave * 64 + lane;
+      int pos = -1;
+      if (r < end) {
+        const uint32_t g = P.rec_g[r];
+        if (g != kInvalidGeo) {
+          const uint32_t item = P.rec_idx[r];
+          int au, ou, avv, ovv;
+          unpack_geo(g, au, ou, avv, ovv);
+          const double norm = nrm[ou + half] * nrm[ovv + half];
+          const bool prim = r < prim_end;
+          R wv[NPOL];
+          Cplx<R> vv[NPOL];
+#pragma unroll
+          for (int p = 0; p < NPOL; ++p) {
+            if (!adjoint) wv[p] = weight[(size_t)item * NPOL + p];
+            if (!psf) vv[p] = ((const Cplx<R> *)vis)[(size_t)item * NPOL + p];
+          }
+          bool any = false;
+#pragma unroll
+          for (int p = 0; p < NPOL; ++p) {
+            R dre, dim;
+            if (adjoint) {
+              R inv = (R)(1.0 / norm);
+              dre = vv[p].re * inv;
+              dim = vv[p].im * inv;
+            } else {
+              R w = wv[p];
+              if (flag && flag[(size_t)item * NPOL + p]) w = (R)0;
+              if (psf) {
+                dre = w;
+                dim = (R)0;
+              } else {
+                dre = vv[p].re * w;
+                dim = vv[p].im * w;
+              }
+            }
+            // standard_grid.py:378: skip NaN or exactly-zero data
+            bool ok = !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0);
+            if (!ok) {
+              dre = (R)0;
+              dim = (R)0;
+            } else {
+              any = true;
+              if (prim && !adjoint) sw[p] += (double)wv[p] * norm;
+            }
+            st_d[lane * 2 * NPOL + 2 * p] = dre;
+            st_d[lane * 2 * NPOL + 2 * p + 1] = dim;
+          }
+          if (any) {
+            pos = (au + kPosBias) | ((avv + kPosBias) << 16);
+#pragma unroll
+            for (int t = 0; t < DVW; ++t) {
+              st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
+              st_cv[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
+            }
+          }
+        }
+      }
+      st_pos[lane] = pos;
+      wave_lds_sync();
+      // ---- scatter-add into the LDS tile
+      const int nst = (int)min((uint32_t)64, end > r0 + wave * 64 ? end - (r0 + wave * 64) : 0u);
+      for (int j0 = 0; j0 < nst; j0 += LM::kItemsPerIter) {
+        const int j = j0 + slot;
+        const int pj = j < nst ? st_pos[j] : -1;
+        if (pj < 0) continue;
+        const int au = (pj & 0xFFFF) - kPosBias;
+        const int avv = (pj >> 16) - kPosBias;
+        const int v = avv + dv - sc;
+        if (dv >= S || v < 0 || v >= Tv) continue;
+        const R cv = st_cv[j * DVW + dv];
+        R x[LM::kQPerLane];
+        int base[LM::kQPerLane];
+#pragma unroll
+        for (int k = 0; k < LM::kQPerLane; ++k) {
+          const int q = lq + k * LM::kQStride;
+          x[k] = st_d[j * 2 * NPOL + q] * cv;
+          base[k] = q * L.ps + v;
+        }
+        const int dlo = max(0, sc - au), dhi = min(S, Tu + sc - au);
+        for (int du = dlo; du < dhi; ++du) {
+          const R cu = st_cu[j * DVW + du];
+          const int row = (au + du - sc) * Tv;
+#pragma unroll
+          for (int k = 0; k < LM::kQPerLane; ++k) atomicAdd(&acc[base[k] + row], x[k] * cu);
+        }
+      }
+      wave_lds_sync();
+      if (COMP && ((round + 1) % kCompChunkRounds == 0) && r0 + kGridThreads < end) {
+        // fold the LDS partial sums into the per-thread compensated sums
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kCpt; ++k) {
+          int idx = tid + k * kGridThreads;
+          if (idx < 2 * NPOL * cells) {
+            int q = idx / cells, c = idx - q * cells;
+            R *a = &acc[q * L.ps + c];
+            R x = *a;
+            *a = (R)0;
+            R t = csum[k] + x;  // Neumaier
+            if (fabs(csum[k]) >= fabs(x))
+              ccmp[k] += (csum[k] - t) + x;
+            else
+              ccmp[k] += (x - t) + csum[k];
+            csum[k] = t;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+
+    // ---- flush the tile
+    const bool multi = (fl & kUnitMulti) != 0;
+    if (COMP) {
+#pragma unroll
+      for (int k = 0; k < kCpt; ++k) {
+        int idx = tid + k * kGridThreads;
+        if (idx < 2 * NPOL * cells) {
+          int q = idx / cells, c = idx - q * cells;
+          R x = acc[q * L.ps + c];
+          R t = csum[k] + x;
+          if (fabs(csum[k]) >= fabs(x))
+            ccmp[k] += (csum[k] - t) + x;
+          else
+            ccmp[k] += (x - t) + csum[k];
+          acc[q * L.ps + c] = t + ccmp[k];
+        }
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+      const int p = idx / cells, c = idx - p * cells;
+      const int gu = u0 + c / Tv, gv = v0 + c % Tv;
+      if (gu >= P.n_u || gv >= P.n_v) continue;
+      Cplx<R> val{acc[(2 * p) * L.ps + c], acc[(2 * p + 1) * L.ps + c]};
+      R *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+      if (multi) {
+        if (val.re != (R)0) atomicAdd(dst, val.re);
+        if (val.im != (R)0) atomicAdd(dst + 1, val.im);
+      } else if (accumulate) {
+        Cplx<R> old = *(Cplx<R> *)dst;
+        old.re += val.re;
+        old.im += val.im;
+        *(Cplx<R> *)dst = old;
+      } else {
+        *(Cplx<R> *)dst = val;
+      }
+    }
+    // ---- sums of weights (FP64)
+    if (!adjoint) {
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        double s = sw[p];
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) red[wave * NPOL + p] = s;
+      }
+      __syncthreads();
+      if (tid < NPOL) {
+        double s = 0.0;
+        for (int w = 0; w < kGridWaves; ++w) s += red[w * NPOL + tid];
+        if (s != 0.0) atomicAdd(&sumwt[ic * NPOL + tid], s);
+      }
+    }
+  }
+}
+
+// zero the tiles of bins that several units add into (overwrite mode)
+template <typename R>
+__global__ void k_zero_multi(PlanDev P, R *grid, int npol) {
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  const int Tu = P.tile_u, Tv = P.tile_v, cells = Tu * Tv;
+  for (uint32_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const uint4 unit = P.gunits[u];
+    if (!((unit.w & kUnitMulti) && (unit.w & kUnitFirst))) continue;
+    const int64_t b = unit.x;
+    const int ic = (int)(b % P.n_img_chan);
+    const int64_t tile = b / P.n_img_chan;
+    const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+    for (int idx = threadIdx.x; idx < npol * cells; idx += blockDim.x) {
+      int p = idx / cells, c = idx - p * cells;
+      int gu = u0 + c / Tv, gv = v0 + c % Tv;
+      if (gu < P.n_u && gv < P.n_v) {
+        R *dst = grid + (((size_t)ic * npol + p) * P.n_u + gu) * P.n_v * 2 + (size_t)gv * 2;
+        dst[0] = (R)0;
+        dst[1] = (R)0;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_f64(double *p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+template <typename R, int NPOL, int DVW, bool COMP>
+static int launch_grid(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                       const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s) {
+  PlanDev &d = plan->d;
+  GridLds L = grid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v);
+  auto kfn = k_grid_tiles<R, NPOL, DVW, COMP>;
+  int dev = 0;
+  AV_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  AV_HIP(hipGetDeviceProperties(&prop, dev));
+  if ((size_t)L.total > prop.sharedMemPerBlock)
+    return fail(AV_ERR_UNSUPPORTED, "av_grid: tile too large for LDS (" +
+                                        std::to_string(L.total) + " bytes)");
+  AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             L.total));
+  int per_cu = 0;
+  AV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kGridThreads, L.total));
+  if (per_cu < 1) per_cu = 1;
+  int blocks = per_cu * prop.multiProcessorCount;
+  int rc = launch_fill_u32(d.counters + 3, 0u, 1, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kGridThreads), L.total, s, d, (const R *)vis,
+                     (const R *)weight, flag, cgk, (R *)grid, sumwt, mode);
+  AV_LAUNCH_CHECK("k_grid_tiles");
+  return AV_OK;
+}
+
+template <typename R, int NPOL>
+static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                         const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s) {
+  const bool comp = (mode & AV_GRID_COMPENSATED) != 0;
+  const PlanDev &d = plan->d;
+  if (comp) {
+    if (sizeof(R) != 4)
+      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode is for FP32 grids");
+    if (2 * NPOL * d.tile_u * d.tile_v > 32 * kGridThreads)
+      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode needs tile area <= 4096/npol");
+  }
+  if (d.support <= 8) {
+    if (comp) return launch_grid<R, NPOL, 8, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+    return launch_grid<R, NPOL, 8, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+  }
+  if (comp) return launch_grid<R, NPOL, 16, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+  return launch_grid<R, NPOL, 16, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+}
+
+template <typename R>
+static int grid_by_pol(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                       int npol, const double *cgk, void *grid, double *sumwt, int mode,
+                       hipStream_t s) {
+  PlanDev &d = plan->d;
+  if (!(mode & AV_GRID_ACCUMULATE)) {
+    int64_t n = (int64_t)d.n_img_chan * npol;
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sumwt, n);
+    AV_LAUNCH_CHECK("k_zero_f64");
+    hipLaunchKernelGGL(k_zero_multi<R>, dim3(1024), dim3(256), 0, s, d, (R *)grid, npol);
+    AV_LAUNCH_CHECK("k_zero_multi");
+  }
+  switch (npol) {
+    case 1: return dispatch_grid<R, 1>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+    case 2: return dispatch_grid<R, 2>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+    case 4: return dispatch_grid<R, 4>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+    default: return fail(AV_ERR_UNSUPPORTED, "av_grid: npol must be 1, 2 or 4");
+  }
+}
+
+}  // namespace av
+
+using namespace av;
+
+extern "C" int av_grid(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                       int32_t n_pol, const double *cgk, void *grid, double *sumwt, int32_t dtype,
+                       int32_t mode, void *stream) {
+  if (!plan || !cgk || !grid || !sumwt) return fail(AV_ERR_INVALID, "av_grid: NULL argument");
+  const bool psf = (mode & AV_GRID_PSF) != 0, adjoint = (mode & AV_GRID_ADJOINT) != 0;
+  if (!psf && !vis) return fail(AV_ERR_INVALID, "av_grid: vis is NULL");
+  if (!adjoint && !weight) return fail(AV_ERR_INVALID, "av_grid: weight is NULL");
+  if (psf && adjoint) return fail(AV_ERR_INVALID, "av_grid: PSF and ADJOINT are exclusive");
+  if (!plan->d.uvw) return fail(AV_ERR_INVALID, "av_grid: plan has no uvw (av_plan_set_uvw)");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == AV_F32)
+    return grid_by_pol<float>(plan, vis, weight, flag, n_pol, cgk, grid, sumwt, mode, s);
+  if (dtype == AV_F64)
+    return grid_by_pol<double>(plan, vis, weight, flag, n_pol, cgk, grid, sumwt, mode, s);
+  return fail(AV_ERR_INVALID, "av_grid: dtype must be AV_F32 or AV_F64");
+}

@@ -1,0 +1,429 @@
+// plan.hip -- binning of samples into (uv tile, image channel) bins.
+//
+// A sample is one (row, channel) pair with all its polarisations.  Its grid
+// centre (standard_grid.py:322-352) and degrid centre
+// (standard_degrid.py:435-462) are computed once, in FP64, here.  The sample
+// is recorded in the bin of its anchor tile ("primary" record, carrying both
+// geometries) and, for gridding, in every other tile its support footprint
+// overlaps ("duplicate" records), so that each tile of the grid is produced
+// by exactly one owner with no halo exchange.  Within a bin, primary records
+// come first.  Bins are ordered tile-major, channel-minor, so neighbouring
+// channels of the same rows are processed close together in time.
+#include <algorithm>
+
+#include "avgrid_internal.h"
+
+namespace av {
+
+struct ItemGeo {
+  int64_t row;
+  int chan, ic;
+  bool gvalid, dvalid;
+  int iu, iv, ou, ov;  // grid centre and kernel offsets
+  int lu, lv, fu, fv;  // degrid centre and kernel offsets
+};
+
+__device__ inline void item_geometry(const PlanDev &P, int64_t item, ItemGeo &g) {
+  if (P.row_chan) {
+    g.row = item;
+    g.chan = P.row_chan[item];
+  } else {
+    g.row = item / P.n_chan;
+    g.chan = (int)(item - g.row * P.n_chan);
+  }
+  bool chan_ok = g.chan >= 0 && g.chan < P.n_chan;
+  int ch = chan_ok ? g.chan : 0;
+  g.ic = P.chan_map[ch];
+  chan_ok = chan_ok && g.ic >= 0 && g.ic < P.n_img_chan;
+  double uu = P.uvw[g.row * 3 + 0];
+  double vv = P.uvw[g.row * 3 + 1];
+  const int sc = P.support / 2;
+  g.dvalid = degrid_geometry(uu, vv, P.freq[ch], P.dscale_u, P.dscale_v, P.doff_u, P.doff_v,
+                             P.n_u, P.n_v, P.support, P.ov, g.lu, g.lv, g.fu, g.fv) &&
+             chan_ok;
+  if (P.geometry == AV_GEOM_DEGRID) {
+    g.gvalid = g.dvalid;
+    g.iu = g.lu;
+    g.iv = g.lv;
+    g.ou = g.fu;
+    g.ov = g.fv;
+  } else {
+    g.gvalid = grid_geometry(uu, vv, P.gscale_u[ch], P.gscale_v[ch], P.n_u, P.n_v, sc, P.ov,
+                             g.iu, g.iv, g.ou, g.ov) &&
+               chan_ok;
+  }
+}
+
+__device__ inline int64_t bin_of(const PlanDev &P, int tu, int tv, int ic) {
+  return ((int64_t)tu * P.ntv + tv) * P.n_img_chan + ic;
+}
+
+// wave-aggregated counter increment; returns this lane's slot
+__device__ inline uint32_t wave_append(uint32_t *ctr, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0) return 0;
+  int lane = __lane_id();
+  int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+  return base + (uint32_t)__popcll(below);
+}
+
+__global__ void k_chan_setup(PlanDev P, const double *freq, const int32_t *chan_map) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.n_chan) return;
+  double f = freq[c];
+  P.freq[c] = f;
+  // numpy: -(freq_chan * delta_lm[0] * n_uv[0]) / c   (standard_grid.py:298-299)
+  P.gscale_u[c] = -(f * P.cell_u * (double)P.n_u) / kLightSpeed;
+  P.gscale_v[c] = -(f * P.cell_v * (double)P.n_v) / kLightSpeed;
+  int m;
+  if (chan_map)
+    m = chan_map[c];
+  else
+    m = (P.n_img_chan == P.n_chan) ? c : 0;
+  P.chan_map[c] = m;
+}
+
+// Records of one sample.  A grid-valid sample gets a primary record in the
+// tile of its grid centre (carrying the degrid geometry too when the two
+// centres are within one pixel, which is always the case for the reference
+// transforms) and a duplicate in every other tile its footprint overlaps.
+// A degrid-valid sample whose degrid centre is elsewhere (a caller-supplied
+// degrid transform) gets its own primary record in the tile of that centre.
+struct RecPlan {
+  bool gprim, dsep;        // grid-primary record, separate degrid record
+  bool share;              // grid-primary record also carries degrid geometry
+  int tu, tv, du_t, dv_t;  // grid anchor tile, degrid anchor tile
+  int tu0, tu1, tv0, tv1;  // tiles overlapped by the grid footprint
+};
+
+__device__ inline RecPlan record_plan(const PlanDev &P, const ItemGeo &g) {
+  RecPlan r;
+  const int sc = P.support / 2;
+  r.gprim = g.gvalid;
+  r.share = g.gvalid && g.dvalid && abs(g.lu - g.iu) <= 1 && abs(g.lv - g.iv) <= 1;
+  r.dsep = g.dvalid && !r.share;
+  r.tu = r.tv = r.du_t = r.dv_t = 0;
+  r.tu0 = r.tu1 = r.tv0 = r.tv1 = 0;
+  if (g.gvalid) {
+    r.tu = g.iu / P.tile_u;
+    r.tv = g.iv / P.tile_v;
+    r.tu0 = (g.iu - sc) / P.tile_u;
+    r.tu1 = (g.iu + P.support - sc - 1) / P.tile_u;
+    r.tv0 = (g.iv - sc) / P.tile_v;
+    r.tv1 = (g.iv + P.support - sc - 1) / P.tile_v;
+  }
+  if (r.dsep) {
+    r.du_t = g.lu / P.tile_u;
+    r.dv_t = g.lv / P.tile_v;
+  }
+  return r;
+}
+
+// Pass 1: count records per bin.
+__global__ __launch_bounds__(256) void k_count(PlanDev P) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t nitems = P.n_items;
+  // keep whole waves iterating together so the ballots see every lane
+  int64_t nround = ((nitems + stride - 1) / stride) * stride;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < nround; it += stride) {
+    bool live = it < nitems;
+    ItemGeo g;
+    g.gvalid = g.dvalid = false;
+    if (live) item_geometry(P, it, g);
+    wave_append(&P.counters[0], g.gvalid);
+    wave_append(&P.counters[1], g.dvalid);
+    wave_append(&P.counters[2], live && !g.dvalid);
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    if (rp.gprim) {
+      int64_t pb = bin_of(P, rp.tu, rp.tv, g.ic);
+      atomicAdd(&P.cnt_prim[pb], 1u);
+      atomicAdd(&P.cnt_all[pb], 1u);
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) atomicAdd(&P.cnt_all[bin_of(P, a, b, g.ic)], 1u);
+    }
+    if (rp.dsep) {
+      int64_t db = bin_of(P, rp.du_t, rp.dv_t, g.ic);
+      atomicAdd(&P.cnt_prim[db], 1u);
+      atomicAdd(&P.cnt_all[db], 1u);
+    }
+  }
+}
+
+__global__ void k_cursors(PlanDev P) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n_bins) return;
+  uint32_t s = P.bin_start[b];
+  P.cur_prim[b] = s;
+  P.cur_dup[b] = s + P.cnt_prim[b];
+}
+
+// Pass 2: write the records.
+__global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t nitems = P.n_items;
+  int64_t nround = ((nitems + stride - 1) / stride) * stride;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < nround; it += stride) {
+    bool live = it < nitems;
+    ItemGeo g;
+    g.gvalid = g.dvalid = false;
+    if (live) item_geometry(P, it, g);
+    uint32_t zslot = wave_append(&P.counters[5], live && !g.dvalid);
+    if (live && !g.dvalid) P.zero_idx[zslot] = (uint32_t)it;
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    if (rp.gprim) {
+      const int u0 = rp.tu * P.tile_u, v0 = rp.tv * P.tile_v;
+      const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.tu, rp.tv, g.ic)], 1u);
+      P.rec_idx[slot] = (uint32_t)it;
+      P.rec_g[slot] = pack_geo(g.iu - u0, g.ou, g.iv - v0, g.ov);
+      P.rec_d[slot] = rp.share ? pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv) : kInvalidGeo;
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) {
+            const uint32_t s2 = atomicAdd(&P.cur_dup[bin_of(P, a, b, g.ic)], 1u);
+            P.rec_idx[s2] = (uint32_t)it;
+            P.rec_g[s2] = pack_geo(g.iu - a * P.tile_u, g.ou, g.iv - b * P.tile_v, g.ov);
+            P.rec_d[s2] = kInvalidGeo;
+          }
+    }
+    if (rp.dsep) {
+      const int u0 = rp.du_t * P.tile_u, v0 = rp.dv_t * P.tile_v;
+      const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.du_t, rp.dv_t, g.ic)], 1u);
+      P.rec_idx[slot] = (uint32_t)it;
+      P.rec_g[slot] = kInvalidGeo;
+      P.rec_d[slot] = pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv);
+    }
+  }
+}
+
+// units per bin: grid units cover every bin (an empty bin still owns its
+// tile), degrid units only the primary records.
+__global__ void k_unit_counts(PlanDev P, uint32_t *nug, uint32_t *nud) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n_bins) return;
+  uint32_t cap = (uint32_t)P.unit_cap;
+  uint32_t na = P.cnt_all[b], np = P.cnt_prim[b];
+  nug[b] = na == 0 ? 1u : (na + cap - 1) / cap;
+  nud[b] = (np + cap - 1) / cap;
+}
+
+__global__ void k_units(PlanDev P) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n_bins) return;
+  uint32_t cap = (uint32_t)P.unit_cap;
+  uint32_t s = P.bin_start[b];
+  uint32_t na = P.cnt_all[b], np = P.cnt_prim[b];
+  uint32_t g0 = P.gunit_start[b], g1 = P.gunit_start[b + 1];
+  uint32_t ng = g1 - g0;
+  for (uint32_t k = 0; k < ng; ++k) {
+    uint32_t first = s + k * cap;
+    uint32_t cnt = na > k * cap ? min(cap, na - k * cap) : 0u;
+    uint32_t fl = (ng > 1 ? kUnitMulti : 0u) | (k == 0 ? kUnitFirst : 0u);
+    P.gunits[g0 + k] = make_uint4((uint32_t)b, first, cnt, fl);
+  }
+  uint32_t d0 = P.dunit_start[b], d1 = P.dunit_start[b + 1];
+  for (uint32_t k = 0; k < d1 - d0; ++k) {
+    uint32_t first = s + k * cap;
+    uint32_t cnt = min(cap, np - k * cap);
+    P.dunits[d0 + k] = make_uint4((uint32_t)b, first, cnt, 0u);
+  }
+}
+
+}  // namespace av
+
+using namespace av;
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" {
+
+int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u, int32_t n_v,
+                   double cell_u, double cell_v, int32_t support, int32_t oversampling,
+                   int32_t tile_u, int32_t tile_v, int32_t geometry, int32_t unit_cap,
+                   av_plan **out) {
+  if (!out) return fail(AV_ERR_INVALID, "av_plan_create: out is NULL");
+  *out = nullptr;
+  if (n_rows < 0 || n_chan < 1 || n_img_chan < 1 || n_u < 1 || n_v < 1)
+    return fail(AV_ERR_INVALID, "av_plan_create: sizes must be positive");
+  if (support < 2 || support > kMaxSupport)
+    return fail(AV_ERR_UNSUPPORTED, "av_plan_create: support must be in [2, 16]");
+  if (oversampling < 1 || oversampling > 250)
+    return fail(AV_ERR_UNSUPPORTED, "av_plan_create: oversampling must be in [1, 250]");
+  if (tile_u == 0) tile_u = 32;
+  if (tile_v == 0) tile_v = 32;
+  if (tile_u < support || tile_v < support || tile_u > 128 || tile_v > 128)
+    return fail(AV_ERR_INVALID, "av_plan_create: tiles must be in [support, 128]");
+  if (geometry != AV_GEOM_REFERENCE && geometry != AV_GEOM_DEGRID)
+    return fail(AV_ERR_INVALID, "av_plan_create: unknown geometry");
+  if (unit_cap == 0) unit_cap = 8192;
+  if (unit_cap < 64) return fail(AV_ERR_INVALID, "av_plan_create: unit_cap < 64");
+  int64_t n_items = n_rows * (int64_t)n_chan;
+  if (n_items > ((int64_t)1 << 32) / 5 - 1)
+    return fail(AV_ERR_UNSUPPORTED, "av_plan_create: more than 858993458 samples per plan");
+
+  av_plan *p = new av_plan();
+  PlanDev &d = p->d;
+  d.n_rows = n_rows;
+  d.n_chan = n_chan;
+  d.n_img_chan = n_img_chan;
+  d.n_u = n_u;
+  d.n_v = n_v;
+  d.cell_u = cell_u;
+  d.cell_v = cell_v;
+  // degrid_spheroid_ms4: scale = -pixelincr * n, offset = n / 2 (float)
+  d.dscale_u = (-cell_u) * (double)n_u;
+  d.dscale_v = (-cell_v) * (double)n_v;
+  d.doff_u = (double)n_u / 2.0;
+  d.doff_v = (double)n_v / 2.0;
+  d.support = support;
+  d.ov = oversampling;
+  d.tile_u = tile_u;
+  d.tile_v = tile_v;
+  d.geometry = geometry;
+  d.unit_cap = unit_cap;
+  d.ntu = (n_u + tile_u - 1) / tile_u;
+  d.ntv = (n_v + tile_v - 1) / tile_v;
+  d.n_bins = (int64_t)d.ntu * d.ntv * n_img_chan;
+  d.n_items = n_items;
+  // a footprint overlaps at most 2x2 tiles, plus one separate degrid record
+  d.max_records = 5 * n_items;
+  d.max_gunits = d.n_bins + d.max_records / unit_cap + 1;
+  // a sample has at most two primary records (grid + separate degrid)
+  d.max_dunits = d.n_bins + 2 * n_items / unit_cap + 1;
+  d.row_chan = nullptr;
+  d.uvw = nullptr;
+
+  const int64_t nb = d.n_bins;
+  int64_t scan_blocks = (std::max<int64_t>(nb, n_items) + 4095) / 4096 + 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += align_up(bytes);
+    return o;
+  };
+  size_t o_gsu = take(sizeof(double) * n_chan), o_gsv = take(sizeof(double) * n_chan),
+         o_freq = take(sizeof(double) * n_chan), o_cmap = take(sizeof(int32_t) * n_chan),
+         o_call = take(4 * nb), o_cprim = take(4 * nb), o_bstart = take(4 * (nb + 1)),
+         o_curp = take(4 * nb), o_curd = take(4 * nb), o_gus = take(4 * (nb + 1)),
+         o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
+         o_ridx = take(4 * d.max_records), o_rg = take(4 * d.max_records),
+         o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
+         o_dun = take(sizeof(uint4) * d.max_dunits), o_zero = take(4 * (n_items + 1)),
+         o_ctr = take(4 * 16);
+  void *base = nullptr;
+  hipError_t e = hipMalloc(&base, off);
+  if (e != hipSuccess) {
+    delete p;
+    return hip_fail(e, "av_plan_create: hipMalloc");
+  }
+  p->alloc_base = base;
+  p->alloc_bytes = off;
+  hipGetDevice(&p->device);
+  char *c = (char *)base;
+  d.gscale_u = (double *)(c + o_gsu);
+  d.gscale_v = (double *)(c + o_gsv);
+  d.freq = (double *)(c + o_freq);
+  d.chan_map = (int32_t *)(c + o_cmap);
+  d.cnt_all = (uint32_t *)(c + o_call);
+  d.cnt_prim = (uint32_t *)(c + o_cprim);
+  d.bin_start = (uint32_t *)(c + o_bstart);
+  d.cur_prim = (uint32_t *)(c + o_curp);
+  d.cur_dup = (uint32_t *)(c + o_curd);
+  d.gunit_start = (uint32_t *)(c + o_gus);
+  d.dunit_start = (uint32_t *)(c + o_dus);
+  d.scan_tmp = (uint32_t *)(c + o_scan);
+  d.rec_idx = (uint32_t *)(c + o_ridx);
+  d.rec_g = (uint32_t *)(c + o_rg);
+  d.rec_d = (uint32_t *)(c + o_rd);
+  d.gunits = (uint4 *)(c + o_gun);
+  d.dunits = (uint4 *)(c + o_dun);
+  d.zero_idx = (uint32_t *)(c + o_zero);
+  d.counters = (uint32_t *)(c + o_ctr);
+  *out = p;
+  return AV_OK;
+}
+
+int av_plan_destroy(av_plan *plan) {
+  if (!plan) return AV_OK;
+  if (plan->alloc_base) hipFree(plan->alloc_base);
+  delete plan;
+  return AV_OK;
+}
+
+int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
+                    const int32_t *chan_map, const int32_t *row_chan, void *stream) {
+  if (!plan || !freq || (plan->d.n_rows > 0 && !uvw))
+    return fail(AV_ERR_INVALID, "av_plan_set_uvw: NULL argument");
+  if (row_chan && plan->d.n_chan < 1)
+    return fail(AV_ERR_INVALID, "av_plan_set_uvw: row_chan needs n_chan >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  PlanDev &d = plan->d;
+  d.uvw = uvw;
+  d.row_chan = (int32_t *)row_chan;
+  if (row_chan) d.n_items = d.n_rows;  // one sample per row
+  else d.n_items = d.n_rows * (int64_t)d.n_chan;
+  hipLaunchKernelGGL(k_chan_setup, dim3((d.n_chan + 255) / 256), dim3(256), 0, s, d, freq,
+                     chan_map);
+  AV_LAUNCH_CHECK("k_chan_setup");
+  int rc;
+  if ((rc = launch_fill_u32(d.cnt_all, 0u, d.n_bins, s))) return rc;
+  if ((rc = launch_fill_u32(d.cnt_prim, 0u, d.n_bins, s))) return rc;
+  if ((rc = launch_fill_u32(d.counters, 0u, 16, s))) return rc;
+  int64_t blocks = (d.n_items + 255) / 256;
+  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
+  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(256), 0, s, d);
+  AV_LAUNCH_CHECK("k_count");
+  if ((rc = scan_exclusive_u32(d.cnt_all, d.bin_start, d.n_bins, d.scan_tmp, s))) return rc;
+  unsigned bb = (unsigned)((d.n_bins + 255) / 256);
+  hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
+  AV_LAUNCH_CHECK("k_cursors");
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, s, d);
+  AV_LAUNCH_CHECK("k_scatter");
+  // cursors are free again: reuse them for the per-bin unit counts
+  hipLaunchKernelGGL(k_unit_counts, dim3(bb), dim3(256), 0, s, d, d.cur_prim, d.cur_dup);
+  AV_LAUNCH_CHECK("k_unit_counts");
+  if ((rc = scan_exclusive_u32(d.cur_prim, d.gunit_start, d.n_bins, d.scan_tmp, s))) return rc;
+  if ((rc = scan_exclusive_u32(d.cur_dup, d.dunit_start, d.n_bins, d.scan_tmp, s))) return rc;
+  hipLaunchKernelGGL(k_units, dim3(bb), dim3(256), 0, s, d);
+  AV_LAUNCH_CHECK("k_units");
+  return AV_OK;
+}
+
+int av_plan_set_degrid_transform(av_plan *plan, double scale_u, double scale_v,
+                                 double offset_u, double offset_v) {
+  if (!plan) return fail(AV_ERR_INVALID, "av_plan_set_degrid_transform: NULL plan");
+  plan->d.dscale_u = scale_u;
+  plan->d.dscale_v = scale_v;
+  plan->d.doff_u = offset_u;
+  plan->d.doff_v = offset_v;
+  return AV_OK;
+}
+
+int av_plan_stats(av_plan *plan, int64_t out[8], void *stream) {
+  if (!plan || !out) return fail(AV_ERR_INVALID, "av_plan_stats: NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  const PlanDev &d = plan->d;
+  uint32_t ctr[16], tot[3];
+  AV_HIP(hipMemcpyAsync(ctr, d.counters, sizeof(ctr), hipMemcpyDeviceToHost, s));
+  AV_HIP(hipMemcpyAsync(&tot[0], d.bin_start + d.n_bins, 4, hipMemcpyDeviceToHost, s));
+  AV_HIP(hipMemcpyAsync(&tot[1], d.gunit_start + d.n_bins, 4, hipMemcpyDeviceToHost, s));
+  AV_HIP(hipMemcpyAsync(&tot[2], d.dunit_start + d.n_bins, 4, hipMemcpyDeviceToHost, s));
+  AV_HIP(hipStreamSynchronize(s));
+  out[0] = d.n_items;
+  out[1] = ctr[0];
+  out[2] = ctr[1];
+  out[3] = tot[0];
+  out[4] = d.n_bins;
+  out[5] = tot[1];
+  out[6] = tot[2];
+  out[7] = ctr[2];
+  return AV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// post.hip -- the passes around the FFT of grid2image_spheroid_ms4
+// (core/imaging/imaging_utils/standard_grid.py:693-708): ifftshift of the
+// grid before the inverse FFT, and fftshift + real part + taper correction
+// + N^2/sumwt normalisation after it, fused into one pass.
+#include "avgrid_internal.h"
+
+namespace av {
+
+template <typename R>
+__global__ void k_ifftshift(const R *in, R *out, int64_t n_planes, int n_u, int n_v) {
+  // np.fft.ifftshift: out[k] = in[(k + n//2) % n] on each of the two axes
+  const int64_t plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / plane;
+    const int64_t c = i - p * plane;
+    const int ku = (int)(c / n_v), kv = (int)(c - (int64_t)ku * n_v);
+    int su = ku + n_u / 2;
+    if (su >= n_u) su -= n_u;
+    int sv = kv + n_v / 2;
+    if (sv >= n_v) sv -= n_v;
+    const R *src = in + (p * plane + (int64_t)su * n_v + sv) * 2;
+    out[i * 2] = src[0];
+    out[i * 2 + 1] = src[1];
+  }
+}
+
+template <typename R, typename O>
+__global__ void k_normalise(const R *fft, const double *sumwt, const double *tu,
+                            const double *tv, int64_t n_planes, int n_u, int n_v, O *out) {
+  // np.fft.fftshift: out[k] = y[(k + n - n//2) % n]; then
+  // real / corrTerm * nx * ny / sumwt, left to right, in FP64.
+  const int64_t plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / plane;
+    const int64_t c = i - p * plane;
+    const int ku = (int)(c / n_v), kv = (int)(c - (int64_t)ku * n_v);
+    int su = ku + n_u - n_u / 2;
+    if (su >= n_u) su -= n_u;
+    int sv = kv + n_v - n_v / 2;
+    if (sv >= n_v) sv -= n_v;
+    const double re = (double)fft[(p * plane + (int64_t)su * n_v + sv) * 2];
+    const double corr = tu[ku] * tv[kv];
+    out[i] = (O)(re / corr * (double)n_u * (double)n_v / sumwt[p]);
+  }
+}
+
+static unsigned grid_blocks(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace av
+
+using namespace av;
+
+extern "C" {
+
+int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u, int32_t n_v,
+                 int32_t dtype, void *stream) {
+  if (!in || !out || n_planes < 0 || n_u < 1 || n_v < 1)
+    return fail(AV_ERR_INVALID, "av_ifftshift: bad arguments");
+  if (in == out) return fail(AV_ERR_INVALID, "av_ifftshift: in and out must differ");
+  const int64_t total = n_planes * n_u * n_v;
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == AV_F32)
+    hipLaunchKernelGGL(k_ifftshift<float>, dim3(grid_blocks(total)), dim3(256), 0, s,
+                       (const float *)in, (float *)out, n_planes, n_u, n_v);
+  else if (dtype == AV_F64)
+    hipLaunchKernelGGL(k_ifftshift<double>, dim3(grid_blocks(total)), dim3(256), 0, s,
+                       (const double *)in, (double *)out, n_planes, n_u, n_v);
+  else
+    return fail(AV_ERR_INVALID, "av_ifftshift: bad dtype");
+  AV_LAUNCH_CHECK("k_ifftshift");
+  return AV_OK;
+}
+
+int av_normalise(const void *fft_planes, const double *sumwt, const double *taper_u,
+                 const double *taper_v, int64_t n_planes, int32_t n_u, int32_t n_v,
+                 int32_t dtype, void *out, int32_t out_dtype, void *stream) {
+  if (!fft_planes || !sumwt || !taper_u || !taper_v || !out || n_planes < 0 || n_u < 1 || n_v < 1)
+    return fail(AV_ERR_INVALID, "av_normalise: bad arguments");
+  const int64_t total = n_planes * n_u * n_v;
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g(grid_blocks(total)), b(256);
+  if (dtype == AV_F32 && out_dtype == AV_F64)
+    hipLaunchKernelGGL((k_normalise<float, double>), g, b, 0, s, (const float *)fft_planes, sumwt,
+                       taper_u, taper_v, n_planes, n_u, n_v, (double *)out);
+  else if (dtype == AV_F32 && out_dtype == AV_F32)
+    hipLaunchKernelGGL((k_normalise<float, float>), g, b, 0, s, (const float *)fft_planes, sumwt,
+                       taper_u, taper_v, n_planes, n_u, n_v, (float *)out);
+  else if (dtype == AV_F64 && out_dtype == AV_F64)
+    hipLaunchKernelGGL((k_normalise<double, double>), g, b, 0, s, (const double *)fft_planes,
+                       sumwt, taper_u, taper_v, n_planes, n_u, n_v, (double *)out);
+  else if (dtype == AV_F64 && out_dtype == AV_F32)
+    hipLaunchKernelGGL((k_normalise<double, float>), g, b, 0, s, (const double *)fft_planes,
+                       sumwt, taper_u, taper_v, n_planes, n_u, n_v, (float *)out);
+  else
+    return fail(AV_ERR_INVALID, "av_normalise: bad dtype");
+  AV_LAUNCH_CHECK("k_normalise");
+  return AV_OK;
+}
+
+}  // extern "C"

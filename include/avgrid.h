@@ -1,0 +1,192 @@
+/*
+ * avgrid.h -- C ABI of libavgrid.so, the MI355X (gfx950) standard gridder.
+ *
+ * Every pointer argument is a DEVICE pointer unless stated otherwise; sizes
+ * are int64_t/int32_t; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every function returns AV_OK (0) or a negative status; the
+ * message of the last failure on the calling thread is av_last_error().
+ * No function allocates device memory except av_plan_create.
+ *
+ * Reference paths are relative to src/astroviper/ of casangi/astroviper
+ * 0.0.34; each entry point names the reference function it replaces.
+ */
+#ifndef AVGRID_H
+#define AVGRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AV_OK 0
+#define AV_ERR_INVALID (-1)     /* bad argument, shape or size */
+#define AV_ERR_HIP (-2)         /* HIP runtime error */
+#define AV_ERR_NOMEM (-3)       /* device allocation failed */
+#define AV_ERR_UNSUPPORTED (-4) /* valid but not implemented combination */
+
+/* element types */
+#define AV_F32 1 /* float32 weights / complex64 visibilities and grids */
+#define AV_F64 2 /* float64 weights / complex128 visibilities and grids */
+
+/* av_grid mode bits */
+#define AV_GRID_PSF 1         /* grid the weights instead of vis*weight */
+#define AV_GRID_ACCUMULATE 2  /* add into grid/sumwt instead of overwriting */
+#define AV_GRID_COMPENSATED 4 /* FP32 only: compensated per-tile sums */
+#define AV_GRID_ADJOINT 8     /* weight each sample by 1/norm (adjoint of
+                                 av_degrid); weights/flags are ignored */
+/* av_degrid mode bits */
+#define AV_DEGRID_INCREMENTAL 1 /* add into vis_out instead of overwriting */
+
+/* av_plan_create geometry */
+#define AV_GEOM_REFERENCE 0 /* grid uses standard_grid_jit rounding and
+                               degrid uses dgrid_numba rounding */
+#define AV_GEOM_DEGRID 1    /* both use dgrid_numba rounding: av_grid with
+                               AV_GRID_ADJOINT is then the exact adjoint of
+                               av_degrid */
+
+const char *av_last_error(void);
+int av_version(void);
+
+/* Device properties used for launch shapes (number of CUs, LDS per CU). */
+int av_device_info(int32_t *num_cus, int32_t *lds_bytes_per_cu,
+                   int32_t *xcds);
+
+/* prolate_spheroidal_function, core/imaging/imaging_utils/
+ * gcf_prolate_spheroidal.py:102-158: grdsf and (1-u^2)*grdsf at n points. */
+int av_ps_function(const double *u, int64_t n, double *grdsf, double *kern,
+                   void *stream);
+
+/* create_prolate_spheroidal_kernel_1D, gcf_prolate_spheroidal.py:87-99:
+ * writes oversampling*(support/2+1) float64 values. */
+int av_ps_kernel_1d(int32_t oversampling, int32_t support, double *out,
+                    void *stream);
+
+/* One axis of the gridding-correction taper kernel_image of
+ * create_prolate_spheroidal_kernel, gcf_prolate_spheroidal.py:72-79:
+ * out[i] = grdsf(|2*(i - n/2)/n|), i < n.  The taper is the outer product
+ * of the u and v axes. */
+int av_ps_correction_1d(int32_t n, double *out, void *stream);
+
+/* The full taper kernel_image of create_prolate_spheroidal_kernel,
+ * gcf_prolate_spheroidal.py:72-79: out[i][j] = grdsf(|2*(i - n_u/2)/n_u|) *
+ * grdsf(|2*(j - n_v/2)/n_v|), float64 [n_u][n_v]. */
+int av_ps_taper(int32_t n_u, int32_t n_v, double *out, void *stream);
+
+/* Outer products of create_prolate_spheroidal_kernel's 4-D kernel,
+ * gcf_prolate_spheroidal.py:65-67: k4[x][y][i][j] = k1[x][i]*k1[y][j] for
+ * x, y < nfill, zero for nfill <= x,y < nrow.  k1 is [nrow][support]. */
+int av_ps_outer(const double *k1, int32_t nrow, int32_t nfill,
+                int32_t support, double *k4, void *stream);
+
+/* ---------------------------------------------------------------- plans --
+ * A plan holds the binning of one set of uvw coordinates into
+ * (uv tile, image channel) bins.  It replaces the per-sample coordinate
+ * arithmetic of standard_grid_jit (standard_grid.py:318-352) and
+ * dgrid_numba (standard_degrid.py:435-462), computed once on the GPU in FP64
+ * and shared by av_grid and av_degrid.
+ *
+ * n_rows      rows (time x baseline) of uvw
+ * n_chan      visibility channels per row (1 when row_chan is used)
+ * n_img_chan  grid channels; n_u, n_v grid axes 2 and 3
+ * cell_u/v    pixelincr in radians (signed)
+ * tile_u/v    tile shape in pixels (0 = default 32)
+ * geometry    AV_GEOM_REFERENCE or AV_GEOM_DEGRID
+ * unit_cap    records per work unit (0 = default 8192)
+ * The plan is device-resident; create it once per (shape, tiling). */
+typedef struct av_plan av_plan;
+
+int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan,
+                   int32_t n_u, int32_t n_v, double cell_u, double cell_v,
+                   int32_t support, int32_t oversampling, int32_t tile_u,
+                   int32_t tile_v, int32_t geometry, int32_t unit_cap,
+                   av_plan **out);
+int av_plan_destroy(av_plan *plan);
+
+/* Bin the samples of uvw [n_rows][3] float64 at frequencies freq [n_chan]
+ * (or, with row_chan [n_rows] int32 non-NULL, row r has one sample at
+ * frequency freq[row_chan[r]] on image channel chan_map[row_chan[r]]).
+ * chan_map [n_chan] int32 maps visibility to image channel (NULL =
+ * identity when n_img_chan == n_chan, all zero when n_img_chan == 1).
+ * Asynchronous on `stream`; no host synchronisation. */
+int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
+                    const int32_t *chan_map, const int32_t *row_chan,
+                    void *stream);
+
+/* Replace the degridder's pixel transform pos = scale*uvw*freq/c + offset
+ * (standard_degrid.py:435-440).  The default, set by av_plan_create, is
+ * scale = -cell*N and offset = N/2 per axis as degrid_spheroid_ms4 passes
+ * (standard_degrid.py:572-573); dgrid_numba itself accepts any values.
+ * Takes effect at the next av_plan_set_uvw. */
+int av_plan_set_degrid_transform(av_plan *plan, double scale_u, double scale_v,
+                                 double offset_u, double offset_v);
+
+/* Counts of the last av_plan_set_uvw (synchronises `stream`):
+ * out[0] samples, out[1] grid-valid, out[2] degrid-valid, out[3] records,
+ * out[4] bins, out[5] grid units, out[6] degrid units, out[7] zero-list. */
+int av_plan_stats(av_plan *plan, int64_t out[8], void *stream);
+
+/* standard_grid_jit, core/imaging/imaging_utils/standard_grid.py:246-432
+ * (with the flag masking of grid2image_spheroid_ms4, :664-668).
+ * vis [rows][n_chan][n_pol] complex (ignored with AV_GRID_PSF),
+ * weight [rows][n_chan][n_pol] real, flag [rows][n_chan][n_pol] uint8 or
+ * NULL (a flagged sample counts as weight 0), cgk the float64 kernel table
+ * of av_ps_kernel_1d, grid [n_img_chan][n_pol][n_u][n_v] complex,
+ * sumwt [n_img_chan][n_pol] float64.  dtype AV_F32 or AV_F64 selects
+ * complex64/float32 or complex128/float64 for vis, weight and grid. */
+int av_grid(av_plan *plan, const void *vis, const void *weight,
+            const uint8_t *flag, int32_t n_pol, const double *cgk,
+            void *grid, double *sumwt, int32_t dtype, int32_t mode,
+            void *stream);
+
+/* dgrid_numba, core/imaging/imaging_utils/standard_degrid.py:388-485 with
+ * zero dphase.  grid [n_img_chan][n_grid_pol][n_u][n_v] complex,
+ * pol_map [n_pol] int32 grid polarisation of each visibility polarisation
+ * (NULL = p % n_grid_pol), flag [rows][n_chan][n_pol] uint8 or NULL,
+ * vis_out [rows][n_chan][n_pol] complex.  Flagged or off-grid samples are
+ * left unchanged with AV_DEGRID_INCREMENTAL and set to 0 without it. */
+int av_degrid(av_plan *plan, const void *grid, int32_t n_grid_pol,
+              const int32_t *pol_map, const uint8_t *flag, int32_t n_pol,
+              const double *cgk, void *vis_out, int32_t dtype, int32_t mode,
+              void *stream);
+
+/* ------------------------------------------------- plain GPU baseline --
+ * One thread per sample of ONE visibility channel, global atomics, no
+ * binning: the straightforward GPU port of standard_grid_jit /
+ * dgrid_numba used as the comparison baseline and as a second GPU check.
+ * Arguments as av_grid/av_degrid plus the channel and its frequency. */
+int av_grid_plain(const double *uvw, int64_t n_rows, int32_t n_chan,
+                  int32_t chan, double freq, int32_t img_chan,
+                  const void *vis, const void *weight, const uint8_t *flag,
+                  int32_t n_pol, const double *cgk, int32_t support,
+                  int32_t oversampling, double cell_u, double cell_v,
+                  int32_t n_u, int32_t n_v, void *grid, double *sumwt,
+                  int32_t dtype, int32_t mode, void *stream);
+int av_degrid_plain(const double *uvw, int64_t n_rows, int32_t n_chan,
+                    int32_t chan, double freq, int32_t img_chan,
+                    const void *grid, int32_t n_grid_pol,
+                    const uint8_t *flag, int32_t n_pol, const double *cgk,
+                    int32_t support, int32_t oversampling, double cell_u,
+                    double cell_v, int32_t n_u, int32_t n_v, void *vis_out,
+                    int32_t dtype, int32_t mode, void *stream);
+
+/* ---------------------------------------------- FFT pre/post passes --
+ * The FFT itself is a library call (rocFFT through the host runtime).
+ * av_ifftshift: out = ifftshift(in) over the last two axes of
+ * n_planes complex planes [n_u][n_v] (np.fft.ifftshift, standard_grid.py:701).
+ * av_normalise: the post-FFT step of grid2image_spheroid_ms4,
+ * standard_grid.py:696-708: out[p] = real(fftshift(Y[p])) / taper * n_u *
+ * n_v / sumwt[p], with taper the outer product of av_ps_correction_1d on
+ * both axes; out is float64 (out_dtype AV_F64) or float32 (AV_F32). */
+int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u,
+                 int32_t n_v, int32_t dtype, void *stream);
+int av_normalise(const void *fft_planes, const double *sumwt,
+                 const double *taper_u, const double *taper_v,
+                 int64_t n_planes, int32_t n_u, int32_t n_v, int32_t dtype,
+                 void *out, int32_t out_dtype, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVGRID_H */

@@ -1,0 +1,361 @@
+/*
+ * avgrid_oracle.c -- FP64 CPU restatement of the reference standard gridder.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is linked by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * nowhere else.  The product (astroviper_amd/) never calls it.
+ *
+ * Every function restates one reference function loop by loop, in the same
+ * order and with the same rounding rules, in plain C99 compiled with
+ * -O3 -ffp-contract=off (no fused multiply-add, no fast-math), so that the
+ * FP64 arithmetic is the same IEEE sequence the reference's numba/NumPy code
+ * performs.  Reference paths are relative to src/astroviper/.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Schwab rational-approximation coefficients, M=6, alpha=1.
+ * core/imaging/imaging_utils/gcf_prolate_spheroidal.py:116-127 */
+static const double PS_P[2][5] = {
+    {8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1},
+    {4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2}};
+static const double PS_Q[2][3] = {{1.0000000e0, 8.212018e-1, 2.078043e-1},
+                                  {1.0000000e0, 9.599102e-1, 2.918724e-1}};
+
+/* prolate_spheroidal_function, gcf_prolate_spheroidal.py:102-158.
+ * For each u: part/uend chosen by |u| (<0.75 -> 0/0.75, [0.75,1] -> 1/1.0,
+ * otherwise the zero-initialised defaults part 0, uend 0), then
+ * top = sum_k p[part,k] * pow(d,k), bot = sum_k q[part,k] * pow(d,k) with
+ * d = u*u - uend*uend; grdsf = top/bot where bot > 0, forced to 0 for |u|>1.
+ * np.power(float64, int) goes through libm pow(), as here.  Returns grdsf
+ * and (1 - u*u) * grdsf. */
+void ora_ps_function(const double *u_in, int64_t n, double *grdsf_out,
+                     double *kern_out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double u = fabs(u_in[i]);
+    int part = 0;
+    double uend = 0.0;
+    if (u >= 0.0 && u < 0.75) {
+      part = 0;
+      uend = 0.75;
+    }
+    if (u >= 0.75 && u <= 1.0) {
+      part = 1;
+      uend = 1.0;
+    }
+    double delusq = u * u - uend * uend;
+    double top = PS_P[part][0];
+    for (int k = 1; k < 5; ++k) top += PS_P[part][k] * pow(delusq, (double)k);
+    double bot = PS_Q[part][0];
+    for (int k = 1; k < 3; ++k) bot += PS_Q[part][k] * pow(delusq, (double)k);
+    double g = 0.0;
+    if (bot > 0.0) g = top / bot;
+    if (u > 1.0) g = 0.0;
+    if (grdsf_out) grdsf_out[i] = g;
+    if (kern_out) kern_out[i] = (1.0 - u * u) * g;
+  }
+}
+
+/* create_prolate_spheroidal_kernel_1D, gcf_prolate_spheroidal.py:87-99.
+ * out has ov*(sc+1) entries; entry i < ov*sc is the gridding kernel at
+ * u = i / (sc*ov), the last ov entries are zero. */
+void ora_ps_kernel_1d(int ov, int support, double *out) {
+  int sc = support / 2;
+  int n = ov * sc;
+  int ntot = ov * (sc + 1);
+  for (int i = 0; i < ntot; ++i) out[i] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double u = (double)i / (double)(sc * ov);
+    ora_ps_function(&u, 1, NULL, &out[i]);
+  }
+}
+
+/* Gridding-correction taper kernel_image of create_prolate_spheroidal_kernel,
+ * gcf_prolate_spheroidal.py:72-79 with coordinates() at :161-163:
+ * taper[i,j] = grdsf(|2*(i - nu//2)/nu|) * grdsf(|2*(j - nv//2)/nv|). */
+void ora_ps_taper(int nu, int nv, double *out) {
+  for (int i = 0; i < nu; ++i) {
+    double cu = (double)(i - nu / 2) / (double)nu;
+    double au = fabs(2.0 * cu), gu;
+    ora_ps_function(&au, 1, &gu, NULL);
+    for (int j = 0; j < nv; ++j) {
+      double cv = (double)(j - nv / 2) / (double)nv;
+      double av = fabs(2.0 * cv), gv;
+      ora_ps_function(&av, 1, &gv, NULL);
+      out[(int64_t)i * nv + j] = gu * gv;
+    }
+  }
+}
+
+/* standard_grid_jit, core/imaging/imaging_utils/standard_grid.py:246-432.
+ *
+ * grid      complex128 (interleaved re,im) [n_img_chan][n_img_pol][n_u][n_v],
+ *           or float64 when complex_grid == 0 (do_psf only), accumulated.
+ * sum_weight float64 [n_img_chan][n_img_pol], accumulated.
+ * vis       complex128 [nt][nb][nchan][nvispol] (ignored when do_psf).
+ * uvw       float64 [nt][nb][3];  weight float64 [nt][nb][nchan][npol].
+ * chan_map  [nchan] image channel, pol_map [npol] image polarisation.
+ * npol = len(pol_map) is the loop count over polarisations (:313).
+ *
+ * Loop order time -> baseline -> channel -> polarisation -> v-tap -> u-tap
+ * (:318-404).  Centre index int(x + 0.5) truncates toward zero (:337-341),
+ * offset index floor((ic - pos) * ov + 0.5) (:349-352), kernel index
+ * |ov * tap + offset| (:384-396).  A sample is skipped if its weighted datum
+ * is NaN or exactly zero (:378).  sum_weight gains weight * norm (:423-430).
+ * do_imaging_weight additionally scatters to the conjugate position and
+ * doubles the sum_weight contribution (:389-430); the conjugate taps are
+ * bounds-checked here (the reference does not check them). */
+void ora_standard_grid(double *grid, int complex_grid, double *sum_weight,
+                       int do_psf, int do_imaging_weight, const double *vis,
+                       int nvispol, const double *uvw, int nt, int nb,
+                       const double *freq_chan, int nchan,
+                       const int64_t *chan_map, const int64_t *pol_map,
+                       int npol, const double *weight, int nwpol,
+                       const double *cgk_1D, int n_img_chan, int n_img_pol,
+                       int n_u, int n_v, double dl_u, double dl_v, int support,
+                       int oversampling) {
+  const double c = 299792458.0;
+  const int support_center = support / 2;
+  const int64_t uv_center_u = n_u / 2, uv_center_v = n_v / 2;
+  const int start_support = -support_center;
+  const int end_support = support - support_center;
+  double *uv_scale_u = (double *)malloc(sizeof(double) * (size_t)(nchan + 1));
+  double *uv_scale_v = (double *)malloc(sizeof(double) * (size_t)(nchan + 1));
+  (void)n_img_chan;
+  (void)n_img_pol;
+  for (int ch = 0; ch < nchan; ++ch) {
+    /* numpy: -(freq_chan * delta_lm[0] * n_uv[0]) / c  (:298-299) */
+    uv_scale_u[ch] = -(freq_chan[ch] * dl_u * (double)n_u) / c;
+    uv_scale_v[ch] = -(freq_chan[ch] * dl_v * (double)n_v) / c;
+  }
+  const int64_t plane = (int64_t)n_u * n_v;
+  for (int t = 0; t < nt; ++t) {
+    for (int b = 0; b < nb; ++b) {
+      const int64_t row = (int64_t)t * nb + b;
+      for (int ch = 0; ch < nchan; ++ch) {
+        int64_t a_chan = chan_map[ch];
+        double u = uvw[row * 3 + 0] * uv_scale_u[ch];
+        double v = uvw[row * 3 + 1] * uv_scale_v[ch];
+        if (isnan(u) || isnan(v)) continue;
+        double u_pos = u + (double)uv_center_u;
+        double v_pos = v + (double)uv_center_v;
+        double u_pos_conj = -u + (double)uv_center_u;
+        double v_pos_conj = -v + (double)uv_center_v;
+        /* int(x + 0.5): truncation toward zero.  Values outside the int64
+         * range cannot pass the bounds test below, so they are rejected
+         * before the cast. */
+        double xu = u_pos + 0.5, xv = v_pos + 0.5;
+        if (!(xu > -9.0e18 && xu < 9.0e18 && xv > -9.0e18 && xv < 9.0e18))
+          continue;
+        int64_t u_ci = (int64_t)xu;
+        int64_t v_ci = (int64_t)xv;
+        int64_t u_ci_conj = (int64_t)(u_pos_conj + 0.5);
+        int64_t v_ci_conj = (int64_t)(v_pos_conj + 0.5);
+        if (!((u_ci + support_center < n_u) && (v_ci + support_center < n_v) &&
+              (u_ci - support_center >= 0) && (v_ci - support_center >= 0)))
+          continue;
+        double u_offset = (double)u_ci - u_pos;
+        int64_t u_coi = (int64_t)floor(u_offset * (double)oversampling + 0.5);
+        double v_offset = (double)v_ci - v_pos;
+        int64_t v_coi = (int64_t)floor(v_offset * (double)oversampling + 0.5);
+        for (int ip = 0; ip < npol; ++ip) {
+          const int64_t widx = ((row * nchan) + ch) * nwpol;
+          double wd_re, wd_im, sel_weight;
+          if (do_psf) {
+            if (npol >= 2 && do_imaging_weight) {
+              wd_re = (weight[widx + 0] + weight[widx + 1]) / 2.0;
+              sel_weight = wd_re;
+            } else {
+              sel_weight = weight[widx + ip];
+              wd_re = weight[widx + ip];
+            }
+            wd_im = 0.0;
+          } else {
+            sel_weight = weight[widx + ip];
+            const int64_t vidx = (((row * nchan) + ch) * nvispol + ip) * 2;
+            wd_re = vis[vidx] * weight[widx + ip];
+            wd_im = vis[vidx + 1] * weight[widx + ip];
+          }
+          if (isnan(wd_re) || isnan(wd_im)) continue;
+          if (wd_re == 0.0 && wd_im == 0.0) continue;
+          int64_t a_pol = pol_map[ip];
+          double *g = grid + ((a_chan * n_img_pol) + a_pol) * plane *
+                                 (complex_grid ? 2 : 1);
+          double norm = 0.0;
+          for (int i_v = start_support; i_v < end_support; ++i_v) {
+            int64_t v_indx = v_ci + i_v;
+            int64_t v_oi = (int64_t)oversampling * i_v + v_coi;
+            if (v_oi < 0) v_oi = -v_oi;
+            double conv_v = cgk_1D[v_oi];
+            int64_t v_indx_conj = v_ci_conj + i_v;
+            for (int i_u = start_support; i_u < end_support; ++i_u) {
+              int64_t u_indx = u_ci + i_u;
+              int64_t u_oi = (int64_t)oversampling * i_u + u_coi;
+              if (u_oi < 0) u_oi = -u_oi;
+              double conv_u = cgk_1D[u_oi];
+              double conv = conv_u * conv_v;
+              int64_t cell = u_indx * n_v + v_indx;
+              if (complex_grid) {
+                g[2 * cell] = g[2 * cell] + conv * wd_re;
+                g[2 * cell + 1] = g[2 * cell + 1] + conv * wd_im;
+              } else {
+                g[cell] = g[cell] + conv * wd_re;
+              }
+              norm = norm + conv;
+              if (do_imaging_weight) {
+                int64_t u_indx_conj = u_ci_conj + i_u;
+                if (u_indx_conj >= 0 && u_indx_conj < n_u && v_indx_conj >= 0 &&
+                    v_indx_conj < n_v) {
+                  int64_t cc = u_indx_conj * n_v + v_indx_conj;
+                  if (complex_grid) {
+                    g[2 * cc] = g[2 * cc] + conv * wd_re;
+                    g[2 * cc + 1] = g[2 * cc + 1] + conv * wd_im;
+                  } else {
+                    g[cc] = g[cc] + conv * wd_re;
+                  }
+                }
+              }
+            }
+          }
+          double *sw = sum_weight + a_chan * n_img_pol + a_pol;
+          *sw = *sw + sel_weight * norm;
+          if (do_imaging_weight) *sw = *sw + sel_weight * norm;
+        }
+      }
+    }
+  }
+  free(uv_scale_u);
+  free(uv_scale_v);
+}
+
+/* np.round(x): round half to even (matches the default rounding mode). */
+static double round_half_even(double x) { return nearbyint(x); }
+
+/* dgrid_numba, core/imaging/imaging_utils/standard_degrid.py:388-485, with
+ * dphase = 0 so the phasor is exactly 1 (:538, :452-453).
+ *
+ * values complex128 [nt][nb][nvischan][nvispol], accumulated (+=).
+ * flag   uint8 [nt][nb][nvischan][nvispol].
+ * grid   complex128 [nchan_g][npol_g][g2][g3]; the reference reads
+ *        nchan, npol, ny, nx = grid.shape (:407), so the u bound uses g3 and
+ *        the v bound uses g2 (:457-462); both are also checked against the
+ *        axis they index, which only differs for non-square grids where the
+ *        reference itself would read out of bounds.
+ * Loop order time -> baseline -> pol -> chan (:416-423), taps ix outer, iy
+ * inner (:465-481); pos = scale*uvw*freq/c + offset evaluated left to right
+ * (:435-440); loc = round_half_even(pos) (:442-443);
+ * off = round_half_even((loc - pos) * sampling) (:445-450); the sum is divided
+ * by the kernel-weight sum norm (:482-485). */
+void ora_dgrid(const double *uvw, double *values, const uint8_t *flag,
+               int nt, int nb, int nvischan, int nvispol, const double *scale,
+               const double *offset, const double *grid, int nchan_g,
+               int npol_g, int g2, int g3, const double *freq, double c,
+               int support, int sampling, const double *convFunc,
+               const int64_t *chanmap, const int64_t *polmap) {
+  const int supp_centre = support / 2;
+  const int supp_beg = -supp_centre;
+  const int supp_end = supp_beg + support;
+  const int64_t ny = g2, nx = g3;
+  for (int t = 0; t < nt; ++t) {
+    for (int b = 0; b < nb; ++b) {
+      const int64_t row = (int64_t)t * nb + b;
+      for (int ipol = 0; ipol < nvispol; ++ipol) {
+        int64_t apol = polmap[ipol];
+        if (!(0 <= apol && apol < npol_g)) continue;
+        for (int ichan = 0; ichan < nvischan; ++ichan) {
+          int64_t achan = chanmap[ichan];
+          const int64_t vi = ((row * nvischan) + ichan) * nvispol + ipol;
+          if (!(0 <= achan && achan < nchan_g && !flag[vi])) continue;
+          double pos0 = scale[0] * uvw[row * 3 + 0] * freq[ichan] / c + offset[0];
+          double pos1 = scale[1] * uvw[row * 3 + 1] * freq[ichan] / c + offset[1];
+          double r0 = round_half_even(pos0), r1 = round_half_even(pos1);
+          if (!(r0 > -2.0e9 && r0 < 2.0e9 && r1 > -2.0e9 && r1 < 2.0e9)) continue;
+          int32_t loc0 = (int32_t)r0, loc1 = (int32_t)r1;
+          int32_t off0 = (int32_t)round_half_even(((double)loc0 - pos0) * (double)sampling);
+          int32_t off1 = (int32_t)round_half_even(((double)loc1 - pos1) * (double)sampling);
+          int64_t x0 = loc0, y0 = loc1;
+          if (!((x0 + supp_beg >= 0) && ((x0 + supp_end - 1) < nx) &&
+                (y0 + supp_beg >= 0) && ((y0 + supp_end - 1) < ny)))
+            continue;
+          if (!((x0 + supp_end - 1) < g2 && (y0 + supp_end - 1) < g3)) continue;
+          const double *g = grid + ((achan * npol_g) + apol) * (int64_t)g2 * g3 * 2;
+          double nv_re = 0.0, nv_im = 0.0, norm = 0.0;
+          for (int ix = supp_beg; ix < supp_end; ++ix) {
+            int64_t il1 = (int64_t)sampling * ix + off0;
+            if (il1 < 0) il1 = -il1;
+            double wtx = convFunc[il1];
+            for (int iy = supp_beg; iy < supp_end; ++iy) {
+              int64_t il2 = (int64_t)sampling * iy + off1;
+              if (il2 < 0) il2 = -il2;
+              double wty = convFunc[il2];
+              double wt = wtx * wty;
+              norm += wt;
+              int64_t cell = (x0 + ix) * g3 + (y0 + iy);
+              nv_re += wt * g[2 * cell];
+              nv_im += wt * g[2 * cell + 1];
+            }
+          }
+          if (norm != 0.0) {
+            values[2 * vi] += nv_re / norm;
+            values[2 * vi + 1] += nv_im / norm;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* Post-FFT normalisation of grid2image_spheroid_ms4,
+ * core/imaging/imaging_utils/standard_grid.py:696-708:
+ * out = real_part / corrTerm * nx * ny / sumwt, evaluated left to right. */
+void ora_normalise(const double *real_part, const double *corr, int64_t nplane,
+                   int nx, int ny, const double *sumwt, double *out) {
+  const int64_t np = (int64_t)nx * ny;
+  for (int64_t p = 0; p < nplane; ++p)
+    for (int64_t k = 0; k < np; ++k)
+      out[p * np + k] =
+          real_part[p * np + k] / corr[k] * (double)nx * (double)ny / sumwt[p];
+}
+
+/* Per-sample geometry of standard_grid_jit (standard_grid.py:296-352):
+ * out[0..3] = u centre, v centre, u offset index, v offset index; returns 1
+ * if the sample passes the bounds test, 0 otherwise (out untouched). */
+int ora_grid_geometry(double uvw_u, double uvw_v, double freq, double dl_u, double dl_v,
+                      int n_u, int n_v, int support, int oversampling, int64_t out[4]) {
+  const double c = 299792458.0;
+  const int sc = support / 2;
+  double su = -(freq * dl_u * (double)n_u) / c;
+  double sv = -(freq * dl_v * (double)n_v) / c;
+  double u = uvw_u * su, v = uvw_v * sv;
+  if (isnan(u) || isnan(v)) return 0;
+  double u_pos = u + (double)(n_u / 2), v_pos = v + (double)(n_v / 2);
+  double xu = u_pos + 0.5, xv = v_pos + 0.5;
+  if (!(xu > -9.0e18 && xu < 9.0e18 && xv > -9.0e18 && xv < 9.0e18)) return 0;
+  int64_t cu = (int64_t)xu, cv = (int64_t)xv;
+  if (!((cu + sc < n_u) && (cv + sc < n_v) && (cu - sc >= 0) && (cv - sc >= 0))) return 0;
+  out[0] = cu;
+  out[1] = cv;
+  out[2] = (int64_t)floor(((double)cu - u_pos) * (double)oversampling + 0.5);
+  out[3] = (int64_t)floor(((double)cv - v_pos) * (double)oversampling + 0.5);
+  return 1;
+}
+
+/* Per-sample geometry of dgrid_numba / sgrid_numba
+ * (standard_degrid.py:367-385, 435-450): pos, loc (round half even) and
+ * off.  pos_out[2], loc_off[4] = loc0, loc1, off0, off1. */
+void ora_degrid_geometry(const double uvw[3], double freq, const double scale[2],
+                         const double offset[2], int sampling, double pos_out[2],
+                         int64_t loc_off[4]) {
+  const double c = 299792458.0;
+  double p0 = scale[0] * uvw[0] * freq / c + offset[0];
+  double p1 = scale[1] * uvw[1] * freq / c + offset[1];
+  pos_out[0] = p0;
+  pos_out[1] = p1;
+  int32_t l0 = (int32_t)round_half_even(p0), l1 = (int32_t)round_half_even(p1);
+  loc_off[0] = l0;
+  loc_off[1] = l1;
+  loc_off[2] = (int32_t)round_half_even(((double)l0 - p0) * (double)sampling);
+  loc_off[3] = (int32_t)round_half_even(((double)l1 - p1) * (double)sampling);
+}

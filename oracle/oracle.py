@@ -1,0 +1,280 @@
+"""FP64 CPU oracle for the standard gridder -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, by __graft_entry__.smoke() and by bench.py's
+cpu_baseline leg, always as the checker and never as the thing measured or
+shipped.  The product (astroviper_amd/) never imports this module.
+
+The arithmetic lives in avgrid_oracle.c (a loop-by-loop C restatement of the
+reference's numba kernels); this module adds the NumPy parts the reference
+itself runs in NumPy (FFT + normalisation, argument preparation).  Reference
+paths below are relative to src/astroviper/.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+C_LIGHT = 299792458.0
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with oracle/Makefile (gcc, no FMA contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        I, I64, D = ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.ora_ps_function.argtypes = [dp, I64, dp, dp]
+        L.ora_ps_kernel_1d.argtypes = [I, I, dp]
+        L.ora_ps_taper.argtypes = [I, I, dp]
+        L.ora_standard_grid.argtypes = [
+            dp, I, dp, I, I, dp, I, dp, I, I, dp, I, i64p, i64p, I, dp, I, dp,
+            I, I, I, I, D, D, I, I]
+        L.ora_dgrid.argtypes = [dp, dp, u8p, I, I, I, I, dp, dp, dp, I, I, I,
+                                I, dp, D, I, I, dp, i64p, i64p]
+        L.ora_normalise.argtypes = [dp, dp, I64, I, I, dp, dp]
+        L.ora_grid_geometry.argtypes = [D, D, D, D, D, I, I, I, I, i64p]
+        L.ora_grid_geometry.restype = ctypes.c_int
+        L.ora_degrid_geometry.argtypes = [dp, D, dp, dp, I, dp, i64p]
+        L.ora_degrid_geometry.restype = None
+        for f in (L.ora_ps_function, L.ora_ps_kernel_1d, L.ora_ps_taper,
+                  L.ora_standard_grid, L.ora_dgrid, L.ora_normalise):
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+# --------------------------------------------------------------------------
+# Kernel table and taper  (core/imaging/imaging_utils/gcf_prolate_spheroidal.py)
+# --------------------------------------------------------------------------
+def prolate_spheroidal_function(u):
+    """gcf_prolate_spheroidal.py:102-158 -> (grdsf, (1-u^2)*grdsf)."""
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    shp = u.shape
+    uf = u.reshape(-1).copy()
+    g = np.zeros_like(uf)
+    k = np.zeros_like(uf)
+    lib().ora_ps_function(_dp(uf), uf.size, _dp(g), _dp(k))
+    return g.reshape(shp), k.reshape(shp)
+
+
+def create_prolate_spheroidal_kernel_1D(oversampling, support):
+    """gcf_prolate_spheroidal.py:87-99."""
+    out = np.zeros(oversampling * (support // 2 + 1), dtype=np.float64)
+    lib().ora_ps_kernel_1d(int(oversampling), int(support), _dp(out))
+    return out
+
+
+def coordinates(npixel):
+    """gcf_prolate_spheroidal.py:161-163."""
+    return (np.arange(npixel) - npixel // 2) / npixel
+
+
+def create_prolate_spheroidal_kernel(oversampling, support, n_uv):
+    """gcf_prolate_spheroidal.py:8-84: (4-D kernel, taper kernel_image)."""
+    sc = support // 2
+    oc = oversampling // 2
+    sv = np.arange(support) - sc
+    nrow = oversampling + 1 if oversampling % 2 == 0 else oversampling
+    ovv = ((np.arange(nrow) - oc) / oversampling)[:, None]
+    pts = (np.broadcast_to(sv, (nrow, support)) + ovv) / sc
+    _, k1 = prolate_spheroidal_function(pts)
+    kernel = np.zeros((nrow, nrow, support, support), dtype=np.double)
+    for x in range(oversampling):
+        for y in range(oversampling):
+            kernel[x, y, :, :] = np.outer(k1[x, :], k1[y, :])
+    taper = np.zeros((int(n_uv[0]), int(n_uv[1])), dtype=np.float64)
+    lib().ora_ps_taper(int(n_uv[0]), int(n_uv[1]), _dp(taper))
+    return kernel, taper
+
+
+def grid_geometry(uvw_u, uvw_v, freq, cell, n_uv, support=7, oversampling=100):
+    """standard_grid.py:296-352 for one sample -> (iu, iv, ou, ov) or None."""
+    out = np.zeros(4, dtype=np.int64)
+    ok = lib().ora_grid_geometry(float(uvw_u), float(uvw_v), float(freq), float(cell[0]),
+                                 float(cell[1]), int(n_uv[0]), int(n_uv[1]), int(support),
+                                 int(oversampling), _i64p(out))
+    return tuple(int(x) for x in out) if ok else None
+
+
+def degrid_geometry(uvw, freq, scale, offset, sampling):
+    """sgrid_numba (standard_degrid.py:367-385) -> (pos[2], loc[2], off[2])."""
+    u = np.ascontiguousarray(np.asarray(uvw, dtype=np.float64)[:3])
+    sc = np.ascontiguousarray(np.asarray(scale, dtype=np.float64)[:2])
+    of = np.ascontiguousarray(np.asarray(offset, dtype=np.float64)[:2])
+    pos = np.zeros(2)
+    lo = np.zeros(4, dtype=np.int64)
+    lib().ora_degrid_geometry(_dp(u), float(freq), _dp(sc), _dp(of), int(sampling), _dp(pos),
+                              _i64p(lo))
+    return pos, lo[:2].copy(), lo[2:].copy()
+
+
+# --------------------------------------------------------------------------
+# Gridder  (core/imaging/imaging_utils/standard_grid.py)
+# --------------------------------------------------------------------------
+def standard_grid(vis, uvw, weight, freq_chan, n_uv, cell, support=7,
+                  oversampling=100, chan_mode="continuum", do_psf=False,
+                  grid=None, sum_weight=None, cgk_1D=None, complex_grid=True,
+                  do_imaging_weight=False):
+    """standard_grid_numpy_wrap (standard_grid.py:88-165) around the C
+    restatement of standard_grid_jit (standard_grid.py:246-432).
+
+    vis (nt, nb, nchan, nvispol) complex, uvw (nt, nb, 3), weight
+    (nt, nb, nchan, npol).  Returns (grid, sum_weight), FP64.
+    """
+    weight = np.ascontiguousarray(weight, dtype=np.float64)
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    freq_chan = np.ascontiguousarray(freq_chan, dtype=np.float64)
+    nt, nb, nchan, npol = weight.shape
+    if chan_mode == "cube":
+        n_img_chan = nchan
+        chan_map = np.arange(nchan, dtype=np.int64)
+    else:
+        n_img_chan = 1
+        chan_map = np.zeros(nchan, dtype=np.int64)
+    pol_map = np.arange(npol, dtype=np.int64)
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    if grid is None or (grid.size == 0 and sum_weight.size == 0):
+        grid = np.zeros((n_img_chan, npol, n_u, n_v),
+                        dtype=np.complex128 if complex_grid else np.float64)
+        sum_weight = np.zeros((n_img_chan, npol), dtype=np.float64)
+    if cgk_1D is None:
+        cgk_1D = create_prolate_spheroidal_kernel_1D(oversampling, support)
+    cgk_1D = np.ascontiguousarray(cgk_1D, dtype=np.float64)
+    if do_psf:
+        vis_c = np.zeros((1, 1, 1, 1), dtype=np.complex128)
+        nvispol = 1
+    else:
+        vis_c = np.ascontiguousarray(vis, dtype=np.complex128)
+        nvispol = vis_c.shape[3]
+    assert grid.flags.c_contiguous and sum_weight.flags.c_contiguous
+    lib().ora_standard_grid(
+        _dp(grid.view(np.float64)), int(np.iscomplexobj(grid)),
+        _dp(sum_weight), int(do_psf), int(do_imaging_weight),
+        _dp(vis_c.view(np.float64)), nvispol, _dp(uvw), nt, nb,
+        _dp(freq_chan), nchan, _i64p(chan_map), _i64p(pol_map), npol,
+        _dp(weight), npol, _dp(cgk_1D), grid.shape[0], grid.shape[1], n_u, n_v,
+        float(cell[0]), float(cell[1]), int(support), int(oversampling))
+    return grid, sum_weight
+
+
+def ifft_uv_to_lm(ary, axes):
+    """core/imaging/ifft.py:5-18."""
+    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(ary, axes=axes),
+                                        axes=axes), axes=axes).real
+
+
+def fft_lm_to_uv(ary, axes):
+    """core/imaging/fft.py:5-18."""
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(ary, axes=axes),
+                                       axes=axes), axes=axes)
+
+
+def normalise(grid, sum_weight, support=7, oversampling=100):
+    """standard_grid.py:693-708: per (chan, corr) plane
+    real(fftshift(ifft2(ifftshift(grid)))) / corrTerm * nx * ny / sumwt."""
+    nx, ny = grid.shape[-2:]
+    _, corr = create_prolate_spheroidal_kernel(oversampling, support,
+                                               np.array([nx, ny], dtype=int))
+    out = np.zeros(grid.shape, dtype=np.float64)
+    for ch in range(grid.shape[0]):
+        for co in range(grid.shape[1]):
+            out[ch, co] = (np.real(np.fft.fftshift(np.fft.ifft2(
+                np.fft.ifftshift(grid[ch, co])))) / corr * nx * ny
+                / sum_weight[ch, co])
+    return out
+
+
+def grid2image(vis, uvw, weight, flag, freq_chan, n_uv, cell, support=7,
+               oversampling=100, chan_mode="continuum", do_psf=False):
+    """grid2image_spheroid_ms4 (standard_grid.py:607-708) on plain arrays:
+    weight is zeroed where flagged or where u/v is NaN (:664-668, on a copy
+    here), the data are gridded, and the dirty image (or PSF) is formed.
+    Returns (image, grid, sum_weight)."""
+    weight = np.array(weight, dtype=np.float64, copy=True)
+    if flag is not None:
+        weight[np.asarray(flag, dtype=bool)] = 0.0
+    nan_uvw = np.isnan(uvw[:, :, 0]) | np.isnan(uvw[:, :, 1])
+    weight[nan_uvw] = 0.0
+    grid, sw = standard_grid(vis, uvw, weight, freq_chan, n_uv, cell,
+                             support=support, oversampling=oversampling,
+                             chan_mode=chan_mode, do_psf=do_psf)
+    img = normalise(grid, sw, support, oversampling)
+    return img, grid, sw
+
+
+# --------------------------------------------------------------------------
+# Degridder  (core/imaging/imaging_utils/standard_degrid.py)
+# --------------------------------------------------------------------------
+def dgrid(uvw, values, flag, scale, offset, grid, freq, support, sampling,
+          convFunc, chanmap, polmap, c=C_LIGHT):
+    """dgrid_numba (standard_degrid.py:388-485), zero dphase, in place."""
+    assert values.dtype == np.complex128 and values.flags.c_contiguous
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    flag_u8 = np.ascontiguousarray(flag, dtype=np.uint8)
+    grid_c = np.ascontiguousarray(grid, dtype=np.complex128)
+    scale = np.ascontiguousarray(scale, dtype=np.float64)
+    offset = np.ascontiguousarray(offset, dtype=np.float64)
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    convFunc = np.ascontiguousarray(convFunc, dtype=np.float64)
+    chanmap = np.ascontiguousarray(chanmap, dtype=np.int64)
+    polmap = np.ascontiguousarray(polmap, dtype=np.int64)
+    nt, nb, nvc, nvp = values.shape
+    g = grid_c.shape
+    lib().ora_dgrid(_dp(uvw), _dp(values.view(np.float64)),
+                    flag_u8.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                    nt, nb, nvc, nvp, _dp(scale), _dp(offset),
+                    _dp(grid_c.view(np.float64)), g[0], g[1], g[2], g[3],
+                    _dp(freq), float(c), int(support), int(sampling),
+                    _dp(convFunc), _i64p(chanmap), _i64p(polmap))
+    return values
+
+
+def degrid(uvw, flag, freq, grid, cell, support=7, sampling=100,
+           model=None):
+    """degrid_spheroid_ms4 (standard_degrid.py:488-608) on plain arrays.
+    NaN-uvw rows are flagged and their uvw set to 1e9 (:557-566, on copies
+    here).  Returns the model visibilities, complex128."""
+    uvw = np.array(uvw, dtype=np.float64, copy=True)
+    flag = np.array(flag, dtype=bool, copy=True)
+    nt, nb, nf, npl = flag.shape
+    if model is None:
+        model = np.zeros((nt, nb, nf, npl), dtype=np.complex128)
+    nan_rows = np.isnan(uvw).any(axis=2)
+    flag[nan_rows] = True
+    uvw[nan_rows] = 1e9
+    nx, ny = grid.shape[-2:]
+    scale = np.array([-cell[0] * nx, -cell[1] * ny, 0.0], dtype=np.float64)
+    offset = np.array([nx / 2, ny / 2, 0.0], dtype=np.float64)
+    conv = create_prolate_spheroidal_kernel_1D(sampling, support)
+    nchan, npol = grid.shape[0], grid.shape[1]
+    chanmap = (np.arange(nf) / nf * nchan).astype(int)
+    polmap = np.round(np.arange(npl) % npol).astype(int)
+    return dgrid(uvw, model, flag, scale, offset, grid, freq, support,
+                 sampling, conv, chanmap, polmap)

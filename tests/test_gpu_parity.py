@@ -1,0 +1,297 @@
+"""GPU parity: libavgrid.so against the FP64 oracle on identical seeded inputs.
+
+Bars: integer geometry is exact (a misplaced sample shows up as an O(1)
+relative error); FP64 results within 1e-6 relative, FP32 within 1e-4
+relative (max |x - ref| / max |ref| per output).  Every test runs the native
+kernels; nothing falls back to the CPU.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+C = 299792458.0
+TOL64 = 1e-6
+TOL32 = 1e-4
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def c1_data(dense=False, flag_frac=0.0, weights=None, seed=1):
+    from astroviper_amd import synth
+    arr = synth.ArrayConfig(seed=seed)
+    sky = synth.SkyConfig()
+    obs = synth.make_observation(arr, sky)
+    vis, w, flag = synth.host_visibilities(obs, sky, flag_frac=flag_frac, weight_range=weights)
+    n = 256
+    if dense:  # every baseline on the grid: many samples per tile
+        maxuv = np.max(np.hypot(obs.uvw[..., 0], obs.uvw[..., 1]))
+        cell = 0.45 * C / obs.freq[-1] / maxuv
+    else:
+        cell = 5.0 * synth.ARCSEC
+    return obs.uvw, obs.freq, vis, w, flag, (n, n), (-cell, cell)
+
+
+def device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, psf=False, support=7,
+                ov=100, tile=(32, 32), unit_cap=0, compensated=False, geometry="reference"):
+    from astroviper_amd.device import GridPlan
+    nt, nb, nchan, npol = w.shape
+    rows = nt * nb
+    plan = GridPlan(rows, nchan, nchan if cube else 1, n_uv, cell, support, ov, tile=tile,
+                    unit_cap=unit_cap, geometry=geometry, device=gpu)
+    plan.set_uvw(torch.from_numpy(np.ascontiguousarray(uvw.reshape(rows, 3))).to(gpu),
+                 torch.from_numpy(freq).to(gpu))
+    rdt = torch.float32 if prec == 32 else torch.float64
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    v = torch.from_numpy(np.ascontiguousarray(vis.reshape(rows * nchan, npol))).to(gpu, cdt)
+    wt = torch.from_numpy(np.ascontiguousarray(w.reshape(rows * nchan, npol))).to(gpu, rdt)
+    fl = None
+    if flag is not None:
+        fl = torch.from_numpy(np.ascontiguousarray(flag.reshape(rows * nchan, npol))
+                              .astype(np.uint8)).to(gpu)
+    g, sw = plan.grid(v, wt, fl, psf=psf, compensated=compensated)
+    return g, sw, plan
+
+
+# ------------------------------------------------------------- PS tables
+def test_ps_tables_match_oracle(gpu, oracle):
+    from astroviper_amd import device as D
+    from astroviper_amd import gcf_prolate_spheroidal as G
+    u = np.linspace(-1.2, 1.2, 2001)
+    g, k = G.prolate_spheroidal_function(u)
+    go, ko = oracle.prolate_spheroidal_function(u)
+    assert np.max(np.abs(g - go)) < 1e-14 and np.max(np.abs(k - ko)) < 1e-14
+    for s in (5, 7, 9):
+        t = G.create_prolate_spheroidal_kernel_1D(100, s)
+        np.testing.assert_allclose(t, oracle.create_prolate_spheroidal_kernel_1D(100, s),
+                                   rtol=1e-14, atol=1e-16)
+    k4, corr = G.create_prolate_spheroidal_kernel(100, 7, np.array([96, 64]))
+    k4o, corro = oracle.create_prolate_spheroidal_kernel(100, 7, np.array([96, 64]))
+    np.testing.assert_allclose(k4, k4o, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(corr, corro, rtol=1e-14)
+    tu = D.ps_correction_1d(96, gpu).cpu().numpy()
+    tv = D.ps_correction_1d(64, gpu).cpu().numpy()
+    np.testing.assert_allclose(np.outer(tu, tv), corro, rtol=1e-14)
+
+
+# --------------------------------------------------------------- gridding
+@pytest.mark.parametrize("cube", [False, True])
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("dense", [False, True])
+def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=dense, flag_frac=0.1,
+                                                  weights=(0.5, 1.5))
+    w0 = w.copy()
+    w0[flag] = 0
+    gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell,
+                                       chan_mode="cube" if cube else "continuum")
+    g, sw, plan = device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec)
+    st = plan.stats()
+    assert st["grid_valid"] > 1000
+    tol = TOL32 if prec == 32 else TOL64
+    assert rel(g.cpu().numpy(), gref) < tol
+    assert rel(sw.cpu().numpy(), swref) < 1e-12  # sums of weights are FP64 either way
+    # dirty image through the device FFT path
+    from astroviper_amd.device import dirty_image
+    img = dirty_image(g, sw).cpu().numpy()
+    iref = oracle.normalise(gref, swref)
+    assert rel(img, iref) < tol
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_psf_c1_matches_oracle(gpu, oracle, prec):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, weights=(0.5, 1.5))
+    gref, swref = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_psf=True)
+    g, sw, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, prec, psf=True)
+    tol = TOL32 if prec == 32 else TOL64
+    assert rel(g.cpu().numpy(), gref) < tol
+    from astroviper_amd.device import dirty_image
+    psf = dirty_image(g, sw).cpu().numpy()
+    pref = oracle.normalise(gref, swref)
+    assert rel(psf, pref) < tol
+    assert abs(psf[0, 0, 128, 128] - 1 / 0.9999996674 ** 2) < 1e-5
+
+
+def test_compensated_fp32_is_closer(gpu, oracle):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, weights=(0.5, 1.5))
+    gref, _ = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_psf=True)
+    g1, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True)
+    g2, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True,
+                           compensated=True, unit_cap=100000)
+    e1, e2 = rel(g1.cpu().numpy(), gref), rel(g2.cpu().numpy(), gref)
+    assert e2 < TOL32 and e1 < TOL32
+    print(f"FP32 grid error: plain {e1:.3e}, compensated {e2:.3e}")
+    assert e2 <= e1 * 1.2
+
+
+@pytest.mark.parametrize("tile,support,npol,cap", [
+    ((16, 16), 7, 2, 0), ((64, 64), 7, 1, 0), ((32, 32), 5, 4, 0), ((32, 32), 9, 4, 0),
+    ((32, 16), 9, 1, 0), ((16, 32), 7, 4, 64), ((32, 32), 7, 2, 64), ((24, 40), 6, 2, 100)])
+def test_grid_shapes_supports_units(gpu, oracle, tile, support, npol, cap):
+    rng = np.random.default_rng(support * 100 + npol)
+    nt, nb, nchan, n = 6, 120, 3, 160
+    uvw = rng.uniform(-1800, 1800, size=(nt, nb, 3))
+    freq = np.linspace(1.0e9, 1.2e9, nchan)
+    cell = (-0.5 * C / freq[-1] / 2600.0, 0.5 * C / freq[-1] / 2600.0)
+    vis = rng.standard_normal((nt, nb, nchan, npol)) + 1j * rng.standard_normal((nt, nb, nchan, npol))
+    w = rng.uniform(0.5, 1.5, size=(nt, nb, nchan, npol))
+    for cube in (False, True):
+        gref, swref = oracle.standard_grid(vis, uvw, w, freq, (n, n), cell, support=support,
+                                           chan_mode="cube" if cube else "continuum")
+        for prec in (64, 32):
+            g, sw, _ = device_grid(gpu, vis, uvw, w, None, freq, (n, n), cell, cube, prec,
+                                   support=support, tile=tile, unit_cap=cap)
+            assert rel(g.cpu().numpy(), gref) < (TOL32 if prec == 32 else TOL64), (cube, prec)
+            assert rel(sw.cpu().numpy(), swref) < 1e-12
+
+
+def test_grid_exact_ties(gpu, oracle):
+    # f = c and cell = 1/N make every position an exact half-integer
+    n = 64
+    rng = np.random.default_rng(3)
+    uvw = np.zeros((1, 400, 3))
+    uvw[0, :, :2] = rng.integers(-25, 25, size=(400, 2)) + 0.5
+    freq = np.array([C])
+    vis = rng.standard_normal((1, 400, 1, 1)) + 1j * rng.standard_normal((1, 400, 1, 1))
+    w = np.ones(vis.shape)
+    cell = (1.0 / n, 1.0 / n)
+    gref, swref = oracle.standard_grid(vis, uvw, w, freq, (n, n), cell)
+    g, sw, plan = device_grid(gpu, vis, uvw, w, None, freq, (n, n), cell, False, 64,
+                              tile=(16, 16))
+    assert rel(g.cpu().numpy(), gref) < 1e-12
+    grid = rng.standard_normal((1, 1, n, n)) + 1j * rng.standard_normal((1, 1, n, n))
+    mref = oracle.degrid(uvw, np.zeros(vis.shape, bool), freq, grid, cell)
+    m = plan.degrid(torch.from_numpy(grid).to(gpu)).cpu().numpy()
+    assert rel(m.reshape(mref.shape), mref) < 1e-12
+
+
+def test_grid_edge_cases(gpu, oracle):
+    from astroviper_amd.device import GridPlan
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data()
+    # everything flagged: zero grid, zero sums
+    g, sw, _ = device_grid(gpu, vis, uvw, w, np.ones(w.shape, bool), freq, n_uv, cell, True, 32)
+    assert not g.cpu().numpy().any() and not sw.cpu().numpy().any()
+    # everything off the grid
+    g, sw, _ = device_grid(gpu, vis, uvw * 1e4, w, None, freq, n_uv, cell, False, 64)
+    assert not g.cpu().numpy().any()
+    # NaN uvw rows and NaN / zero visibilities are skipped
+    uvw2 = uvw.copy()
+    uvw2[3, :40, 0] = np.nan
+    vis2 = vis.copy()
+    vis2[5, :, :, 0] = np.nan
+    vis2[6, :, :, 1] = 0
+    w2 = w.copy()
+    w2[3, :40] = 0
+    gref, swref = oracle.standard_grid(vis2, uvw, w2, freq, n_uv, cell)
+    g, sw, _ = device_grid(gpu, vis2, uvw2, w, None, freq, n_uv, cell, False, 64)
+    assert rel(g.cpu().numpy(), gref) < TOL64
+    assert rel(sw.cpu().numpy(), swref) < 1e-12
+    # an empty plan
+    plan = GridPlan(0, 4, 1, n_uv, cell, device=gpu)
+    plan.set_uvw(torch.zeros((0, 3), dtype=torch.float64, device=gpu),
+                 torch.from_numpy(freq).to(gpu))
+    g, sw = plan.grid(torch.zeros((0, 2), dtype=torch.complex64, device=gpu),
+                      torch.zeros((0, 2), dtype=torch.float32, device=gpu))
+    assert not g.cpu().numpy().any()
+
+
+def test_accumulate_mode(gpu, oracle):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
+    g, sw, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, True, 64)
+    rows, nchan, npol = uvw.shape[0] * uvw.shape[1], freq.size, w.shape[3]
+    v = torch.from_numpy(vis.reshape(rows * nchan, npol)).to(gpu)
+    wt = torch.from_numpy(w.reshape(rows * nchan, npol)).to(gpu)
+    g2, sw2 = plan.grid(v, wt, None, grid=g.clone(), sumwt=sw.clone(), accumulate=True)
+    assert rel(g2.cpu().numpy(), 2 * g.cpu().numpy()) < 1e-14
+    assert rel(sw2.cpu().numpy(), 2 * sw.cpu().numpy()) < 1e-14
+
+
+# ------------------------------------------------------------- degridding
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("cube", [False, True])
+def test_degrid_c1_matches_oracle(gpu, oracle, prec, cube):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, flag_frac=0.1)
+    rng = np.random.default_rng(5)
+    nchan_g = freq.size if cube else 1
+    grid = rng.standard_normal((nchan_g, 2) + n_uv) + 1j * rng.standard_normal((nchan_g, 2) + n_uv)
+    mref = oracle.degrid(uvw, flag, freq, grid, cell)
+    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, cube, 64)
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    gd = torch.from_numpy(grid).to(gpu, cdt)
+    fl = torch.from_numpy(flag.reshape(-1, 2).astype(np.uint8)).to(gpu)
+    m = plan.degrid(gd, fl).cpu().numpy().reshape(mref.shape)
+    assert rel(m, mref) < (TOL32 if prec == 32 else TOL64)
+    assert np.all(m[flag] == 0)
+    # incremental adds onto what is there; flagged samples stay untouched
+    start = torch.ones((m.size // 2, 2), dtype=cdt, device=gpu)
+    m2 = plan.degrid(gd, fl, start, incremental=True).cpu().numpy().reshape(mref.shape)
+    assert rel(m2, mref + 1.0) < (TOL32 if prec == 32 else TOL64)
+
+
+def test_degrid_pol_map_and_constant_grid(gpu, oracle):
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
+    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 64)
+    g = torch.full((1, 1) + n_uv, 2.0 + 0j, dtype=torch.complex128, device=gpu)
+    m = plan.degrid(g, n_pol=2, pol_map=torch.zeros(2, dtype=torch.int32)).cpu().numpy()
+    on = np.abs(m) > 0
+    assert on.mean() > 0.5
+    assert np.all(m[on] == 2.0)  # exactly, as test_standard_degrid.py:150 asserts
+    g32 = g.to(torch.complex64)
+    m32 = plan.degrid(g32, n_pol=2, pol_map=torch.zeros(2, dtype=torch.int32)).cpu().numpy()
+    assert np.all(m32[np.abs(m32) > 0] == np.float32(2.0))
+
+
+# --------------------------------------------------------------- adjoint
+def test_adjoint_pair_dot_product(gpu):
+    """<D x, y> = <x, D^H y> with D = degrid (geometry 'degrid') and D^H =
+    grid with the ADJOINT weighting, to FP64 rounding."""
+    from astroviper_amd.device import GridPlan
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
+    rows, nchan = uvw.shape[0] * uvw.shape[1], freq.size
+    plan = GridPlan(rows, nchan, 1, n_uv, cell, geometry="degrid", device=gpu)
+    plan.set_uvw(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu), torch.from_numpy(freq).to(gpu))
+    rng = np.random.default_rng(8)
+    x = torch.from_numpy(rng.standard_normal((1, 1) + n_uv) + 1j * rng.standard_normal((1, 1) + n_uv)).to(gpu)
+    y = torch.from_numpy(rng.standard_normal((rows * nchan, 1)) + 1j * rng.standard_normal((rows * nchan, 1))).to(gpu)
+    dx = plan.degrid(x)
+    dhy, _ = plan.grid(y, None, adjoint=True, n_pol=1)
+    lhs = torch.vdot(dx.flatten(), y.flatten()).item()
+    rhs = torch.vdot(x.flatten(), dhy.flatten()).item()
+    assert abs(lhs - rhs) / (torch.linalg.norm(dx) * torch.linalg.norm(y)).item() < 1e-12
+
+
+# ------------------------------------------------------------ plain path
+@pytest.mark.parametrize("prec", [64, 32])
+def test_plain_gridder_matches_oracle(gpu, oracle, prec):
+    from astroviper_amd.device import degrid_plain, grid_plain
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, flag_frac=0.1)
+    w0 = w.copy()
+    w0[flag] = 0
+    gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell, chan_mode="cube")
+    rows, nchan, npol = uvw.shape[0] * uvw.shape[1], freq.size, 2
+    rdt = torch.float32 if prec == 32 else torch.float64
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    u_t = torch.from_numpy(uvw.reshape(rows, 3)).to(gpu)
+    f_t = torch.from_numpy(freq).to(gpu)
+    g, sw = grid_plain(u_t, f_t, torch.from_numpy(vis.reshape(rows, nchan, npol)).to(gpu, cdt),
+                       torch.from_numpy(w.reshape(rows, nchan, npol)).to(gpu, rdt),
+                       torch.from_numpy(flag.reshape(rows, nchan, npol)).to(gpu), n_uv, cell,
+                       nchan, list(range(nchan)))
+    tol = TOL32 if prec == 32 else TOL64
+    assert rel(g.cpu().numpy(), gref) < tol
+    assert rel(sw.cpu().numpy(), swref) < 1e-12
+    grid = gref.astype(np.complex64 if prec == 32 else np.complex128)
+    mref = oracle.degrid(uvw, flag, freq, gref, cell)
+    m = degrid_plain(u_t, f_t, torch.from_numpy(grid).to(gpu),
+                     torch.from_numpy(flag.reshape(rows, nchan, npol)).to(gpu), cell,
+                     list(range(nchan)), npol).cpu().numpy()
+    assert rel(m.reshape(mref.shape), mref) < tol
