@@ -58,7 +58,7 @@ SIGNATURES = {
     "av_degrid_plain": (_c.c_int, [_P, _I64, _I32, _I32, _D, _I32, _P, _I32, _P, _I32,
                                    _P, _I32, _I32, _D, _D, _I32, _I32, _P, _I32, _I32,
                                    _P]),
-    "av_ifftshift": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _P]),
+    "av_ifftshift": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P]),
     "av_normalise": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _I32, _P]),
 }
 

@@ -6,8 +6,8 @@
 
 namespace av {
 
-template <typename R>
-__global__ void k_ifftshift(const R *in, R *out, int64_t n_planes, int n_u, int n_v) {
+template <typename R, typename O>
+__global__ void k_ifftshift(const R *in, O *out, int64_t n_planes, int n_u, int n_v) {
   // np.fft.ifftshift: out[k] = in[(k + n//2) % n] on each of the two axes
   const int64_t plane = (int64_t)n_u * n_v;
   const int64_t total = n_planes * plane;
@@ -21,8 +21,8 @@ __global__ void k_ifftshift(const R *in, R *out, int64_t n_planes, int n_u, int 
     int sv = kv + n_v / 2;
     if (sv >= n_v) sv -= n_v;
     const R *src = in + (p * plane + (int64_t)su * n_v + sv) * 2;
-    out[i * 2] = src[0];
-    out[i * 2 + 1] = src[1];
+    out[i * 2] = (O)src[0];
+    out[i * 2 + 1] = (O)src[1];
   }
 }
 
@@ -62,19 +62,26 @@ using namespace av;
 extern "C" {
 
 int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u, int32_t n_v,
-                 int32_t dtype, void *stream) {
+                 int32_t dtype, int32_t out_dtype, void *stream) {
   if (!in || !out || n_planes < 0 || n_u < 1 || n_v < 1)
     return fail(AV_ERR_INVALID, "av_ifftshift: bad arguments");
   if (in == out) return fail(AV_ERR_INVALID, "av_ifftshift: in and out must differ");
   const int64_t total = n_planes * n_u * n_v;
   if (total == 0) return AV_OK;
   hipStream_t s = (hipStream_t)stream;
-  if (dtype == AV_F32)
-    hipLaunchKernelGGL(k_ifftshift<float>, dim3(grid_blocks(total)), dim3(256), 0, s,
-                       (const float *)in, (float *)out, n_planes, n_u, n_v);
-  else if (dtype == AV_F64)
-    hipLaunchKernelGGL(k_ifftshift<double>, dim3(grid_blocks(total)), dim3(256), 0, s,
-                       (const double *)in, (double *)out, n_planes, n_u, n_v);
+  dim3 g(grid_blocks(total)), b(256);
+  if (dtype == AV_F32 && out_dtype == AV_F32)
+    hipLaunchKernelGGL((k_ifftshift<float, float>), g, b, 0, s, (const float *)in, (float *)out,
+                       n_planes, n_u, n_v);
+  else if (dtype == AV_F32 && out_dtype == AV_F64)
+    hipLaunchKernelGGL((k_ifftshift<float, double>), g, b, 0, s, (const float *)in,
+                       (double *)out, n_planes, n_u, n_v);
+  else if (dtype == AV_F64 && out_dtype == AV_F64)
+    hipLaunchKernelGGL((k_ifftshift<double, double>), g, b, 0, s, (const double *)in,
+                       (double *)out, n_planes, n_u, n_v);
+  else if (dtype == AV_F64 && out_dtype == AV_F32)
+    hipLaunchKernelGGL((k_ifftshift<double, float>), g, b, 0, s, (const double *)in,
+                       (float *)out, n_planes, n_u, n_v);
   else
     return fail(AV_ERR_INVALID, "av_ifftshift: bad dtype");
   AV_LAUNCH_CHECK("k_ifftshift");

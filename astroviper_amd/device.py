@@ -258,22 +258,29 @@ class GridPlan:
 # --------------------------------------------------------- dirty image --
 def dirty_image(grid: torch.Tensor, sumwt: torch.Tensor, support: int = 7,
                 oversampling: int = 100, out_dtype: torch.dtype = torch.float64,
+                fft_dtype: torch.dtype = torch.complex128,
                 work: Optional[torch.Tensor] = None) -> torch.Tensor:
     """real(fftshift(ifft2(ifftshift(grid)))) / taper * nx*ny / sumwt per plane
-    (standard_grid.py:693-708).  grid [..., n_u, n_v] complex on the GPU."""
+    (standard_grid.py:693-708).  grid [..., n_u, n_v] complex on the GPU.
+
+    The FFT runs in `fft_dtype` (FP64 by default, also for an FP32 grid: the
+    taper correction divides by down to grdsf(1)^2 = 1.6e-5 at the image
+    edge and would magnify FP32 FFT rounding to ~1e-3 of the peak).
+    """
     code = _real_code(grid.dtype)
+    fcode = _real_code(fft_dtype)
     n_u, n_v = grid.shape[-2:]
     planes = grid.numel() // (n_u * n_v)
     g = grid.contiguous()
     if work is None:
-        work = torch.empty_like(g)
-    call("av_ifftshift", _ptr(g), _ptr(work), planes, n_u, n_v, code, _stream())
+        work = torch.empty(g.shape, dtype=fft_dtype, device=g.device)
+    call("av_ifftshift", _ptr(g), _ptr(work), planes, n_u, n_v, code, fcode, _stream())
     torch.fft.ifft2(work, dim=(-2, -1), out=work)
     tu = ps_correction_1d(n_u, grid.device)
     tv = tu if n_v == n_u else ps_correction_1d(n_v, grid.device)
     out = torch.empty(grid.shape, dtype=out_dtype, device=grid.device)
     call("av_normalise", _ptr(work), _ptr(sumwt.contiguous().to(torch.float64)), _ptr(tu),
-         _ptr(tv), planes, n_u, n_v, code, _ptr(out), _real_code(out_dtype), _stream())
+         _ptr(tv), planes, n_u, n_v, fcode, _ptr(out), _real_code(out_dtype), _stream())
     return out
 
 

@@ -1,0 +1,271 @@
+"""Benchmark: visibilities/s of the batched FP32 standard gridder (grid +
+degrid, support 7, oversampling 100) on one MI355X.
+
+Workload (default): 64 antennas (2016 baselines) x 1000 time samples x 64
+channels x 4 polarisations = 516,096,000 complex64 visibilities with FP64
+uvw, 10% flagged, weights U(0.5, 1.5); 2048 x 2048 grid, 1" cells, cube
+mode (256 planes).  Inputs are generated on the GPU and stay resident.
+
+A step is one pass of the hot path over the batch: bin the samples
+(av_plan_set_uvw), grid them (av_grid) and degrid them back (av_degrid).
+
+    python bench.py [--steps K] [--warmup W] [--mode cube|continuum]
+
+Prints one JSON line.  The CPU baseline is the FP64 C oracle (a loop-by-loop
+restatement of the reference's numba kernels) timed single-threaded on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(obs, uvw_d, vis_d, w_d, flag_d, n_uv, cell, cube, budget_s=12.0):
+    """Single-threaded FP64 oracle grid+degrid on a bounded sample: the first
+    rows (time-major) in the first 4 channels, grown until the pair takes a
+    few seconds.  Per-visibility cost does not depend on the channel, and 4
+    channels keep the FP64 host grid at 1 GB."""
+    from oracle import oracle as O
+
+    nt, nb, _ = obs.uvw.shape
+    npol = vis_d.shape[2]
+    nch = min(4, vis_d.shape[1])
+    freq = obs.freq[:nch]
+    total_rows = nt * nb
+    rows = 20000
+    while True:
+        rows = min(rows, total_rows)
+        sel = slice(0, rows)
+        uvw = uvw_d[sel].cpu().numpy().reshape(1, rows, 3)
+        vis = vis_d[sel, :nch].cpu().numpy().astype(np.complex128).reshape(1, rows, nch, npol)
+        w = w_d[sel, :nch].cpu().numpy().astype(np.float64).reshape(1, rows, nch, npol)
+        fl = flag_d[sel, :nch].cpu().numpy().astype(bool).reshape(1, rows, nch, npol)
+        w[fl] = 0
+        t0 = time.perf_counter()
+        g, sw = O.standard_grid(vis, uvw, w, freq, n_uv, cell,
+                                chan_mode="cube" if cube else "continuum")
+        t1 = time.perf_counter()
+        O.degrid(uvw, fl, freq, g, cell)
+        t2 = time.perf_counter()
+        if (t2 - t0) > budget_s / 4 or rows * 4 > total_rows:
+            break
+        rows *= 4
+    tg, td = t1 - t0, t2 - t1
+    nvis = rows * nch * npol
+    return {
+        "value": round(nvis / (tg + td) / 1e6, 3),
+        "unit": "M visibilities/sec",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {rows} of {total_rows} rows x first {nch} of {vis_d.shape[1]} "
+                  f"channels x {npol} pols = {nvis} visibilities, FP64 grid {tg:.2f} s + "
+                  f"degrid {td:.2f} s, one thread of {cpu_model()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", choices=["cube", "continuum"], default="cube")
+    ap.add_argument("--tile", type=int, nargs=2, default=[32, 32])
+    ap.add_argument("--unit-cap", type=int, default=0)
+    ap.add_argument("--compensated", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--time-samples", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from astroviper_amd import synth
+    from astroviper_amd.device import GridPlan, degrid_plain, grid_plain
+
+    dev = torch.device("cuda", 0)
+    cfg = synth.CONFIG2
+    arr = synth.ArrayConfig(**{**cfg["array"].__dict__, "n_time": args.time_samples})
+    sky = cfg["sky"]
+    n_uv = cfg["n_uv"]
+    cellv = cfg["cell_arcsec"] * synth.ARCSEC
+    cell = (-cellv, cellv)
+    cube = args.mode == "cube"
+    log(f"bench: generating workload ({arr.n_ant} antennas, {arr.n_time} times, "
+        f"{arr.n_chan} channels, {arr.n_pol} pols) on the GPU")
+    obs = synth.make_observation(arr, sky)
+    uvw, vis, w, flag = synth.device_visibilities(obs, sky, cfg["flag_frac"],
+                                                  cfg["weight_range"], device=dev)
+    rows, nchan, npol = vis.shape
+    V = rows * nchan * npol
+    n_img = nchan if cube else 1
+    freq = torch.from_numpy(obs.freq).to(dev)
+    vis2 = vis.reshape(rows * nchan, npol)
+    w2 = w.reshape(rows * nchan, npol)
+    fl2 = flag.reshape(rows * nchan, npol)
+    plan = GridPlan(rows, nchan, n_img, n_uv, cell, 7, 100, tile=tuple(args.tile),
+                    unit_cap=args.unit_cap, device=dev)
+    grid = torch.empty((n_img, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
+    sumwt = torch.empty((n_img, npol), dtype=torch.float64, device=dev)
+    model = torch.empty((rows * nchan, npol), dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize()
+    log(f"bench: {V} visibilities resident; {n_img * npol} planes of {n_uv[0]}x{n_uv[1]}")
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        plan.set_uvw(uvw, freq)
+        if ev is not None:
+            ev[1].record()
+        plan.grid(vis2, w2, fl2, grid=grid, sumwt=sumwt, compensated=args.compensated)
+        if ev is not None:
+            ev[2].record()
+        plan.degrid(grid, fl2, model)
+        if ev is not None:
+            ev[3].record()
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"bench: warmup step {i + 1}/{args.warmup} done")
+    st = plan.stats()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    t_plan = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    t_grid = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    t_degrid = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
+    value = V / (ms / 1e3) / 1e6
+    log(f"bench: {args.steps} steps, {ms:.2f} ms/step (plan {t_plan:.2f}, grid {t_grid:.2f}, "
+        f"degrid {t_degrid:.2f} ms) -> {value:.0f} M vis/s")
+
+    cells = n_img * npol * n_uv[0] * n_uv[1]
+    b_grid = V * 13 + cells * 8          # vis 8 + weight 4 + flag 1; grid written once
+    b_degrid = V * 9 + cells * 8         # flag 1 + model 8; grid read once
+    b_plan = rows * 24 + nchan * 8       # uvw once, frequencies
+    b_pair = V * 22 + rows * 48 + cells * 16 + nchan * 16
+    kern = {"grid": (t_grid, b_grid), "degrid": (t_degrid, b_degrid)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    tdom, bdom = kern[dom]
+    achieved = bdom / (tdom / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_grid_tiles" if dom == "grid" else "k_degrid_tiles",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+                "algorithmic_bytes": bdom, "avg_ms": round(tdom, 3)}
+
+    extra = {"plan_ms": round(t_plan, 3), "grid_ms": round(t_grid, 3),
+             "degrid_ms": round(t_degrid, 3),
+             "pair_roofline_frac": round(b_pair / (ms / 1e3) / HBM_PEAK, 4),
+             "pair_algorithmic_bytes": b_pair, "plan_stats": st}
+
+    # plain one-thread-per-sample gridder on a channel subset, scaled
+    if not args.no_plain:
+        chans = list(range(0, nchan, max(1, nchan // 8)))
+        cmap = list(range(nchan)) if cube else [0] * nchan
+        pg = torch.zeros_like(grid)
+        psw = torch.zeros_like(sumwt)
+        pm = torch.zeros((rows, nchan, npol), dtype=torch.complex64, device=dev)
+        cgk = plan.cgk
+        grid_plain(uvw, freq, vis, w, flag, n_uv, cell, n_img, cmap, grid=pg, sumwt=psw,
+                   cgk=cgk, chans=chans[:1])
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        pg.zero_()
+        grid_plain(uvw, freq, vis, w, flag, n_uv, cell, n_img, cmap, grid=pg, sumwt=psw,
+                   cgk=cgk, chans=chans)
+        degrid_plain(uvw, freq, pg, flag, cell, cmap, npol, vis_out=pm, cgk=cgk, chans=chans)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        vsub = rows * len(chans) * npol
+        plain = vsub / (tb - ta) / 1e6
+        extra["plain_gpu"] = {"value": round(plain, 1), "unit": "M visibilities/sec",
+                              "sample": f"{len(chans)} of {nchan} channels, one launch each"}
+        extra["speedup_vs_plain_gpu"] = round(value / plain, 2)
+        log(f"bench: plain GPU gridder {plain:.0f} M vis/s on {len(chans)} channels")
+        del pg, pm
+
+    # accuracy of this FP32 run against an FP64 GPU run on the same inputs
+    if not args.no_check:
+        g64, sw64 = plan.grid(vis2.to(torch.complex128), w2.to(torch.float64), fl2)
+        m64 = plan.degrid(g64, fl2)
+        torch.cuda.synchronize()
+
+        def relerr(a, b):
+            return float((a - b).abs().max() / b.abs().max())
+
+        extra["max_rel_err_vs_fp64"] = {
+            "grid": relerr(grid.to(torch.complex128), g64),
+            "sumwt": relerr(sumwt, sw64),
+            "degrid_vs_fp64_grid": relerr(model.to(torch.complex128), m64),
+        }
+        log(f"bench: errors vs FP64 {extra['max_rel_err_vs_fp64']}")
+        del g64, m64
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(obs, uvw, vis, w, flag, n_uv, cell, cube)
+        log(f"bench: cpu baseline {cpu['value']:.2f} M vis/s ({cpu['sample']})")
+
+    out = {
+        "metric": "M visibilities/sec (grid+degrid, support 7) batched FP32 vs plain GPU @<=1e-4 rel err",
+        "value": round(value, 1),
+        "unit": "M visibilities/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 coordinates and weight sums)",
+        "data": "synthetic (seeded point-source sky, earth-rotation uvw)",
+        "config": {"workload": f"standard gridder grid+degrid, {arr.n_ant} antennas x "
+                               f"{arr.n_time} times x {nchan} chan x {npol} pol = {V} vis, "
+                               f"{n_uv[0]}x{n_uv[1]} {args.mode}, support 7, oversampling 100, "
+                               f"10% flagged",
+                   "mode": args.mode, "tile": list(args.tile), "visibilities": V,
+                   "compensated": bool(args.compensated)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

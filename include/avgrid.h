@@ -173,13 +173,16 @@ int av_degrid_plain(const double *uvw, int64_t n_rows, int32_t n_chan,
 /* ---------------------------------------------- FFT pre/post passes --
  * The FFT itself is a library call (rocFFT through the host runtime).
  * av_ifftshift: out = ifftshift(in) over the last two axes of
- * n_planes complex planes [n_u][n_v] (np.fft.ifftshift, standard_grid.py:701).
+ * n_planes complex planes [n_u][n_v] (np.fft.ifftshift, standard_grid.py:701),
+ * converting from dtype to out_dtype (an FP32 grid can be transformed in
+ * FP64: the taper correction divides by down to grdsf(1)^2 = 1.6e-5 at the
+ * image edge, which magnifies FFT rounding). */
  * av_normalise: the post-FFT step of grid2image_spheroid_ms4,
  * standard_grid.py:696-708: out[p] = real(fftshift(Y[p])) / taper * n_u *
  * n_v / sumwt[p], with taper the outer product of av_ps_correction_1d on
  * both axes; out is float64 (out_dtype AV_F64) or float32 (AV_F32). */
 int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u,
-                 int32_t n_v, int32_t dtype, void *stream);
+                 int32_t n_v, int32_t dtype, int32_t out_dtype, void *stream);
 int av_normalise(const void *fft_planes, const double *sumwt,
                  const double *taper_u, const double *taper_v,
                  int64_t n_planes, int32_t n_u, int32_t n_v, int32_t dtype,

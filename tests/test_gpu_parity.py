@@ -26,6 +26,20 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
 
 
+def as_prec(prec, *arrays):
+    """Inputs as the GPU sees them: FP32 runs grid float32/complex64 samples,
+    so the oracle gets the same rounded values (then computes in FP64)."""
+    if prec != 32:
+        return arrays if len(arrays) > 1 else arrays[0]
+    out = []
+    for a in arrays:
+        if np.iscomplexobj(a):
+            out.append(a.astype(np.complex64).astype(np.complex128))
+        else:
+            out.append(a.astype(np.float32).astype(np.float64))
+    return out if len(out) > 1 else out[0]
+
+
 def c1_data(dense=False, flag_frac=0.0, weights=None, seed=1):
     from astroviper_amd import synth
     arr = synth.ArrayConfig(seed=seed)
@@ -90,6 +104,7 @@ def test_ps_tables_match_oracle(gpu, oracle):
 def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=dense, flag_frac=0.1,
                                                   weights=(0.5, 1.5))
+    vis, w = as_prec(prec, vis, w)
     w0 = w.copy()
     w0[flag] = 0
     gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell,
@@ -110,6 +125,7 @@ def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
 @pytest.mark.parametrize("prec", [64, 32])
 def test_psf_c1_matches_oracle(gpu, oracle, prec):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, weights=(0.5, 1.5))
+    vis, w = as_prec(prec, vis, w)
     gref, swref = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_psf=True)
     g, sw, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, prec, psf=True)
     tol = TOL32 if prec == 32 else TOL64
@@ -123,6 +139,7 @@ def test_psf_c1_matches_oracle(gpu, oracle, prec):
 
 def test_compensated_fp32_is_closer(gpu, oracle):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, weights=(0.5, 1.5))
+    vis, w = as_prec(32, vis, w)
     gref, _ = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_psf=True)
     g1, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True)
     g2, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True,
@@ -145,9 +162,11 @@ def test_grid_shapes_supports_units(gpu, oracle, tile, support, npol, cap):
     vis = rng.standard_normal((nt, nb, nchan, npol)) + 1j * rng.standard_normal((nt, nb, nchan, npol))
     w = rng.uniform(0.5, 1.5, size=(nt, nb, nchan, npol))
     for cube in (False, True):
-        gref, swref = oracle.standard_grid(vis, uvw, w, freq, (n, n), cell, support=support,
-                                           chan_mode="cube" if cube else "continuum")
         for prec in (64, 32):
+            vis_p, w_p = as_prec(prec, vis, w)
+            gref, swref = oracle.standard_grid(vis_p, uvw, w_p, freq, (n, n), cell,
+                                               support=support,
+                                               chan_mode="cube" if cube else "continuum")
             g, sw, _ = device_grid(gpu, vis, uvw, w, None, freq, (n, n), cell, cube, prec,
                                    support=support, tile=tile, unit_cap=cap)
             assert rel(g.cpu().numpy(), gref) < (TOL32 if prec == 32 else TOL64), (cube, prec)
@@ -223,6 +242,7 @@ def test_degrid_c1_matches_oracle(gpu, oracle, prec, cube):
     rng = np.random.default_rng(5)
     nchan_g = freq.size if cube else 1
     grid = rng.standard_normal((nchan_g, 2) + n_uv) + 1j * rng.standard_normal((nchan_g, 2) + n_uv)
+    grid = as_prec(prec, grid)
     mref = oracle.degrid(uvw, flag, freq, grid, cell)
     _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, cube, 64)
     cdt = torch.complex64 if prec == 32 else torch.complex128
@@ -274,6 +294,7 @@ def test_adjoint_pair_dot_product(gpu):
 def test_plain_gridder_matches_oracle(gpu, oracle, prec):
     from astroviper_amd.device import degrid_plain, grid_plain
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, flag_frac=0.1)
+    vis, w = as_prec(prec, vis, w)
     w0 = w.copy()
     w0[flag] = 0
     gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell, chan_mode="cube")
@@ -290,7 +311,7 @@ def test_plain_gridder_matches_oracle(gpu, oracle, prec):
     assert rel(g.cpu().numpy(), gref) < tol
     assert rel(sw.cpu().numpy(), swref) < 1e-12
     grid = gref.astype(np.complex64 if prec == 32 else np.complex128)
-    mref = oracle.degrid(uvw, flag, freq, gref, cell)
+    mref = oracle.degrid(uvw, flag, freq, as_prec(prec, gref), cell)
     m = degrid_plain(u_t, f_t, torch.from_numpy(grid).to(gpu),
                      torch.from_numpy(flag.reshape(rows, nchan, npol)).to(gpu), cell,
                      list(range(nchan)), npol).cpu().numpy()

@@ -1,0 +1,234 @@
+"""The reference's own hot-path tests, run through the GPU mirror of its API.
+
+Each test restates a test of tests/unit/core/imaging/... on the same kind of
+input (synthetic uvw where the reference downloads a measurement set) and
+also compares with the FP64 oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+C = 299792458.0
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")
+
+
+def gold(name):
+    with open(GOLDEN) as f:
+        return json.load(f)[name]
+
+
+def point_source_setup():
+    """test_standard_grid.py:15-41: 200 x 200 regular uv grid carrying
+    conj(FFT(model)) of two point sources."""
+    from astroviper_amd.fft import fft_lm_to_uv
+    g = gold("standard_grid_point_sources")
+    nant, maxuv = g["nant"], g["maxUV"]
+    freq = np.array([g["freq"]])
+    cell = 0.5 * C / freq[0] / maxuv
+    mod = np.zeros((nant, nant))
+    for (x, y, s) in g["sources"]:
+        mod[x, y] = s
+    ft = np.conj(fft_lm_to_uv(mod, axes=[0, 1]))
+    ax = np.linspace(-maxuv, maxuv, nant)
+    vis = np.zeros((1, nant * nant, 1, 1), dtype=np.complex128)
+    uvw = np.zeros((1, nant * nant, 3))
+    for ui, u in enumerate(ax):
+        for vi, v in enumerate(ax):
+            b = ui * nant + vi
+            uvw[0, b] = [u, v, 0.0]
+            vis[0, b, 0, 0] = ft[ui, vi]
+    return g, vis, uvw, np.ones(vis.shape), freq, np.array([nant, nant]), np.array([cell, cell])
+
+
+def test_standard_grid_numpy_wrap_input_checked(gpu, oracle):
+    from astroviper_amd.gcf_prolate_spheroidal import (create_prolate_spheroidal_kernel,
+                                                       create_prolate_spheroidal_kernel_1D)
+    from astroviper_amd.ifft import ifft_uv_to_lm
+    from astroviper_amd.standard_grid import standard_grid_numpy_wrap_input_checked
+    g, vis, uvw, w, freq, gs, cs = point_source_setup()
+    cgk = create_prolate_spheroidal_kernel_1D(100, 7)
+    grid, sumwt = standard_grid_numpy_wrap_input_checked(
+        vis_data=vis, grid=np.array([]), sum_weight=np.array([]), uvw=uvw, weight=w,
+        freq_chan=freq, cgk_1D=cgk, image_size=gs, cell_size=cs, oversampling=100, support=7,
+        complex_grid=True, do_psf=False, chan_mode="continuum")
+    assert grid.dtype == np.complex128 and grid.shape == (1, 1, 200, 200)
+    _, corr = create_prolate_spheroidal_kernel(100, 7, gs)
+    dirty = ifft_uv_to_lm(grid, axes=[2, 3]) / corr * gs[0] * gs[1] / sumwt
+    for (x, y, thr) in g["dirty_thresholds"]:
+        assert dirty[0, 0, x, y] > thr
+    gref, swref = oracle.standard_grid(vis, uvw, w, freq, gs, cs)
+    assert np.max(np.abs(grid - gref)) <= 1e-6 * np.max(np.abs(gref))
+    grid, sumwt = standard_grid_numpy_wrap_input_checked(
+        vis_data=vis, grid=np.array([]), sum_weight=np.array([]), uvw=uvw, weight=w,
+        freq_chan=freq, cgk_1D=cgk, image_size=gs, cell_size=cs, do_psf=True)
+    psf = ifft_uv_to_lm(grid, axes=[2, 3]) / corr * gs[0] * gs[1] / sumwt
+    assert abs(psf[0, 0, 100, 100] - 1.0) < g["psf_centre_tol"]
+    # accumulation into a supplied grid doubles it (standard_grid.py:133-143)
+    g2, s2 = standard_grid_numpy_wrap_input_checked(
+        vis_data=vis, grid=grid.copy(), sum_weight=sumwt.copy(), uvw=uvw, weight=w,
+        freq_chan=freq, cgk_1D=cgk, image_size=gs, cell_size=cs, do_psf=True)
+    np.testing.assert_allclose(g2, 2 * grid, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(s2, 2 * sumwt, rtol=1e-12)
+
+
+def cube_dataset(nchan=16, npol=2, n=200, seed=2):
+    """A measurement set whose channel k carries the constant k + 1
+    (standard_gridding_example.py:23-52 on synthetic uvw)."""
+    from astroviper_amd import synth
+    from astroviper_amd.msv4 import make_visibility_dataset
+    arr = synth.ArrayConfig(n_ant=12, n_time=30, n_chan=nchan, n_pol=npol, seed=seed,
+                            freq_lo=1.0e9, freq_hi=1.05e9)
+    obs = synth.make_observation(arr, synth.SkyConfig())
+    maxuv = np.max(np.hypot(obs.uvw[..., 0], obs.uvw[..., 1]))
+    cell = 0.5 * C / obs.freq[-1] / maxuv
+    nt, nb = obs.uvw.shape[:2]
+    vis = np.ones((nt, nb, nchan, npol), complex) * (np.arange(nchan) + 1)[None, None, :, None]
+    ds = make_visibility_dataset(vis, obs.uvw.copy(), np.ones(vis.shape), np.zeros(vis.shape, bool),
+                                 obs.freq)
+    return cell, ds
+
+
+def test_grid2image_spheroid_ms4_cube_constant(gpu, oracle):
+    from astroviper_amd.standard_grid import grid2image_spheroid_ms4
+    cell, ms4 = cube_dataset()
+    dirty = np.zeros([16, 2, 200, 200])
+    grid2image_spheroid_ms4(vis=ms4, resid_array=dirty, pixelincr=np.array([-cell, cell]),
+                            support=7, sampling=100, dopsf=False, column="VISIBILITY",
+                            chan_mode="cube")
+    for k in range(16):
+        # test_standard_grid.py:126-132
+        assert round(dirty[k, 0, 100, 100] - np.round(float(k + 1)), 4) == 0
+    ref, _, _ = oracle.grid2image(ms4.VISIBILITY.data, ms4.UVW.data, ms4.WEIGHT.data,
+                                  ms4.FLAG.data, ms4.coords["frequency"].values, (200, 200),
+                                  (-cell, cell), chan_mode="cube")
+    assert np.max(np.abs(dirty - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+def test_grid2image_mutates_weights_and_checks_shapes(gpu):
+    from astroviper_amd.standard_grid import grid2image_spheroid_ms4
+    cell, ms4 = cube_dataset(nchan=4)
+    ms4.FLAG.data[0, :3] = True
+    ms4.UVW.data[1, 2, 0] = np.nan
+    dirty = np.zeros([1, 2, 200, 200])
+    grid2image_spheroid_ms4(ms4, dirty, np.array([-cell, cell]))
+    assert np.all(ms4.WEIGHT.data[0, :3] == 0) and np.all(ms4.WEIGHT.data[1, 2] == 0)
+    with pytest.raises(RuntimeError, match="Shapes of gridded vis and image do no match"):
+        grid2image_spheroid_ms4(ms4, np.zeros([2, 2, 200, 200]), np.array([-cell, cell]))
+    with pytest.raises(TypeError, match="not an xarray datatree"):
+        grid2image_spheroid_ms4([ms4, "not a dataset"], dirty, np.array([-cell, cell]))
+
+
+def test_grid2image_list_of_datasets_accumulates(gpu, oracle):
+    from astroviper_amd.standard_grid import grid2image_spheroid_ms4
+    cell, a = cube_dataset(nchan=3, seed=4)
+    _, b = cube_dataset(nchan=3, seed=5)
+    both = np.zeros([1, 2, 200, 200])
+    grid2image_spheroid_ms4([a, b], both, np.array([-cell, cell]))
+    uvw = np.concatenate([a.UVW.data, b.UVW.data], axis=1)
+    vis = np.concatenate([a.VISIBILITY.data, b.VISIBILITY.data], axis=1)
+    w = np.ones(vis.shape)
+    ref, _, _ = oracle.grid2image(vis, uvw, w, None, a.coords["frequency"].values, (200, 200),
+                                  (-cell, cell))
+    assert np.max(np.abs(both - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+def test_dgrid_numba_reference_call(gpu, oracle):
+    """test_standard_degrid.py:183-210 (dgrid_numba with scale = cell_size,
+    offset = grid_size // 2) on a constant grid of 2 -> max |vis| == 2.0."""
+    from astroviper_amd.gcf_prolate_spheroidal import create_prolate_spheroidal_kernel_1D
+    from astroviper_amd.standard_degrid import dgrid_numba
+    g = gold("dgrid_constant_grid")
+    npix, maxuv = g["npix"], g["maxUV"]
+    freq = np.array([g["freq"]])
+    cell = 0.5 * C / freq[0] / maxuv
+    rng = np.random.default_rng(9)
+    uvw = np.zeros((1, g["n_uvw"], 3))
+    uvw[0, :, :2] = rng.uniform(-maxuv, maxuv, size=(g["n_uvw"], 2))
+    grid = np.full((1, 1, npix, npix), g["grid_value"] + 0j)
+    vals = np.zeros((1, g["n_uvw"], 1, 1), dtype=np.complex128)
+    flag = np.zeros(vals.shape, bool)
+    conv = create_prolate_spheroidal_kernel_1D(100, 7)
+    dgrid_numba(uvw, np.zeros((1, g["n_uvw"])), vals, flag, np.array([cell, cell]),
+                np.array([npix, npix]) // 2, grid, freq, C, 7, 100, conv,
+                np.zeros(1, dtype=int), np.zeros(1, dtype=int))
+    assert np.max(np.abs(vals)) == g["expected_max_abs"]
+    # against the oracle on a random grid, same call
+    rg = rng.standard_normal((1, 1, npix, npix)) + 1j * rng.standard_normal((1, 1, npix, npix))
+    v1 = np.zeros_like(vals)
+    dgrid_numba(uvw, np.zeros((1, g["n_uvw"])), v1, flag, np.array([cell, cell]),
+                np.array([npix, npix]) // 2, rg, freq, C, 7, 100, conv,
+                np.zeros(1, dtype=int), np.zeros(1, dtype=int))
+    v2 = np.zeros_like(vals)
+    oracle.dgrid(uvw, v2, flag, np.array([cell, cell, 0]), np.array([npix // 2, npix // 2, 0]), rg,
+                 freq, 7, 100, conv, np.zeros(1, dtype=int), np.zeros(1, dtype=int))
+    assert np.max(np.abs(v1 - v2)) <= 1e-6 * np.max(np.abs(v2))
+
+
+def test_degrid_spheroid_ms4(gpu, oracle):
+    """test_standard_degrid.py:242-257 plus parity and the NaN-uvw mutation."""
+    from astroviper_amd.standard_degrid import degrid_spheroid_ms4
+    cell, ms = cube_dataset(nchan=3)
+    ms.UVW.data[2, 5, 1] = np.nan
+    ms.FLAG.data[4, :2, 1, 0] = True
+    const = np.full((1, 1, 200, 200), 2.0 + 0j)
+    degrid_spheroid_ms4(vis=ms, grid=const, pixelincr=np.array([cell, cell]), support=7,
+                        sampling=100, whichFunc=0)
+    m = ms.VISIBILITY_MODEL.data
+    assert np.max(m) == 2.0 and m.dtype == np.complex128
+    assert ms.VISIBILITY_MODEL.dims == ms.VISIBILITY.dims
+    assert np.all(ms.FLAG.data[2, 5]) and np.all(ms.UVW.data[2, 5] == 1e9)
+    assert np.all(m[2, 5] == 0) and np.all(m[4, :2, 1, 0] == 0)
+    rng = np.random.default_rng(10)
+    grid = rng.standard_normal((3, 2, 200, 200)) + 1j * rng.standard_normal((3, 2, 200, 200))
+    degrid_spheroid_ms4(ms, grid, np.array([cell, cell]))
+    ref = oracle.degrid(ms.UVW.data, ms.FLAG.data, ms.coords["frequency"].values, grid,
+                        (cell, cell))
+    assert np.max(np.abs(ms.VISIBILITY_MODEL.data - ref)) <= 1e-6 * np.max(np.abs(ref))
+    # incremental adds to the existing model (standard_degrid.py:540-551)
+    before = ms.VISIBILITY_MODEL.data.copy()
+    degrid_spheroid_ms4(ms, grid, np.array([cell, cell]), incremental=True)
+    np.testing.assert_allclose(ms.VISIBILITY_MODEL.data, 2 * before, rtol=1e-12, atol=1e-14)
+
+
+def test_make_psf_and_imager_calls(gpu, oracle):
+    from astroviper_amd.imager import (compute_residual_visibilities, grid_visibilities,
+                                       predict_model_visibilities)
+    from astroviper_amd.make_point_spread_function import make_psf
+    cell, ms = cube_dataset(nchan=2)
+    psf = make_psf(ms, {"cell_size": (cell, cell), "image_size": (200, 200),
+                        "phase_center": (0.0, 0.5), "chan_mode": "cube"},
+                   {"sampling": 100, "support": 7})
+    p = psf["POINT_SPREAD_FUNCTION"].data
+    assert p.shape == (1, 2, 2, 200, 200)
+    assert psf["POINT_SPREAD_FUNCTION"].dims == ("time", "frequency", "polarization", "l", "m")
+    pk = np.unravel_index(np.argmax(p[0, 0, 0]), (200, 200))
+    assert abs(pk[0] - 100) <= 1 and abs(pk[1] - 100) <= 1 and p.max() > 0.1
+    dirty = grid_visibilities(ms, (200, 200), (-cell, cell), chan_mode="cube")
+    assert dirty.shape == (2, 2, 200, 200)
+    model = np.zeros((2, 2, 200, 200))
+    model[:, :, 100, 100] = 1.0
+    predict_model_visibilities(ms, model, (cell, cell))
+    compute_residual_visibilities(ms)
+    assert ms.RESIDUAL.data.shape == ms.VISIBILITY.data.shape
+
+
+def test_fft_round_trip_and_known_answers(gpu):
+    """test_fft_ifft.py:9-24 and test_standard_image_grid_preparation.py:127-142."""
+    from astroviper_amd.fft import fft_lm_to_uv
+    from astroviper_amd.ifft import ifft_uv_to_lm
+    g = gold("fft")
+    sky = np.zeros((128, 128))
+    sky[64, 64] = 1
+    sky[50, 40] = 1
+    rt = ifft_uv_to_lm(fft_lm_to_uv(sky, (0, 1)), (0, 1))
+    assert np.max(np.abs(sky - rt)) < g["roundtrip_tol"]
+    uv = fft_lm_to_uv(np.ones((200, 200)), (0, 1))
+    assert abs(uv[100, 100] - g["ones_200_dc"]) < 1e-9
+    vis = np.zeros((200, 200), complex)
+    vis[100, 100] = 10
+    assert abs(ifft_uv_to_lm(vis, (0, 1)).max() - g["delta_10_max"]) < 1e-15
