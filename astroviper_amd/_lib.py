@@ -11,6 +11,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
+AV_ABI_VERSION = 2  # must equal AV_ABI_VERSION of include/avgrid.h
+
 AV_OK = 0
 AV_ERR_INVALID = -1
 AV_ERR_HIP = -2
@@ -81,6 +83,10 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        ver = L.av_version()
+        if ver != AV_ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {ver}, the bindings need "
+                               f"{AV_ABI_VERSION}: rebuild with `python -m astroviper_amd.build`")
         _lib = L
     return _lib
 

@@ -237,7 +237,7 @@ extern "C" {
 
 const char *av_last_error(void) { return g_last_error.c_str(); }
 
-int av_version(void) { return 100; }
+int av_version(void) { return AV_ABI_VERSION; }
 
 int av_device_info(int32_t *num_cus, int32_t *lds_bytes_per_cu, int32_t *xcds) {
   int dev = 0;

@@ -45,6 +45,10 @@ extern "C" {
                                AV_GRID_ADJOINT is then the exact adjoint of
                                av_degrid */
 
+/* ABI version: bumped whenever a signature changes; the bindings refuse a
+ * library whose version differs from the one they were written for. */
+#define AV_ABI_VERSION 2
+
 const char *av_last_error(void);
 int av_version(void);
 
@@ -172,11 +176,11 @@ int av_degrid_plain(const double *uvw, int64_t n_rows, int32_t n_chan,
 
 /* ---------------------------------------------- FFT pre/post passes --
  * The FFT itself is a library call (rocFFT through the host runtime).
- * av_ifftshift: out = ifftshift(in) over the last two axes of
- * n_planes complex planes [n_u][n_v] (np.fft.ifftshift, standard_grid.py:701),
- * converting from dtype to out_dtype (an FP32 grid can be transformed in
- * FP64: the taper correction divides by down to grdsf(1)^2 = 1.6e-5 at the
- * image edge, which magnifies FFT rounding). */
+ * av_ifftshift: out = ifftshift(in) over the last two axes of n_planes
+ * complex planes [n_u][n_v] (np.fft.ifftshift, standard_grid.py:701),
+ * converting from dtype to out_dtype: an FP32 grid can be transformed in
+ * FP64, because the taper correction divides by down to grdsf(1)^2 = 1.6e-5
+ * at the image edge, which magnifies FFT rounding.
  * av_normalise: the post-FFT step of grid2image_spheroid_ms4,
  * standard_grid.py:696-708: out[p] = real(fftshift(Y[p])) / taper * n_u *
  * n_v / sumwt[p], with taper the outer product of av_ps_correction_1d on

@@ -38,3 +38,10 @@ def test_error_path_without_gpu_work():
     assert b"support" in L.av_last_error()
     rc = L.av_plan_create(10, 1, 1, 64, 64, 1e-5, 1e-5, 7, 100, 0, 0, 0, 0, None)
     assert rc == _lib.AV_ERR_INVALID
+
+
+def test_abi_version_matches_header():
+    from astroviper_amd import _lib
+    src = open(os.path.join(ROOT, "include", "avgrid.h")).read()
+    m = re.search(r"#define AV_ABI_VERSION (\d+)", src)
+    assert int(m.group(1)) == _lib.AV_ABI_VERSION == _lib.lib().av_version()
