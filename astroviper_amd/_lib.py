@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 2  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 3  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -61,6 +61,8 @@ SIGNATURES = {
                                    _P, _I32, _I32, _D, _D, _I32, _I32, _P, _I32, _I32,
                                    _P]),
     "av_ifftshift": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P]),
+    "av_shift_taper": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I32, _D,
+                                  _I32, _I32, _I32, _I32, _P]),
     "av_normalise": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _I32, _P]),
 }
 

@@ -48,6 +48,38 @@ __global__ void k_normalise(const R *fft, const double *sumwt, const double *tu,
   }
 }
 
+// out[k] = scale * in[(k + shift) % n] (per axis), optionally divided by the
+// taper at the source or the destination index, real or complex in/out.
+template <typename R, typename O, bool IN_REAL, bool OUT_REAL>
+__global__ void k_shift_taper(const R *in, O *out, int64_t n_planes, int n_u, int n_v,
+                              int su, int sv, const double *tu, const double *tv, int taper,
+                              double scale) {
+  const int64_t plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / plane;
+    const int64_t c = i - p * plane;
+    const int ku = (int)(c / n_v), kv = (int)(c - (int64_t)ku * n_v);
+    int iu = ku + su;
+    iu -= (iu >= n_u) ? n_u : 0;
+    int iv = kv + sv;
+    iv -= (iv >= n_v) ? n_v : 0;
+    const int64_t src = p * plane + (int64_t)iu * n_v + iv;
+    double re = IN_REAL ? (double)in[src] : (double)in[2 * src];
+    double im = IN_REAL ? 0.0 : (double)in[2 * src + 1];
+    double f = scale;
+    if (taper == 1) f /= tu[iu] * tv[iv];
+    if (taper == 2) f /= tu[ku] * tv[kv];
+    if (OUT_REAL) {
+      out[i] = (O)(re * f);
+    } else {
+      out[2 * i] = (O)(re * f);
+      out[2 * i + 1] = (O)(im * f);
+    }
+  }
+}
+
 static unsigned grid_blocks(int64_t total) {
   int64_t b = (total + 255) / 256;
   if (b > 65536) b = 65536;
@@ -85,6 +117,50 @@ int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u, int32
   else
     return fail(AV_ERR_INVALID, "av_ifftshift: bad dtype");
   AV_LAUNCH_CHECK("k_ifftshift");
+  return AV_OK;
+}
+
+int av_shift_taper(const void *in, void *out, int64_t n_planes, int32_t n_u, int32_t n_v,
+                   int32_t shift_u, int32_t shift_v, const double *taper_u,
+                   const double *taper_v, int32_t taper, double scale, int32_t in_dtype,
+                   int32_t in_real, int32_t out_dtype, int32_t out_real, void *stream) {
+  if (!in || !out || in == out || n_planes < 0 || n_u < 1 || n_v < 1 || shift_u < 0 ||
+      shift_u >= n_u || shift_v < 0 || shift_v >= n_v || taper < 0 || taper > 2 ||
+      (taper && (!taper_u || !taper_v)))
+    return fail(AV_ERR_INVALID, "av_shift_taper: bad arguments");
+  const int64_t total = n_planes * n_u * n_v;
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g(grid_blocks(total)), b(256);
+#define AV_ST(R, O, IR, OR_)                                                                  \
+  hipLaunchKernelGGL((k_shift_taper<R, O, IR, OR_>), g, b, 0, s, (const R *)in, (O *)out,      \
+                     n_planes, n_u, n_v, shift_u, shift_v, taper_u, taper_v, taper, scale)
+  const bool f32i = in_dtype == AV_F32, f32o = out_dtype == AV_F32;
+  if ((in_dtype != AV_F32 && in_dtype != AV_F64) || (out_dtype != AV_F32 && out_dtype != AV_F64))
+    return fail(AV_ERR_INVALID, "av_shift_taper: bad dtype");
+  if (f32i && f32o) {
+    if (in_real && out_real) AV_ST(float, float, true, true);
+    else if (in_real) AV_ST(float, float, true, false);
+    else if (out_real) AV_ST(float, float, false, true);
+    else AV_ST(float, float, false, false);
+  } else if (f32i) {
+    if (in_real && out_real) AV_ST(float, double, true, true);
+    else if (in_real) AV_ST(float, double, true, false);
+    else if (out_real) AV_ST(float, double, false, true);
+    else AV_ST(float, double, false, false);
+  } else if (f32o) {
+    if (in_real && out_real) AV_ST(double, float, true, true);
+    else if (in_real) AV_ST(double, float, true, false);
+    else if (out_real) AV_ST(double, float, false, true);
+    else AV_ST(double, float, false, false);
+  } else {
+    if (in_real && out_real) AV_ST(double, double, true, true);
+    else if (in_real) AV_ST(double, double, true, false);
+    else if (out_real) AV_ST(double, double, false, true);
+    else AV_ST(double, double, false, false);
+  }
+#undef AV_ST
+  AV_LAUNCH_CHECK("k_shift_taper");
   return AV_OK;
 }
 

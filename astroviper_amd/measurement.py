@@ -1,0 +1,104 @@
+"""Differentiable measurement operator: sky image -> visibilities.
+
+    A x = D F (x / t)
+
+t is the gridding-correction taper (the outer product of grdsf on both
+axes, gcf_prolate_spheroidal.py:72-79), F = fftshift . fft2 . ifftshift
+(fft_lm_to_uv, core/imaging/fft.py:5-18) and D the normalised degridder
+(dgrid_numba, standard_degrid.py:388-485).  The reference's predictor
+(imager.py:88-105) is F then D without the taper; dividing by the taper
+first makes D F (x / t) the model visibilities of the sky x.
+
+The adjoint is A^H y = (F^H D^H y) / t with F^H = fftshift . (N^2 ifft2)
+. ifftshift and D^H the gridding kernel weighting each sample by 1/norm
+(av_grid with AV_GRID_ADJOINT on a plan whose gridding geometry is the
+degridder's), so grid and degrid are exact transposes of each other.
+`MeasurementFunction` wraps A as a torch.autograd.Function whose backward is
+that adjoint (the real part for a real image).
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import torch
+
+from . import device as _dev
+from ._lib import call
+
+
+class MeasurementOperator:
+    """A = D F (. / t) for fixed uvw; images [n_img_chan, n_pol, n_u, n_v]."""
+
+    def __init__(self, uvw: torch.Tensor, freq: torch.Tensor, n_uv, cell, n_pol: int = 1,
+                 chan_map: Optional[torch.Tensor] = None, n_img_chan: int = 1,
+                 support: int = 7, oversampling: int = 100, dtype=torch.complex64,
+                 tile=(32, 32), device=None):
+        self.device = _dev.require_gpu(device)
+        self.n_u, self.n_v = int(n_uv[0]), int(n_uv[1])
+        self.n_pol = int(n_pol)
+        self.n_img_chan = int(n_img_chan)
+        self.cdtype = dtype
+        self.code = _dev._real_code(dtype)
+        rows = uvw.shape[0]
+        nchan = freq.numel()
+        self.plan = _dev.GridPlan(rows, nchan, n_img_chan, n_uv, cell, support, oversampling,
+                                  tile=tile, geometry="degrid", device=self.device)
+        self.plan.set_uvw(uvw, freq, chan_map)
+        self.tu = _dev.ps_correction_1d(self.n_u, self.device)
+        self.tv = _dev.ps_correction_1d(self.n_v, self.device)
+        self.shape = (self.n_img_chan, self.n_pol, self.n_u, self.n_v)
+        self.n_samples = self.plan.n_samples
+
+    def _pass(self, src, dst, shift_u, shift_v, taper, scale, src_real, dst_real):
+        planes = src.numel() // (self.n_u * self.n_v)
+        call("av_shift_taper", _dev._ptr(src), _dev._ptr(dst), planes, self.n_u, self.n_v,
+             shift_u, shift_v, _dev._ptr(self.tu), _dev._ptr(self.tv), taper, float(scale),
+             self.code, int(src_real), self.code, int(dst_real), _dev._stream())
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        """Model visibilities [samples, n_pol] of image x (real or complex)."""
+        x = x.reshape(self.shape).contiguous()
+        real = not x.is_complex()
+        rdt = _dev.real_dtype(self.code)
+        x = x.to(rdt if real else self.cdtype)
+        w = torch.empty(self.shape, dtype=self.cdtype, device=self.device)
+        # ifftshift, dividing by the taper at the source pixel
+        self._pass(x, w, self.n_u // 2, self.n_v // 2, 1, 1.0, real, False)
+        torch.fft.fft2(w, dim=(-2, -1), out=w)
+        g = torch.empty_like(w)
+        self._pass(w, g, self.n_u - self.n_u // 2, self.n_v - self.n_v // 2, 0, 1.0, False, False)
+        return self.plan.degrid(g, n_pol=self.n_pol)
+
+    def adjoint(self, y: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """A^H y for visibilities y [samples, n_pol]; real part if `real`."""
+        y = y.reshape(self.n_samples, self.n_pol).to(self.cdtype).contiguous()
+        g, _ = self.plan.grid(y, None, adjoint=True, n_pol=self.n_pol)
+        w = torch.empty_like(g)
+        self._pass(g, w, self.n_u // 2, self.n_v // 2, 0, 1.0, False, False)
+        torch.fft.ifft2(w, dim=(-2, -1), norm="forward", out=w)  # N^2 * ifft2
+        out = torch.empty(self.shape, dtype=_dev.real_dtype(self.code) if real else self.cdtype,
+                          device=self.device)
+        self._pass(w, out, self.n_u - self.n_u // 2, self.n_v - self.n_v // 2, 2, 1.0, False,
+                   real)
+        return out
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        """Differentiable A x."""
+        return MeasurementFunction.apply(x, self)
+
+
+class MeasurementFunction(torch.autograd.Function):
+    """y = A x with the exact adjoint as backward."""
+
+    @staticmethod
+    def forward(ctx, x, op: MeasurementOperator):
+        ctx.op = op
+        ctx.real = not x.is_complex()
+        ctx.shape = x.shape
+        ctx.dtype = x.dtype
+        return op.forward(x.detach())
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        g = ctx.op.adjoint(grad_y, real=ctx.real)
+        return g.reshape(ctx.shape).to(ctx.dtype), None

@@ -47,7 +47,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 2
+#define AV_ABI_VERSION 3
 
 const char *av_last_error(void);
 int av_version(void);
@@ -187,6 +187,18 @@ int av_degrid_plain(const double *uvw, int64_t n_rows, int32_t n_chan,
  * both axes; out is float64 (out_dtype AV_F64) or float32 (AV_F32). */
 int av_ifftshift(const void *in, void *out, int64_t n_planes, int32_t n_u,
                  int32_t n_v, int32_t dtype, int32_t out_dtype, void *stream);
+/* General shift + taper + scale pass used by the measurement operator
+ * (image -> taper -> FFT -> degrid, and its adjoint; no reference
+ * counterpart, imager.py:88-105 is the untapered forward path):
+ * out[k] = scale * in[(k + shift) mod n] per axis, divided by the taper at
+ * the source index (taper 1) or destination index (taper 2), 0 = no taper;
+ * in/out real (flag 1) or complex (0) of dtype AV_F32/AV_F64; the real part
+ * is taken when a complex input goes to a real output. */
+int av_shift_taper(const void *in, void *out, int64_t n_planes, int32_t n_u,
+                   int32_t n_v, int32_t shift_u, int32_t shift_v,
+                   const double *taper_u, const double *taper_v, int32_t taper,
+                   double scale, int32_t in_dtype, int32_t in_real,
+                   int32_t out_dtype, int32_t out_real, void *stream);
 int av_normalise(const void *fft_planes, const double *sumwt,
                  const double *taper_u, const double *taper_v,
                  int64_t n_planes, int32_t n_u, int32_t n_v, int32_t dtype,

@@ -359,3 +359,70 @@ void ora_degrid_geometry(const double uvw[3], double freq, const double scale[2]
   loc_off[2] = (int32_t)round_half_even(((double)l0 - p0) * (double)sampling);
   loc_off[3] = (int32_t)round_half_even(((double)l1 - p1) * (double)sampling);
 }
+
+/* Transpose of ora_dgrid (no reference function: the measurement operator's
+ * backward).  For every unflagged sample with dgrid_numba's geometry,
+ * grid[x0+ix, y0+iy] += wt * values / norm with wt and norm exactly as in
+ * ora_dgrid, so <dgrid(g), y> = <g, ora_grid_adjoint(y)> to rounding. */
+void ora_grid_adjoint(const double *uvw, const double *values, const uint8_t *flag,
+                      int nt, int nb, int nvischan, int nvispol, const double *scale,
+                      const double *offset, double *grid, int nchan_g, int npol_g, int g2,
+                      int g3, const double *freq, double c, int support, int sampling,
+                      const double *convFunc, const int64_t *chanmap, const int64_t *polmap) {
+  const int supp_centre = support / 2;
+  const int supp_beg = -supp_centre;
+  const int supp_end = supp_beg + support;
+  const int64_t ny = g2, nx = g3;
+  for (int t = 0; t < nt; ++t) {
+    for (int b = 0; b < nb; ++b) {
+      const int64_t row = (int64_t)t * nb + b;
+      for (int ipol = 0; ipol < nvispol; ++ipol) {
+        int64_t apol = polmap[ipol];
+        if (!(0 <= apol && apol < npol_g)) continue;
+        for (int ichan = 0; ichan < nvischan; ++ichan) {
+          int64_t achan = chanmap[ichan];
+          const int64_t vi = ((row * nvischan) + ichan) * nvispol + ipol;
+          if (!(0 <= achan && achan < nchan_g && !(flag && flag[vi]))) continue;
+          double pos0 = scale[0] * uvw[row * 3 + 0] * freq[ichan] / c + offset[0];
+          double pos1 = scale[1] * uvw[row * 3 + 1] * freq[ichan] / c + offset[1];
+          double r0 = round_half_even(pos0), r1 = round_half_even(pos1);
+          if (!(r0 > -2.0e9 && r0 < 2.0e9 && r1 > -2.0e9 && r1 < 2.0e9)) continue;
+          int32_t loc0 = (int32_t)r0, loc1 = (int32_t)r1;
+          int32_t off0 = (int32_t)round_half_even(((double)loc0 - pos0) * (double)sampling);
+          int32_t off1 = (int32_t)round_half_even(((double)loc1 - pos1) * (double)sampling);
+          int64_t x0 = loc0, y0 = loc1;
+          if (!((x0 + supp_beg >= 0) && ((x0 + supp_end - 1) < nx) && (y0 + supp_beg >= 0) &&
+                ((y0 + supp_end - 1) < ny)))
+            continue;
+          if (!((x0 + supp_end - 1) < g2 && (y0 + supp_end - 1) < g3)) continue;
+          double *g = grid + ((achan * npol_g) + apol) * (int64_t)g2 * g3 * 2;
+          double norm = 0.0;
+          for (int ix = supp_beg; ix < supp_end; ++ix) {
+            int64_t il1 = (int64_t)sampling * ix + off0;
+            if (il1 < 0) il1 = -il1;
+            for (int iy = supp_beg; iy < supp_end; ++iy) {
+              int64_t il2 = (int64_t)sampling * iy + off1;
+              if (il2 < 0) il2 = -il2;
+              norm += convFunc[il1] * convFunc[il2];
+            }
+          }
+          if (norm == 0.0) continue;
+          double yr = values[2 * vi] / norm, yi = values[2 * vi + 1] / norm;
+          for (int ix = supp_beg; ix < supp_end; ++ix) {
+            int64_t il1 = (int64_t)sampling * ix + off0;
+            if (il1 < 0) il1 = -il1;
+            double wtx = convFunc[il1];
+            for (int iy = supp_beg; iy < supp_end; ++iy) {
+              int64_t il2 = (int64_t)sampling * iy + off1;
+              if (il2 < 0) il2 = -il2;
+              double wt = wtx * convFunc[il2];
+              int64_t cell = (x0 + ix) * g3 + (y0 + iy);
+              g[2 * cell] += wt * yr;
+              g[2 * cell + 1] += wt * yi;
+            }
+          }
+        }
+      }
+    }
+  }
+}

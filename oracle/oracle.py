@@ -55,6 +55,9 @@ def lib():
         L.ora_grid_geometry.restype = ctypes.c_int
         L.ora_degrid_geometry.argtypes = [dp, D, dp, dp, I, dp, i64p]
         L.ora_degrid_geometry.restype = None
+        L.ora_grid_adjoint.argtypes = [dp, dp, u8p, I, I, I, I, dp, dp, dp, I, I, I, I, dp, D,
+                                       I, I, dp, i64p, i64p]
+        L.ora_grid_adjoint.restype = None
         for f in (L.ora_ps_function, L.ora_ps_kernel_1d, L.ora_ps_taper,
                   L.ora_standard_grid, L.ora_dgrid, L.ora_normalise):
             f.restype = None
@@ -278,3 +281,55 @@ def degrid(uvw, flag, freq, grid, cell, support=7, sampling=100,
     polmap = np.round(np.arange(npl) % npol).astype(int)
     return dgrid(uvw, model, flag, scale, offset, grid, freq, support,
                  sampling, conv, chanmap, polmap)
+
+
+# --------------------------------------------------------------------------
+# Measurement operator  A x = D F (x / t)  and its adjoint (no reference
+# function; built from dgrid_numba, fft_lm_to_uv and the taper)
+# --------------------------------------------------------------------------
+def _degrid_setup(n_uv, cell, nf, nchan_g, npol_g, npl):
+    nx, ny = n_uv
+    scale = np.array([-cell[0] * nx, -cell[1] * ny, 0.0])
+    offset = np.array([nx / 2, ny / 2, 0.0])
+    chanmap = (np.arange(nf) / nf * nchan_g).astype(np.int64)
+    polmap = (np.arange(npl) % npol_g).astype(np.int64)
+    return scale, offset, chanmap, polmap
+
+
+def grid_adjoint(uvw, values, flag, freq, n_uv, cell, nchan_g, npol_g, support=7,
+                 sampling=100):
+    """Transpose of degrid(): D^H y on a [nchan_g, npol_g, nx, ny] grid."""
+    values = np.ascontiguousarray(values, dtype=np.complex128)
+    nt, nb, nf, npl = values.shape
+    scale, offset, chanmap, polmap = _degrid_setup(n_uv, cell, nf, nchan_g, npol_g, npl)
+    grid = np.zeros((nchan_g, npol_g, int(n_uv[0]), int(n_uv[1])), dtype=np.complex128)
+    conv = create_prolate_spheroidal_kernel_1D(sampling, support)
+    fl = np.zeros(values.shape, np.uint8) if flag is None else np.ascontiguousarray(flag, np.uint8)
+    u = np.ascontiguousarray(uvw, dtype=np.float64)
+    f = np.ascontiguousarray(freq, dtype=np.float64)
+    lib().ora_grid_adjoint(_dp(u), _dp(values.view(np.float64)),
+                           fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), nt, nb, nf, npl,
+                           _dp(scale), _dp(offset), _dp(grid.view(np.float64)), nchan_g, npol_g,
+                           grid.shape[2], grid.shape[3], _dp(f), C_LIGHT, int(support),
+                           int(sampling), _dp(conv), _i64p(chanmap), _i64p(polmap))
+    return grid
+
+
+def measure(x, uvw, freq, cell, npol=1, support=7, sampling=100):
+    """A x for an image x [nchan_g, npol_g, nx, ny]: degrid(fft_lm_to_uv(x / t))."""
+    x = np.asarray(x)
+    n_uv = x.shape[-2:]
+    _, t = create_prolate_spheroidal_kernel(sampling, support, np.array(n_uv))
+    g = fft_lm_to_uv(x / t, axes=[2, 3])
+    nt, nb = uvw.shape[:2]
+    flag = np.zeros((nt, nb, len(freq), npol), bool)
+    return degrid(uvw, flag, freq, g, cell, support, sampling)
+
+
+def measure_adjoint(y, uvw, freq, n_uv, cell, nchan_g=1, npol_g=1, support=7, sampling=100):
+    """A^H y = fftshift(N^2 ifft2(ifftshift(D^H y))) / t."""
+    g = grid_adjoint(uvw, y, None, freq, n_uv, cell, nchan_g, npol_g, support, sampling)
+    _, t = create_prolate_spheroidal_kernel(sampling, support, np.array(n_uv))
+    nn = n_uv[0] * n_uv[1]
+    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(g, axes=[2, 3]), axes=[2, 3]),
+                           axes=[2, 3]) * nn / t

@@ -356,3 +356,35 @@ def test_fft_known_answers(oracle):
     vis = np.zeros((200, 200), complex)
     vis[100, 100] = 10
     assert abs(oracle.ifft_uv_to_lm(vis, (0, 1)).max() - 10.0 / 200 / 200) < 1e-15
+
+
+# ------------------------------------------------------- measurement operator
+def test_oracle_grid_adjoint_is_transpose_of_degrid(oracle):
+    rng = np.random.default_rng(18)
+    n = 40
+    cell = (-0.5 * C / 1.2e9 / 1500.0, 0.5 * C / 1.2e9 / 1500.0)
+    uvw = np.zeros((2, 60, 3))
+    uvw[..., :2] = rng.uniform(-1200, 1200, size=(2, 60, 2))
+    freq = np.array([1.0e9, 1.2e9])
+    g = rng.standard_normal((1, 2, n, n)) + 1j * rng.standard_normal((1, 2, n, n))
+    y = rng.standard_normal((2, 60, 2, 2)) + 1j * rng.standard_normal((2, 60, 2, 2))
+    flag = rng.random(y.shape) < 0.2
+    dg = oracle.degrid(uvw, flag, freq, g, cell)
+    dhy = oracle.grid_adjoint(uvw, y, flag, freq, (n, n), cell, 1, 2)
+    lhs = np.vdot(y, dg)
+    rhs = np.vdot(dhy, g)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(dg) * np.linalg.norm(y)
+
+
+def test_oracle_measurement_operator_adjoint(oracle):
+    rng = np.random.default_rng(19)
+    n = 32
+    cell = (-0.5 * C / 1.2e9 / 1200.0, 0.5 * C / 1.2e9 / 1200.0)
+    uvw = np.zeros((1, 200, 3))
+    uvw[..., :2] = rng.uniform(-1000, 1000, size=(1, 200, 2))
+    freq = np.array([1.2e9])
+    x = rng.standard_normal((1, 1, n, n)) + 1j * rng.standard_normal((1, 1, n, n))
+    y = rng.standard_normal((1, 200, 1, 1)) + 1j * rng.standard_normal((1, 200, 1, 1))
+    ax = oracle.measure(x, uvw, freq, cell)
+    ahy = oracle.measure_adjoint(y, uvw, freq, (n, n), cell)
+    assert abs(np.vdot(y, ax) - np.vdot(ahy, x)) <= 1e-12 * np.linalg.norm(ax) * np.linalg.norm(y)
