@@ -140,11 +140,15 @@ struct PlanDev {
   uint32_t *rec_idx, *rec_g, *rec_d;  // [max_records]
   // work units: x = bin, y = first record, z = count, w = flags
   uint4 *gunits, *dunits;
-  // zero list: samples no degrid visits
-  uint32_t *zero_idx;               // [n_items]
-  // counters: [0] valid-grid, [1] valid-degrid, [2] zero count,
-  // [3] grid work cursor, [4] degrid work cursor, [5] zero cursor
-  uint32_t *counters;
+  // validity bitmasks, one bit per sample: grid-valid, degrid-valid
+  unsigned long long *gmask, *dmask;  // [n_words]
+  int64_t n_words;
+  // per-call tables: FP32 kernel table, FP64 per-axis kernel sums by offset
+  float *tab32;                     // [ov * (support/2 + 1)]
+  double *nrm;                      // [2 * (ov/2 + 2) + 1]
+  // per grid unit, per polarisation: sum of weight * norm (FP64)
+  double *upart;                    // [max_gunits * 4]
+  uint32_t *counters;               // scratch for statistics
 };
 
 constexpr uint32_t kUnitMulti = 1u;  // bin split over several units
@@ -156,6 +160,15 @@ struct av_plan_host;  // defined in plan.hip
 int launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n,
                        uint32_t *tmp, hipStream_t s);
+int prepare_tables(const PlanDev &d, const double *cgk, hipStream_t s);
+
+// properties of the current device, queried once per device
+struct DeviceProps {
+  hipError_t error;
+  int num_cus;
+  size_t lds_per_block;
+};
+const DeviceProps &device_props();
 
 }  // namespace av
 

@@ -21,6 +21,31 @@ int hip_fail(hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? AV_ERR_NOMEM : AV_ERR_HIP;
 }
 
+const DeviceProps &device_props() {
+  static std::mutex mu;
+  static DeviceProps cache[64];
+  static bool have[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  static DeviceProps bad;
+  if (e != hipSuccess || dev < 0 || dev >= 64) {
+    bad.error = e != hipSuccess ? e : hipErrorInvalidDevice;
+    return bad;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!have[dev]) {
+    hipDeviceProp_t prop;
+    DeviceProps p;
+    p.error = hipGetDeviceProperties(&prop, dev);
+    p.num_cus = prop.multiProcessorCount;
+    p.lds_per_block = prop.sharedMemPerBlock;
+    cache[dev] = p;
+    have[dev] = p.error == hipSuccess;
+    if (!have[dev]) return cache[dev];
+  }
+  return cache[dev];
+}
+
 // ------------------------------------------------------------- fill ----
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -1,13 +1,13 @@
 // degrid.hip -- tiled degridding (replaces dgrid_numba,
 // core/imaging/imaging_utils/standard_degrid.py:388-485, zero dphase).
 //
-// A persistent kernel pulls degrid units (primary records of one bin).  The
-// unit copies its tile plus a halo of support/2 + 1 pixels of every grid
-// polarisation plane into LDS with coalesced reads, then each wavefront
-// evaluates samples: lanes = (component, v-tap) read one LDS row per u-tap
-// (bank-conflict free), multiply by the u and v kernel values, and a DPP
-// reduction over the v-taps gives the sample, divided by the kernel-weight
-// sum as in the reference.  Each sample is written exactly once.
+// One workgroup per degrid unit (primary records of one bin).  The unit
+// copies its tile plus a halo of support/2 + 1 pixels of every grid
+// polarisation plane into LDS (one wavefront per row, coalesced), then each
+// wavefront evaluates samples: lanes = (component, v-tap) read one LDS row
+// per u-tap (bank-conflict free), multiply by the u and v kernel values, and
+// a DPP reduction over the v-taps gives the sample, divided by the
+// kernel-weight sum as in the reference.  Each sample is written exactly once.
 #include <algorithm>
 
 #include "tile_common.h"
@@ -18,7 +18,7 @@ constexpr int kDegridThreads = 256;
 constexpr int kDegridWaves = kDegridThreads / 64;
 
 struct DegridLds {
-  int tab, reg, st_cu, st_cv, st_pos, st_item, s_unit, total;
+  int tab, reg, st_cu, st_cv, st_pos, st_item, total;
   int ntab, halo, ru, rv, rps;
 };
 
@@ -44,7 +44,6 @@ __host__ __device__ inline DegridLds degrid_lds_layout(int ov, int support, int 
   L.st_cv = take(kDegridWaves * 64 * DVW * (int)sizeof(R));
   L.st_pos = take(kDegridWaves * 64 * (int)sizeof(int));
   L.st_item = take(kDegridWaves * 64 * (int)sizeof(uint32_t));
-  L.s_unit = take(16);
   L.total = o;
   return L;
 }
@@ -56,6 +55,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
     int mode) {
   using LM = LaneMap<NPOL, DVW>;
   extern __shared__ __align__(16) char smem[];
+  const uint32_t u = blockIdx.x;
+  if (u >= P.dunit_start[P.n_bins]) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
   const DegridLds L = degrid_lds_layout<R, DVW>(ov, S, Tu, Tv, npol_g);
@@ -66,12 +67,11 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   R *st_cv = (R *)(smem + L.st_cv) + wave * 64 * DVW;
   int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
   uint32_t *st_item = (uint32_t *)(smem + L.st_item) + wave * 64;
-  uint32_t *s_unit = (uint32_t *)(smem + L.s_unit);
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
   const int H = L.halo;
 
-  for (int i = tid; i < L.ntab; i += kDegridThreads) tab[i] = (R)cgk[i];
-  const uint32_t n_units = P.dunit_start[P.n_bins];
+  for (int i = tid; i < L.ntab; i += kDegridThreads)
+    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
 
   const int slot = lane / LM::kLanesPerItem;
   const int lq = (lane % LM::kLanesPerItem) / DVW;
@@ -86,122 +86,120 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
     gq[k] = 2 * pg + (q & 1);
   }
 
-  for (;;) {
-    __syncthreads();
-    if (tid == 0) *s_unit = atomicAdd(&P.counters[4], 1u);
-    __syncthreads();
-    const uint32_t u = *s_unit;
-    if (u >= n_units) break;
-    const uint4 unit = P.dunits[u];
-    const int64_t b = unit.x;
-    const uint32_t first = unit.y, cnt = unit.z;
-    const int ic = (int)(b % P.n_img_chan);
-    const int64_t tile = b / P.n_img_chan;
-    const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
-    const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
-    const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+  const uint4 unit = P.dunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, cnt = unit.z;
+  const int ic = (int)(b % P.n_img_chan);
+  const int64_t tile = b / P.n_img_chan;
+  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
-    // ---- tile + halo of every grid polarisation into LDS (zero outside)
-    const int rcells = L.ru * L.rv;
-    for (int idx = tid; idx < npol_g * rcells; idx += kDegridThreads) {
-      const int p = idx / rcells, c = idx - p * rcells;
-      const int ru = c / L.rv, rv = c - ru * L.rv;
-      const int gu = u0 - H + ru, gv = v0 - H + rv;
+  // ---- tile + halo of every grid polarisation into LDS (zero outside);
+  // one wavefront per region row, lanes along the row
+  for (int row = wave; row < npol_g * L.ru; row += kDegridWaves) {
+    const int p = row / L.ru, ru = row - p * L.ru;
+    const int gu = u0 - H + ru;
+    const bool urow = gu >= 0 && gu < P.n_u;
+    const R *src = gplane + p * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2;
+    R *dre = reg + (2 * p) * L.rps + ru * L.rv;
+    R *dim = reg + (2 * p + 1) * L.rps + ru * L.rv;
+    for (int rv = lane; rv < L.rv; rv += 64) {
+      const int gv = v0 - H + rv;
       R re = (R)0, im = (R)0;
-      if (gu >= 0 && gu < P.n_u && gv >= 0 && gv < P.n_v) {
-        const R *src = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
-        re = src[0];
-        im = src[1];
+      if (urow && gv >= 0 && gv < P.n_v) {
+        re = src[(size_t)gv * 2];
+        im = src[(size_t)gv * 2 + 1];
       }
-      reg[(2 * p) * L.rps + c] = re;
-      reg[(2 * p + 1) * L.rps + c] = im;
+      dre[rv] = re;
+      dim[rv] = im;
     }
-    __syncthreads();
+  }
+  __syncthreads();
 
-    const uint32_t end = first + cnt;
-    for (uint32_t r0 = first; r0 < end; r0 += kDegridThreads) {
-      const uint32_t r = r0 + wave * 64 + lane;
-      int pos = -1;
-      if (r < end) {
-        const uint32_t g = P.rec_d[r];
-        if (g != kInvalidGeo) {
-          int au, ou, avv, ovv;
-          unpack_geo(g, au, ou, avv, ovv);
-          pos = (au + kPosBias) | ((avv + kPosBias) << 16);
-          st_item[lane] = P.rec_idx[r];
+  const uint32_t end = first + cnt;
+  for (uint32_t r0 = first; r0 < end; r0 += kDegridThreads) {
+    const uint32_t r = r0 + wave * 64 + lane;
+    int pos = -1;
+    if (r < end) {
+      const uint32_t g = P.rec_d[r];
+      if (g != kInvalidGeo) {
+        int au, ou, avv, ovv;
+        unpack_geo(g, au, ou, avv, ovv);
+        pos = (au + kPosBias) | ((avv + kPosBias) << 16);
+        st_item[lane] = P.rec_idx[r];
 #pragma unroll
-          for (int t = 0; t < DVW; ++t) {
-            st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
-            st_cv[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
-          }
+        for (int t = 0; t < DVW; ++t) {
+          st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
+          st_cv[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
         }
       }
-      st_pos[lane] = pos;
-      wave_lds_sync();
-      const int nst = (int)min((uint32_t)64, end > r0 + wave * 64 ? end - (r0 + wave * 64) : 0u);
-      for (int j0 = 0; j0 < nst; j0 += LM::kItemsPerIter) {
-        const int j = j0 + slot;
-        const int pj = j < nst ? st_pos[j] : -1;
-        // The kernel-weight sum is formed with exactly the operations applied
-        // to the data, so a constant grid degrids to that constant exactly
-        // (the reference sums wt*g and wt in the same loop).
-        R accv[LM::kQPerLane];
-        R nacc = (R)0;
-#pragma unroll
-        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = (R)0;
-        int au = 0, avv = 0;
-        if (pj >= 0) {
-          au = (pj & 0xFFFF) - kPosBias;
-          avv = (pj >> 16) - kPosBias;
-        }
-        if (pj >= 0 && dv < S) {
-          const int col = avv + H + dv - sc;
-          for (int du = 0; du < S; ++du) {
-            const R cu = st_cu[j * DVW + du];
-            const int row = (au + H + du - sc) * L.rv + col;
-            nacc += cu;
-#pragma unroll
-            for (int k = 0; k < LM::kQPerLane; ++k) accv[k] += cu * reg[gq[k] * L.rps + row];
-          }
-          const R cv = st_cv[j * DVW + dv];
-          nacc *= cv;
-#pragma unroll
-          for (int k = 0; k < LM::kQPerLane; ++k) accv[k] *= cv;
-        }
-        // every lane takes part in the DPP reductions
-#pragma unroll
-        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = group_sum_to_last<DVW>(accv[k]);
-        nacc = group_sum_to_last<DVW>(nacc);
-        if (pj >= 0 && dv == DVW - 1) {
-          const uint32_t item = st_item[j];
-          const R nv = nacc;
-#pragma unroll
-          for (int k = 0; k < LM::kQPerLane; ++k) {
-            const int q = lq + k * LM::kQStride;
-            const int p = q >> 1;
-            const size_t o = ((size_t)item * NPOL + p) * 2 + (q & 1);
-            const bool flagged = flag && flag[(size_t)item * NPOL + p];
-            if (incremental) {
-              if (!flagged) vis_out[o] += accv[k] / nv;
-            } else {
-              vis_out[o] = flagged ? (R)0 : accv[k] / nv;
-            }
-          }
-        }
-      }
-      wave_lds_sync();
     }
+    st_pos[lane] = pos;
+    wave_lds_sync();
+    const int nst = (int)min((uint32_t)64, end > r0 + wave * 64 ? end - (r0 + wave * 64) : 0u);
+    for (int j0 = 0; j0 < nst; j0 += LM::kItemsPerIter) {
+      const int j = j0 + slot;
+      const int pj = j < nst ? st_pos[j] : -1;
+      // The kernel-weight sum is formed with exactly the operations applied
+      // to the data, so a constant grid degrids to that constant exactly
+      // (the reference sums wt*g and wt in the same loop).
+      R accv[LM::kQPerLane];
+      R nacc = (R)0;
+#pragma unroll
+      for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = (R)0;
+      if (pj >= 0 && dv < S) {
+        const int au = (pj & 0xFFFF) - kPosBias;
+        const int avv = (pj >> 16) - kPosBias;
+        const int col = avv + H + dv - sc;
+        for (int du = 0; du < S; ++du) {
+          const R cu = st_cu[j * DVW + du];
+          const int row = (au + H + du - sc) * L.rv + col;
+          nacc += cu;
+#pragma unroll
+          for (int k = 0; k < LM::kQPerLane; ++k) accv[k] += cu * reg[gq[k] * L.rps + row];
+        }
+        const R cv = st_cv[j * DVW + dv];
+        nacc *= cv;
+#pragma unroll
+        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] *= cv;
+      }
+      // every lane takes part in the DPP reductions
+#pragma unroll
+      for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = group_sum_to_last<DVW>(accv[k]);
+      nacc = group_sum_to_last<DVW>(nacc);
+      if (pj >= 0 && dv == DVW - 1) {
+        const uint32_t item = st_item[j];
+#pragma unroll
+        for (int k = 0; k < LM::kQPerLane; ++k) {
+          const int q = lq + k * LM::kQStride;
+          const int p = q >> 1;
+          const size_t o = ((size_t)item * NPOL + p) * 2 + (q & 1);
+          const bool flagged = flag && flag[(size_t)item * NPOL + p];
+          if (incremental) {
+            if (!flagged) vis_out[o] += accv[k] / nacc;
+          } else {
+            vis_out[o] = flagged ? (R)0 : accv[k] / nacc;
+          }
+        }
+      }
+    }
+    wave_lds_sync();
   }
 }
 
-// non-incremental mode: samples no unit visits (off-grid, NaN, bad channel)
+// non-incremental mode: zero the samples no unit visits (off-grid, NaN uvw,
+// channel outside the grid), read from the plan's degrid-validity bitmask
 template <typename R>
-__global__ void k_zero_samples(PlanDev P, R *vis_out, int npol) {
-  const int64_t n = (int64_t)P.counters[2] * npol * 2;
+__global__ void k_zero_invalid(PlanDev P, R *vis_out, int npol) {
+  const int64_t n = P.n_items;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t z = i / (2 * npol), k = i - z * 2 * npol;
-    vis_out[(size_t)P.zero_idx[z] * npol * 2 + k] = (R)0;
+    const unsigned long long m = P.dmask[i >> 6];
+    if (!((m >> (i & 63)) & 1ull)) {
+      R *o = vis_out + (size_t)i * npol * 2;
+      for (int k = 0; k < 2 * npol; ++k) o[k] = (R)0;
+    }
   }
 }
 
@@ -212,27 +210,23 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
   PlanDev &d = plan->d;
   DegridLds L = degrid_lds_layout<R, DVW>(d.ov, d.support, d.tile_u, d.tile_v, npol_g);
   auto kfn = k_degrid_tiles<R, NPOL, DVW>;
-  int dev = 0;
-  AV_HIP(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  AV_HIP(hipGetDeviceProperties(&prop, dev));
-  if ((size_t)L.total > prop.sharedMemPerBlock)
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
+  if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_degrid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
   AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                              L.total));
-  int per_cu = 0;
-  AV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kDegridThreads, L.total));
-  if (per_cu < 1) per_cu = 1;
-  int blocks = per_cu * prop.multiProcessorCount;
-  int rc = launch_fill_u32(d.counters + 4, 0u, 1, s);
+  int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  if (!(mode & AV_DEGRID_INCREMENTAL)) {
-    hipLaunchKernelGGL(k_zero_samples<R>, dim3(2048), dim3(256), 0, s, d, (R *)vis_out, NPOL);
-    AV_LAUNCH_CHECK("k_zero_samples");
+  if (!(mode & AV_DEGRID_INCREMENTAL) && d.n_items > 0) {
+    int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero_invalid<R>, dim3((unsigned)blocks), dim3(256), 0, s, d,
+                       (R *)vis_out, NPOL);
+    AV_LAUNCH_CHECK("k_zero_invalid");
   }
-  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kDegridThreads), L.total, s, d, (const R *)grid,
-                     npol_g, pol_map, flag, cgk, (R *)vis_out, mode);
+  hipLaunchKernelGGL(kfn, dim3((unsigned)d.max_dunits), dim3(kDegridThreads), L.total, s, d,
+                     (const R *)grid, npol_g, pol_map, flag, cgk, (R *)vis_out, mode);
   AV_LAUNCH_CHECK("k_degrid_tiles");
   return AV_OK;
 }

@@ -1,15 +1,17 @@
 // grid.hip -- tiled convolutional gridding (replaces standard_grid_jit,
 // core/imaging/imaging_utils/standard_grid.py:246-432).
 //
-// One persistent kernel pulls work units (a bin, or a slice of a large bin)
-// from a device counter.  A unit owns a tile_u x tile_v tile of every
-// polarisation plane of one image channel.  The tile is accumulated in LDS
+// One workgroup per work unit (a bin, or a slice of a large bin); the launch
+// covers the plan's unit capacity and surplus workgroups exit at once.  A
+// unit owns a tile_u x tile_v tile of every polarisation plane of one image
+// channel.  The tile is accumulated in LDS
 // with LDS atomics (one wavefront per sample, lanes = (component, v-tap),
 // loop over u-taps; only taps inside the tile are added, which is why each
 // footprint is recorded in every tile it overlaps) and written to HBM once,
 // row by row, coalesced.  The kernel table sits in LDS.  A bin split over
 // several units adds its partial tiles with global atomics into a tile that
-// a pre-pass zeroed.  Sums of weights are FP64.
+// a pre-pass zeroed.  Sums of weights are FP64: each unit writes its
+// partial sums and a reduction pass adds them per plane.
 #include <algorithm>
 
 #include "tile_common.h"
@@ -69,8 +71,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
                                                              const R *__restrict__ weight,
                                                              const uint8_t *__restrict__ flag,
                                                              const double *__restrict__ cgk,
-                                                             R *__restrict__ grid,
-                                                             double *__restrict__ sumwt, int mode) {
+                                                             R *__restrict__ grid, int mode) {
   using LM = LaneMap<NPOL, DVW>;
   extern __shared__ __align__(16) char smem[];
   const int S = P.support, sc = S / 2, ov = P.ov;
@@ -85,29 +86,29 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   R *st_cv = (R *)(smem + L.st_cv) + wave * 64 * DVW;
   int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
   double *red = (double *)(smem + L.red);
-  uint32_t *s_unit = (uint32_t *)(red + kGridWaves * NPOL);
 
   const bool psf = (mode & AV_GRID_PSF) != 0;
   const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
   const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
   const int half = ov / 2 + 2;
 
-  for (int i = tid; i < L.ntab; i += kGridThreads) tab[i] = (R)cgk[i];
-  for (int i = tid; i < L.nnrm; i += kGridThreads) nrm[i] = axis_norm(cgk, ov, S, i - half);
-  const uint32_t n_units = P.gunit_start[P.n_bins];
+  const uint32_t u = blockIdx.x;
+  if (u >= P.gunit_start[P.n_bins]) return;
+  for (int i = tid; i < L.ntab; i += kGridThreads)
+    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+  for (int i = tid; i < L.nnrm; i += kGridThreads) nrm[i] = P.nrm[i];
   const int cells = Tu * Tv;
 
   // lane -> (slot, component, v-tap)
   const int slot = lane / LM::kLanesPerItem;
   const int lq = (lane % LM::kLanesPerItem) / DVW;
   const int dv = lane % DVW;
+  // flush layout: a thread owns one column of the tile when tile_v divides
+  // the workgroup, so the flush loops need no division
+  const bool rowwise = (kGridThreads % Tv) == 0;
+  const int fcol = tid % Tv, frow = tid / Tv, fstep = kGridThreads / Tv;
 
-  for (;;) {
-    __syncthreads();
-    if (tid == 0) *s_unit = atomicAdd(&P.counters[3], 1u);
-    __syncthreads();
-    const uint32_t u = *s_unit;
-    if (u >= n_units) break;
+  {
     const uint4 unit = P.gunits[u];
     const int64_t b = unit.x;
     const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
@@ -119,20 +120,36 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
     const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
     if (cnt == 0) {
+      if (tid < NPOL) P.upart[(size_t)u * 4 + tid] = 0.0;
       if (!accumulate) {
-        for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
-          int p = idx / cells, c = idx - p * cells;
-          int gu = u0 + c / Tv, gv = v0 + c % Tv;
-          if (gu < P.n_u && gv < P.n_v) {
-            Cplx<R> z{(R)0, (R)0};
-            *(Cplx<R> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) = z;
+        if (rowwise) {
+          for (int p = 0; p < NPOL; ++p)
+            for (int uu = frow; uu < Tu; uu += fstep) {
+              const int gu = u0 + uu, gv = v0 + fcol;
+              if (gu < P.n_u && gv < P.n_v) {
+                Cplx<R> z{(R)0, (R)0};
+                *(Cplx<R> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) = z;
+              }
+            }
+        } else {
+          for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+            int p = idx / cells, c = idx - p * cells;
+            int gu = u0 + c / Tv, gv = v0 + c % Tv;
+            if (gu < P.n_u && gv < P.n_v) {
+              Cplx<R> z{(R)0, (R)0};
+              *(Cplx<R> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) = z;
+            }
           }
         }
       }
-      continue;
+      return;
     }
 
-    for (int i = tid; i < 2 * NPOL * L.ps; i += kGridThreads) acc[i] = (R)0;
+    {
+      uint4 *a4 = (uint4 *)acc;
+      const int n16 = 2 * NPOL * L.ps * (int)sizeof(R) / 16;
+      for (int i = tid; i < n16; i += kGridThreads) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
     // compensated mode: per-thread running sums of the cells it flushes
     constexpr int kCpt = COMP ? 32 : 1;
     R csum[kCpt], ccmp[kCpt];
@@ -280,10 +297,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
       }
       __syncthreads();
     }
-    for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
-      const int p = idx / cells, c = idx - p * cells;
-      const int gu = u0 + c / Tv, gv = v0 + c % Tv;
-      if (gu >= P.n_u || gv >= P.n_v) continue;
+    auto flush_cell = [&](int p, int c, int gu, int gv) {
       Cplx<R> val{acc[(2 * p) * L.ps + c], acc[(2 * p + 1) * L.ps + c]};
       R *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
       if (multi) {
@@ -297,22 +311,64 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
       } else {
         *(Cplx<R> *)dst = val;
       }
+    };
+    if (rowwise) {
+      for (int p = 0; p < NPOL; ++p)
+        for (int uu = frow; uu < Tu; uu += fstep) {
+          const int gu = u0 + uu, gv = v0 + fcol;
+          if (gu < P.n_u && gv < P.n_v) flush_cell(p, uu * Tv + fcol, gu, gv);
+        }
+    } else {
+      for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+        const int p = idx / cells, c = idx - p * cells;
+        const int gu = u0 + c / Tv, gv = v0 + c % Tv;
+        if (gu < P.n_u && gv < P.n_v) flush_cell(p, c, gu, gv);
+      }
     }
-    // ---- sums of weights (FP64)
-    if (!adjoint) {
+    // ---- sums of weights (FP64): this unit's partial per polarisation
 #pragma unroll
-      for (int p = 0; p < NPOL; ++p) {
-        double s = sw[p];
-        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) red[wave * NPOL + p] = s;
-      }
-      __syncthreads();
-      if (tid < NPOL) {
-        double s = 0.0;
-        for (int w = 0; w < kGridWaves; ++w) s += red[w * NPOL + tid];
-        if (s != 0.0) atomicAdd(&sumwt[ic * NPOL + tid], s);
+    for (int p = 0; p < NPOL; ++p) {
+      double sv = adjoint ? 0.0 : sw[p];
+      for (int off = 32; off >= 1; off >>= 1) sv += __shfl_xor(sv, off);
+      if (lane == 0) red[wave * NPOL + p] = sv;
+    }
+    __syncthreads();
+    if (tid < NPOL) {
+      double sv = 0.0;
+      for (int w = 0; w < kGridWaves; ++w) sv += red[w * NPOL + tid];
+      P.upart[(size_t)u * 4 + tid] = sv;
+    }
+  }
+}
+
+// sums of weights per (image channel, polarisation) from the unit partials,
+// gathered in LDS (FP64 LDS atomics) and added to HBM once per workgroup
+__global__ __launch_bounds__(256) void k_sumwt_reduce(PlanDev P, int npol, double *sumwt,
+                                                     int use_lds) {
+  extern __shared__ double sacc[];
+  const int nplanes = P.n_img_chan * npol;
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  if (use_lds) {
+    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) sacc[i] = 0.0;
+    __syncthreads();
+  }
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_units;
+       u += gridDim.x * blockDim.x) {
+    const int ic = (int)(P.gunits[u].x % P.n_img_chan);
+    for (int p = 0; p < npol; ++p) {
+      const double v = P.upart[(size_t)u * 4 + p];
+      if (v != 0.0) {
+        if (use_lds)
+          atomicAdd(&sacc[ic * npol + p], v);
+        else
+          atomicAdd(&sumwt[ic * npol + p], v);
       }
     }
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nplanes; i += blockDim.x)
+      if (sacc[i] != 0.0) atomicAdd(&sumwt[i], sacc[i]);
   }
 }
 
@@ -351,24 +407,26 @@ static int launch_grid(av_plan *plan, const void *vis, const void *weight, const
   PlanDev &d = plan->d;
   GridLds L = grid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v);
   auto kfn = k_grid_tiles<R, NPOL, DVW, COMP>;
-  int dev = 0;
-  AV_HIP(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  AV_HIP(hipGetDeviceProperties(&prop, dev));
-  if ((size_t)L.total > prop.sharedMemPerBlock)
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
+  if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_grid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
   AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                              L.total));
-  int per_cu = 0;
-  AV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kGridThreads, L.total));
-  if (per_cu < 1) per_cu = 1;
-  int blocks = per_cu * prop.multiProcessorCount;
-  int rc = launch_fill_u32(d.counters + 3, 0u, 1, s);
+  int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kGridThreads), L.total, s, d, (const R *)vis,
-                     (const R *)weight, flag, cgk, (R *)grid, sumwt, mode);
+  // one workgroup per unit; the plan's capacity bounds the unit count
+  hipLaunchKernelGGL(kfn, dim3((unsigned)d.max_gunits), dim3(kGridThreads), L.total, s, d,
+                     (const R *)vis, (const R *)weight, flag, cgk, (R *)grid, mode);
   AV_LAUNCH_CHECK("k_grid_tiles");
+  if (!(mode & AV_GRID_ADJOINT)) {
+    const int nplanes = d.n_img_chan * NPOL;
+    const int use_lds = nplanes * 8 <= 65536;
+    hipLaunchKernelGGL(k_sumwt_reduce, dim3(prop.num_cus), dim3(256),
+                       use_lds ? nplanes * 8 : 0, s, d, NPOL, sumwt, use_lds);
+    AV_LAUNCH_CHECK("k_sumwt_reduce");
+  }
   return AV_OK;
 }
 

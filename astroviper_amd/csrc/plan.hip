@@ -58,19 +58,6 @@ __device__ inline int64_t bin_of(const PlanDev &P, int tu, int tv, int ic) {
   return ((int64_t)tu * P.ntv + tv) * P.n_img_chan + ic;
 }
 
-// wave-aggregated counter increment; returns this lane's slot
-__device__ inline uint32_t wave_append(uint32_t *ctr, bool pred) {
-  unsigned long long m = __ballot(pred);
-  if (m == 0) return 0;
-  int lane = __lane_id();
-  int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-  return base + (uint32_t)__popcll(below);
-}
-
 __global__ void k_chan_setup(PlanDev P, const double *freq, const int32_t *chan_map) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.n_chan) return;
@@ -134,9 +121,13 @@ __global__ __launch_bounds__(256) void k_count(PlanDev P) {
     ItemGeo g;
     g.gvalid = g.dvalid = false;
     if (live) item_geometry(P, it, g);
-    wave_append(&P.counters[0], g.gvalid);
-    wave_append(&P.counters[1], g.dvalid);
-    wave_append(&P.counters[2], live && !g.dvalid);
+    // a wave covers 64 consecutive samples starting at a multiple of 64
+    const unsigned long long gm = __ballot(g.gvalid), dm = __ballot(g.dvalid);
+    const int64_t first = it - __lane_id();
+    if (__lane_id() == 0 && first < nitems) {
+      P.gmask[first >> 6] = gm;
+      P.dmask[first >> 6] = dm;
+    }
     if (!(g.gvalid || g.dvalid)) continue;
     const RecPlan rp = record_plan(P, g);
     if (rp.gprim) {
@@ -173,8 +164,6 @@ __global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
     ItemGeo g;
     g.gvalid = g.dvalid = false;
     if (live) item_geometry(P, it, g);
-    uint32_t zslot = wave_append(&P.counters[5], live && !g.dvalid);
-    if (live && !g.dvalid) P.zero_idx[zslot] = (uint32_t)it;
     if (!(g.gvalid || g.dvalid)) continue;
     const RecPlan rp = record_plan(P, g);
     if (rp.gprim) {
@@ -233,6 +222,42 @@ __global__ void k_units(PlanDev P) {
     uint32_t cnt = min(cap, np - k * cap);
     P.dunits[d0 + k] = make_uint4((uint32_t)b, first, cnt, 0u);
   }
+}
+
+// per-call tables: the FP32 kernel table and, for every kernel offset, the
+// FP64 sum of the support taps of one axis (norm = nrm[ou] * nrm[ov])
+__global__ void k_prepare_tables(PlanDev P, const double *cgk) {
+  const int sc = P.support / 2, ov = P.ov;
+  const int ntab = ov * (sc + 1), half = ov / 2 + 2, nnrm = 2 * half + 1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ntab + nnrm;
+       i += gridDim.x * blockDim.x) {
+    if (i < ntab) {
+      P.tab32[i] = (float)cgk[i];
+    } else {
+      const int off = i - ntab - half;
+      double s = 0.0;
+      for (int t = 0; t < P.support; ++t) {
+        int k = ov * (t - sc) + off;
+        s += cgk[k < 0 ? -k : k];
+      }
+      P.nrm[i - ntab] = s;
+    }
+  }
+}
+
+int prepare_tables(const PlanDev &d, const double *cgk, hipStream_t s) {
+  hipLaunchKernelGGL(k_prepare_tables, dim3(4), dim3(256), 0, s, d, cgk);
+  AV_LAUNCH_CHECK("k_prepare_tables");
+  return AV_OK;
+}
+
+__global__ void k_popcount(const unsigned long long *m, int64_t n, uint32_t *out) {
+  uint32_t c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c += (uint32_t)__popcll(m[i]);
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+  if (__lane_id() == 0 && c) atomicAdd(out, c);
 }
 
 }  // namespace av
@@ -314,8 +339,11 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
          o_ridx = take(4 * d.max_records), o_rg = take(4 * d.max_records),
          o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
-         o_dun = take(sizeof(uint4) * d.max_dunits), o_zero = take(4 * (n_items + 1)),
-         o_ctr = take(4 * 16);
+         o_dun = take(sizeof(uint4) * d.max_dunits),
+         o_gm = take(8 * ((n_items + 63) / 64 + 1)), o_dm = take(8 * ((n_items + 63) / 64 + 1)),
+         o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
+         o_nrm = take(sizeof(double) * (2 * (oversampling / 2 + 2) + 1)),
+         o_upart = take(sizeof(double) * 4 * d.max_gunits), o_ctr = take(4 * 16);
   void *base = nullptr;
   hipError_t e = hipMalloc(&base, off);
   if (e != hipSuccess) {
@@ -343,7 +371,12 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.rec_d = (uint32_t *)(c + o_rd);
   d.gunits = (uint4 *)(c + o_gun);
   d.dunits = (uint4 *)(c + o_dun);
-  d.zero_idx = (uint32_t *)(c + o_zero);
+  d.gmask = (unsigned long long *)(c + o_gm);
+  d.dmask = (unsigned long long *)(c + o_dm);
+  d.n_words = (n_items + 63) / 64;
+  d.tab32 = (float *)(c + o_tab);
+  d.nrm = (double *)(c + o_nrm);
+  d.upart = (double *)(c + o_upart);
   d.counters = (uint32_t *)(c + o_ctr);
   *out = p;
   return AV_OK;
@@ -368,13 +401,13 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   d.row_chan = (int32_t *)row_chan;
   if (row_chan) d.n_items = d.n_rows;  // one sample per row
   else d.n_items = d.n_rows * (int64_t)d.n_chan;
+  d.n_words = (d.n_items + 63) / 64;
   hipLaunchKernelGGL(k_chan_setup, dim3((d.n_chan + 255) / 256), dim3(256), 0, s, d, freq,
                      chan_map);
   AV_LAUNCH_CHECK("k_chan_setup");
   int rc;
   if ((rc = launch_fill_u32(d.cnt_all, 0u, d.n_bins, s))) return rc;
   if ((rc = launch_fill_u32(d.cnt_prim, 0u, d.n_bins, s))) return rc;
-  if ((rc = launch_fill_u32(d.counters, 0u, 16, s))) return rc;
   int64_t blocks = (d.n_items + 255) / 256;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
   hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(256), 0, s, d);
@@ -409,7 +442,16 @@ int av_plan_stats(av_plan *plan, int64_t out[8], void *stream) {
   if (!plan || !out) return fail(AV_ERR_INVALID, "av_plan_stats: NULL argument");
   hipStream_t s = (hipStream_t)stream;
   const PlanDev &d = plan->d;
-  uint32_t ctr[16], tot[3];
+  int rc = launch_fill_u32(d.counters, 0u, 2, s);
+  if (rc) return rc;
+  if (d.n_words > 0) {
+    hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, s, d.gmask, d.n_words, d.counters);
+    AV_LAUNCH_CHECK("k_popcount");
+    hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, s, d.dmask, d.n_words,
+                       d.counters + 1);
+    AV_LAUNCH_CHECK("k_popcount");
+  }
+  uint32_t ctr[2], tot[3];
   AV_HIP(hipMemcpyAsync(ctr, d.counters, sizeof(ctr), hipMemcpyDeviceToHost, s));
   AV_HIP(hipMemcpyAsync(&tot[0], d.bin_start + d.n_bins, 4, hipMemcpyDeviceToHost, s));
   AV_HIP(hipMemcpyAsync(&tot[1], d.gunit_start + d.n_bins, 4, hipMemcpyDeviceToHost, s));
@@ -422,7 +464,7 @@ int av_plan_stats(av_plan *plan, int64_t out[8], void *stream) {
   out[4] = d.n_bins;
   out[5] = tot[1];
   out[6] = tot[2];
-  out[7] = ctr[2];
+  out[7] = d.n_items - ctr[1];  // samples no degrid unit visits
   return AV_OK;
 }
 
