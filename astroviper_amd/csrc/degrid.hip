@@ -2,12 +2,14 @@
 // core/imaging/imaging_utils/standard_degrid.py:388-485, zero dphase).
 //
 // One workgroup per degrid unit (primary records of one bin).  The unit
-// copies its tile plus a halo of support/2 + 1 pixels of every grid
-// polarisation plane into LDS (one wavefront per row, coalesced), then each
-// wavefront evaluates samples: lanes = (component, v-tap) read one LDS row
-// per u-tap (bank-conflict free), multiply by the u and v kernel values, and
-// a DPP reduction over the v-taps gives the sample, divided by the
-// kernel-weight sum as in the reference.  Each sample is written exactly once.
+// copies its tile plus a halo of support/2 + 1 pixels into LDS, cell-major
+// and already in visibility-polarisation order (through pol_map), one
+// wavefront per row.  Each wavefront then evaluates its staged samples with
+// lanes = (sample, v-tap, 16-byte chunk): a lane reads one chunk per u-tap
+// (eight lanes cover 128 contiguous bytes, so no bank conflicts), weights it
+// by the u and v kernel values, and a DPP reduction over the v-taps gives
+// the sample, divided by the kernel-weight sum as in the reference.  Each
+// sample is written exactly once.
 #include <algorithm>
 
 #include "tile_common.h"
@@ -19,19 +21,17 @@ constexpr int kDegridWaves = kDegridThreads / 64;
 
 struct DegridLds {
   int tab, reg, st_cu, st_cv, st_pos, st_item, total;
-  int ntab, halo, ru, rv, rps;
+  int ntab, halo, ru, rv;
 };
 
-template <typename R, int DVW>
-__host__ __device__ inline DegridLds degrid_lds_layout(int ov, int support, int tu, int tv,
-                                                       int npol_g) {
+template <typename R, int NPOL, int DVW>
+__host__ __device__ inline DegridLds degrid_lds_layout(int ov, int support, int tu, int tv) {
   DegridLds L;
   int sc = support / 2;
   L.ntab = ov * (sc + 1);
   L.halo = sc + 1;
   L.ru = tu + 2 * L.halo;
   L.rv = tv + 2 * L.halo;
-  L.rps = plane_stride(L.ru * L.rv, DVW);
   int o = 0;
   auto take = [&](int bytes) {
     int r = o;
@@ -39,11 +39,11 @@ __host__ __device__ inline DegridLds degrid_lds_layout(int ov, int support, int 
     return r;
   };
   L.tab = take(L.ntab * (int)sizeof(R));
-  L.reg = take(2 * npol_g * L.rps * (int)sizeof(R));
-  L.st_cu = take(kDegridWaves * 64 * DVW * (int)sizeof(R));
-  L.st_cv = take(kDegridWaves * 64 * DVW * (int)sizeof(R));
-  L.st_pos = take(kDegridWaves * 64 * (int)sizeof(int));
-  L.st_item = take(kDegridWaves * 64 * (int)sizeof(uint32_t));
+  L.reg = take(L.ru * L.rv * 2 * NPOL * (int)sizeof(R));
+  L.st_cu = take(kDegridThreads * DVW * (int)sizeof(R));
+  L.st_cv = take(kDegridThreads * DVW * (int)sizeof(R));
+  L.st_pos = take(kDegridThreads * (int)sizeof(int));
+  L.st_item = take(kDegridThreads * (int)sizeof(uint32_t));
   L.total = o;
   return L;
 }
@@ -53,13 +53,16 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
     PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
     const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
     int mode) {
-  using LM = LaneMap<NPOL, DVW>;
+  using CL = CellLayout<R, NPOL>;
+  constexpr int C = CL::kC, CC = CL::kCC, H = CL::kH;
+  constexpr int SPI = 64 / (H * DVW) > 0 ? 64 / (H * DVW) : 1;  // samples per wave step
+  using V = Vec<R, CC>;
   extern __shared__ __align__(16) char smem[];
   const uint32_t u = blockIdx.x;
   if (u >= P.dunit_start[P.n_bins]) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
-  const DegridLds L = degrid_lds_layout<R, DVW>(ov, S, Tu, Tv, npol_g);
+  const DegridLds L = degrid_lds_layout<R, NPOL, DVW>(ov, S, Tu, Tv);
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -68,23 +71,10 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
   uint32_t *st_item = (uint32_t *)(smem + L.st_item) + wave * 64;
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
-  const int H = L.halo;
+  const int Hh = L.halo, RV = L.rv;
 
   for (int i = tid; i < L.ntab; i += kDegridThreads)
     tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
-
-  const int slot = lane / LM::kLanesPerItem;
-  const int lq = (lane % LM::kLanesPerItem) / DVW;
-  const int dv = lane % DVW;
-  int gq[LM::kQPerLane];  // LDS plane of each component this lane handles
-#pragma unroll
-  for (int k = 0; k < LM::kQPerLane; ++k) {
-    int q = lq + k * LM::kQStride;
-    int p = q >> 1;
-    int pg = pol_map ? pol_map[p] : p % npol_g;
-    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);  // stay inside the LDS region
-    gq[k] = 2 * pg + (q & 1);
-  }
 
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
@@ -95,28 +85,28 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
-  // ---- tile + halo of every grid polarisation into LDS (zero outside);
-  // one wavefront per region row, lanes along the row
-  for (int row = wave; row < npol_g * L.ru; row += kDegridWaves) {
+  // ---- tile + halo into LDS in visibility-polarisation order (zero outside
+  // the grid); one wavefront per (polarisation, region row)
+  for (int row = wave; row < NPOL * L.ru; row += kDegridWaves) {
     const int p = row / L.ru, ru = row - p * L.ru;
-    const int gu = u0 - H + ru;
+    int pg = pol_map ? pol_map[p] : p % npol_g;
+    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);
+    const int gu = u0 - Hh + ru;
     const bool urow = gu >= 0 && gu < P.n_u;
-    const R *src = gplane + p * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2;
-    R *dre = reg + (2 * p) * L.rps + ru * L.rv;
-    R *dim = reg + (2 * p + 1) * L.rps + ru * L.rv;
-    for (int rv = lane; rv < L.rv; rv += 64) {
-      const int gv = v0 - H + rv;
-      R re = (R)0, im = (R)0;
-      if (urow && gv >= 0 && gv < P.n_v) {
-        re = src[(size_t)gv * 2];
-        im = src[(size_t)gv * 2 + 1];
-      }
-      dre[rv] = re;
-      dim[rv] = im;
+    const Cplx2<R> *src = (const Cplx2<R> *)(gplane + pg * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2);
+    for (int rv = lane; rv < RV; rv += 64) {
+      const int gv = v0 - Hh + rv;
+      Cplx2<R> val{(R)0, (R)0};
+      if (urow && gv >= 0 && gv < P.n_v) val = src[gv];
+      *(Cplx2<R> *)(reg + (ru * RV + rv) * C + 2 * p) = val;
     }
   }
   __syncthreads();
 
+  // lanes: (sample slot s, v-tap dv, chunk h)
+  const int h = lane % H;
+  const int dv = (lane / H) % DVW;
+  const int slot = lane / (H * DVW);
   const uint32_t end = first + cnt;
   for (uint32_t r0 = first; r0 < end; r0 += kDegridThreads) {
     const uint32_t r = r0 + wave * 64 + lane;
@@ -138,50 +128,60 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
     st_pos[lane] = pos;
     wave_lds_sync();
     const int nst = (int)min((uint32_t)64, end > r0 + wave * 64 ? end - (r0 + wave * 64) : 0u);
-    for (int j0 = 0; j0 < nst; j0 += LM::kItemsPerIter) {
+    for (int j0 = 0; j0 < nst; j0 += SPI) {
       const int j = j0 + slot;
-      const int pj = j < nst ? st_pos[j] : -1;
+      const int pj = (slot < SPI && j < nst) ? st_pos[j] : -1;
       // The kernel-weight sum is formed with exactly the operations applied
       // to the data, so a constant grid degrids to that constant exactly
       // (the reference sums wt*g and wt in the same loop).
-      R accv[LM::kQPerLane];
+      R a[CC];
       R nacc = (R)0;
 #pragma unroll
-      for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = (R)0;
+      for (int c = 0; c < CC; ++c) a[c] = (R)0;
       if (pj >= 0 && dv < S) {
         const int au = (pj & 0xFFFF) - kPosBias;
         const int avv = (pj >> 16) - kPosBias;
-        const int col = avv + H + dv - sc;
+        const R *base = reg + ((au + Hh - sc) * RV + (avv + Hh - sc + dv)) * C + h * CC;
         for (int du = 0; du < S; ++du) {
           const R cu = st_cu[j * DVW + du];
-          const int row = (au + H + du - sc) * L.rv + col;
+          const V gv = *(const V *)(base + du * RV * C);
           nacc += cu;
 #pragma unroll
-          for (int k = 0; k < LM::kQPerLane; ++k) accv[k] += cu * reg[gq[k] * L.rps + row];
+          for (int c = 0; c < CC; ++c) a[c] += cu * gv.v[c];
         }
         const R cv = st_cv[j * DVW + dv];
         nacc *= cv;
 #pragma unroll
-        for (int k = 0; k < LM::kQPerLane; ++k) accv[k] *= cv;
+        for (int c = 0; c < CC; ++c) a[c] *= cv;
       }
-      // every lane takes part in the DPP reductions
+      // every lane takes part in the reductions over the v-taps
 #pragma unroll
-      for (int k = 0; k < LM::kQPerLane; ++k) accv[k] = group_sum_to_last<DVW>(accv[k]);
-      nacc = group_sum_to_last<DVW>(nacc);
+      for (int c = 0; c < CC; ++c) a[c] = sum_over_dv<H, DVW>(a[c]);
+      nacc = sum_over_dv<H, DVW>(nacc);
       if (pj >= 0 && dv == DVW - 1) {
         const uint32_t item = st_item[j];
+        uint32_t fbits = 0;
+        if (flag) {
+          if (NPOL == 4)
+            fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+          else
+            for (int p = 0; p < NPOL; ++p) fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+        }
+        V *dst = (V *)(vis_out + (size_t)item * C + h * CC);
+        V o;
 #pragma unroll
-        for (int k = 0; k < LM::kQPerLane; ++k) {
-          const int q = lq + k * LM::kQStride;
-          const int p = q >> 1;
-          const size_t o = ((size_t)item * NPOL + p) * 2 + (q & 1);
-          const bool flagged = flag && flag[(size_t)item * NPOL + p];
+        for (int c = 0; c < CC; ++c) o.v[c] = incremental ? dst->v[c] : (R)0;
+#pragma unroll
+        for (int c = 0; c < CC; ++c) {
+          const int p = (h * CC + c) >> 1;
+          const bool flagged = (fbits >> (8 * p)) & 0xFFu;
           if (incremental) {
-            if (!flagged) vis_out[o] += accv[k] / nacc;
+            if (!flagged) o.v[c] += a[c] / nacc;
           } else {
-            vis_out[o] = flagged ? (R)0 : accv[k] / nacc;
+            o.v[c] = flagged ? (R)0 : a[c] / nacc;
           }
         }
+        *dst = o;
       }
     }
     wave_lds_sync();
@@ -208,7 +208,7 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
                          const uint8_t *flag, const double *cgk, void *vis_out, int mode,
                          hipStream_t s) {
   PlanDev &d = plan->d;
-  DegridLds L = degrid_lds_layout<R, DVW>(d.ov, d.support, d.tile_u, d.tile_v, npol_g);
+  DegridLds L = degrid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v);
   auto kfn = k_degrid_tiles<R, NPOL, DVW>;
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");

@@ -1,48 +1,44 @@
-// tile_common.h -- lane layout and LDS helpers shared by the tiled gridder
-// and degridder.
+// tile_common.h -- cell layout, lane maps and small helpers shared by the
+// tiled gridder and degridder.
 //
-// A wavefront works on whole samples.  Inside one sample the 64 lanes are
-// laid out as (component q, v-tap dv): q = 2*pol + {re,im} picks an LDS
-// plane and dv a column of the support footprint, and the u-taps are a loop.
-// For a fixed u-tap the lanes of one sample touch consecutive words of up to
-// four planes whose strides are = DVW (mod 32), so every LDS access of a
-// sample is free of bank conflicts.
+// LDS tiles are stored cell-major: a cell (u, v) holds the 2*npol real
+// components of all its polarisations contiguously (32 bytes for four FP32
+// polarisations).  Lanes move one 16-byte chunk each (ds_read_b128 /
+// ds_write_b128), and eight consecutive lanes cover 128 contiguous bytes, so
+// every wavefront access is free of bank conflicts.
+//
+// Float atomics on LDS run at about 1/25 of a plain read-modify-write on
+// gfx950, so no kernel here uses them: a tile row band is only ever updated
+// by the one wavefront that owns it.
 #pragma once
 
 #include "avgrid_internal.h"
 
 namespace av {
 
-template <int NPOL, int DVW>
-struct LaneMap {
-  static constexpr int kQ = 2 * NPOL;                        // components per sample
-  static constexpr int kLpi = kQ * DVW;                      // lanes wanted per sample
-  static constexpr int kLanesPerItem = kLpi > 64 ? 64 : kLpi;
-  static constexpr int kItemsPerIter = 64 / kLanesPerItem;   // samples per wave step
-  static constexpr int kQPerLane = kLpi > 64 ? kLpi / 64 : 1;  // components per lane
-  static constexpr int kQStride = kLanesPerItem / DVW;       // q distance between them
+template <typename R, int N>
+struct alignas(sizeof(R) * N) Vec {
+  R v[N];
 };
 
-// plane stride: >= n and congruent to DVW modulo 32 (see above)
-__host__ __device__ inline int plane_stride(int n, int dvw) {
-  int r = n % 32;
-  int s = n - r + dvw;
-  if (s < n) s += 32;
-  return s;
-}
+template <typename R>
+struct alignas(2 * sizeof(R)) Cplx2 {
+  R re, im;
+};
 
-// Wave-local ordering point: LDS operations of one wavefront execute in
-// order, so this only has to stop the compiler from moving LDS accesses
-// across it.
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// Components of one cell and how they split into 16-byte chunks.
+template <typename R, int NPOL>
+struct CellLayout {
+  static constexpr int kC = 2 * NPOL;                       // real components per cell
+  static constexpr int kPerChunk = 16 / (int)sizeof(R);     // components in 16 bytes
+  static constexpr int kCC = kC < kPerChunk ? kC : kPerChunk;  // components per lane chunk
+  static constexpr int kH = kC / kCC;                       // chunks per cell
+};
 
-// Sum over each group of DVW consecutive lanes; the total lands in the last
-// lane of the group.  Row shifts stay inside 16-lane DPP rows.
-__device__ inline float dpp_shr(float v, int n) {
+// Sum over each group of DVW lanes spaced H apart (lane = ... + dv*H + h);
+// the total lands in the group's last v-tap lane (dv = DVW - 1).  Shifts
+// inside a 16-lane DPP row use row_shr; wider ones use a lane shuffle.
+__device__ inline float dpp_shr_f(float v, int n) {
   int iv = __builtin_bit_cast(int, v);
   int r;
   switch (n) {
@@ -53,39 +49,37 @@ __device__ inline float dpp_shr(float v, int n) {
   }
   return __builtin_bit_cast(float, r);
 }
-__device__ inline double dpp_shr(double v, int n) {
-  long long iv = __builtin_bit_cast(long long, v);
-  int lo = (int)(iv & 0xffffffffll), hi = (int)(iv >> 32);
-  int rl, rh;
-  switch (n) {
-    case 1:
-      rl = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xF, 0xF, true);
-      rh = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xF, 0xF, true);
-      break;
-    case 2:
-      rl = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xF, 0xF, true);
-      rh = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xF, 0xF, true);
-      break;
-    case 4:
-      rl = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xF, 0xF, true);
-      rh = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xF, 0xF, true);
-      break;
-    default:
-      rl = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xF, 0xF, true);
-      rh = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xF, 0xF, true);
-      break;
+__device__ inline float shr_any(float v, int n) {
+  if (n < 16) return dpp_shr_f(v, n);
+  return __shfl_up(v, n);
+}
+__device__ inline double shr_any(double v, int n) {
+  if (n < 16) {
+    long long iv = __builtin_bit_cast(long long, v);
+    int lo = (int)(iv & 0xffffffffll), hi = (int)(iv >> 32);
+    float flo = __builtin_bit_cast(float, lo), fhi = __builtin_bit_cast(float, hi);
+    int rl = __builtin_bit_cast(int, dpp_shr_f(flo, n));
+    int rh = __builtin_bit_cast(int, dpp_shr_f(fhi, n));
+    long long r = ((long long)(unsigned)rl) | ((long long)rh << 32);
+    return __builtin_bit_cast(double, r);
   }
-  long long r = ((long long)(unsigned)rl) | ((long long)rh << 32);
-  return __builtin_bit_cast(double, r);
+  return __shfl_up(v, n);
 }
 
-template <int DVW, typename R>
-__device__ inline R group_sum_to_last(R v) {
-  if (DVW >= 16) v += dpp_shr(v, 8);
-  v += dpp_shr(v, 4);
-  v += dpp_shr(v, 2);
-  v += dpp_shr(v, 1);
+template <int H, int DVW, typename R>
+__device__ inline R sum_over_dv(R v) {
+#pragma unroll
+  for (int off = H; off < H * DVW; off *= 2) v += shr_any(v, off);
   return v;
+}
+
+// Wave-local ordering point: LDS operations of one wavefront execute in
+// order, so this only has to stop the compiler from moving LDS accesses
+// across it.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // |ov*(tap - sc) + off| into the half-kernel table (standard_grid.py:384-396)
