@@ -1,20 +1,22 @@
 // grid.hip -- tiled convolutional gridding (replaces standard_grid_jit,
 // core/imaging/imaging_utils/standard_grid.py:246-432).
 //
-// One workgroup per work unit (a bin, or a slice of a large bin); the launch
-// covers the plan's unit capacity and surplus workgroups exit at once.  A
-// unit owns a tile_u x tile_v tile of every polarisation plane of one image
-// channel, held in LDS cell-major.  Records are staged 256 at a time; each of
-// the four wavefronts owns a band of tile rows and walks all staged samples,
-// applying the taps that fall in its band with plain 16-byte read-modify-
-// writes (lanes = (u-tap group, v-tap, chunk); one sample at a time, so the
-// lanes of an update touch distinct cells and no atomics are needed).  Only
-// taps inside the tile are applied, which is why each footprint is recorded
-// in every tile it overlaps.  The finished tile is written to HBM once, row
-// by row, coalesced.  A bin split over several units adds its partial tiles
-// with global atomics into a tile that a pre-pass zeroed.  Sums of weights
-// are FP64: each unit writes its partial sums and a reduction pass adds them
-// per plane.
+// One wavefront per work unit (a bin, or a slice of a large bin); a
+// workgroup holds up to four independent wavefronts (fewer when their tiles
+// do not fit in LDS together) and the launch covers the plan's unit
+// capacity.  A unit owns a tile_u x tile_v tile of every
+// polarisation plane of one image channel, held in the wavefront's own LDS
+// region cell-major.  Records are staged 64 at a time (one per lane) while
+// the next batches' records and sample gathers are already in flight; the
+// wavefront then applies the staged samples one after another with plain
+// 16-byte read-modify-writes (lanes = (u-tap group, v-tap, chunk)): the lanes
+// of one update touch distinct cells and only this wavefront touches the
+// tile, so no atomics are needed.  Only taps inside the tile are applied,
+// which is why each footprint is recorded in every tile it overlaps.  The
+// finished tile is written to HBM once, row by row, coalesced.  A bin split
+// over several units adds its partial tiles with global atomics into a tile
+// that a pre-pass zeroed.  Sums of weights are FP64: each unit writes its
+// partial sums and a reduction pass adds them per plane.
 #include <algorithm>
 
 #include "tile_common.h"
@@ -22,8 +24,8 @@
 namespace av {
 
 constexpr int kGridThreads = 256;
-constexpr int kGridWaves = kGridThreads / 64;
-constexpr int kCompChunkRounds = 4;  // compensated mode: fold every 4*256 records
+constexpr int kGridMaxWaves = kGridThreads / 64;
+constexpr int kCompChunkBatches = 4;  // compensated mode: fold every 4*64 records
 
 template <typename R>
 struct Cplx {
@@ -31,13 +33,13 @@ struct Cplx {
 };
 
 struct GridLds {
-  // offsets in bytes inside dynamic LDS
-  int tab, nrm, acc, st_d, st_cu, st_cv, st_pos, red, total;
+  // offsets in bytes inside dynamic LDS; acc/st_d are per wavefront
+  int tab, nrm, acc, st_d, wave_bytes, total;
   int ntab, nnrm;
 };
 
 template <typename R, int NPOL, int DVW>
-__host__ __device__ inline GridLds grid_lds_layout(int ov, int support, int tu, int tv) {
+__host__ __device__ inline GridLds grid_lds_layout(int ov, int support, int tu, int tv, int nw) {
   GridLds L;
   int sc = support / 2;
   L.ntab = ov * (sc + 1);
@@ -50,14 +52,41 @@ __host__ __device__ inline GridLds grid_lds_layout(int ov, int support, int tu, 
   };
   L.tab = take(L.ntab * (int)sizeof(R));
   L.nrm = take(L.nnrm * (int)sizeof(double));
-  L.acc = take(tu * tv * 2 * NPOL * (int)sizeof(R));
-  L.st_d = take(kGridThreads * 2 * NPOL * (int)sizeof(R));
-  L.st_cu = take(kGridThreads * DVW * (int)sizeof(R));
-  L.st_cv = take(kGridThreads * DVW * (int)sizeof(R));
-  L.st_pos = take(kGridThreads * (int)sizeof(int));
-  L.red = take(kGridWaves * NPOL * (int)sizeof(double) + 16);
+  const int acc_bytes = (tu * tv * 2 * NPOL * (int)sizeof(R) + 15) & ~15;
+  const int std_bytes = 64 * 2 * NPOL * (int)sizeof(R);
+  L.acc = o;
+  L.st_d = o + acc_bytes;
+  L.wave_bytes = acc_bytes + std_bytes;
+  o += nw * L.wave_bytes;
   L.total = o;
   return L;
+}
+
+// Raw per-record loads of one lane: visibilities, weights, flags.
+template <typename R, int NPOL>
+struct RawSample {
+  Cplx<R> vv[NPOL];
+  R wv[NPOL];
+  uint32_t fbits;
+};
+
+template <typename R, int NPOL>
+__device__ inline void fetch_raw(RawSample<R, NPOL> &s, uint32_t g, uint32_t item,
+                                 const R *__restrict__ vis, const R *__restrict__ weight,
+                                 const uint8_t *__restrict__ flag, bool psf, bool adjoint) {
+  s.fbits = 0;
+  if (g == kInvalidGeo) return;
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) {
+    if (!adjoint) s.wv[p] = weight[(size_t)item * NPOL + p];
+    if (!psf) s.vv[p] = ((const Cplx<R> *)vis)[(size_t)item * NPOL + p];
+  }
+  if (flag && !adjoint) {
+    if (NPOL == 4)
+      s.fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+    else
+      for (int p = 0; p < NPOL; ++p) s.fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+  }
 }
 
 template <typename R, int NPOL, int DVW, bool COMP>
@@ -68,32 +97,32 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
                                                              R *__restrict__ grid, int mode) {
   using CL = CellLayout<R, NPOL>;
   constexpr int C = CL::kC, CC = CL::kCC, H = CL::kH;
-  constexpr int DQ = 64 / (H * DVW) > 0 ? 64 / (H * DVW) : 1;  // u-taps per wave step
+  constexpr int DQ = 64 / (H * DVW) > 0 ? 64 / (H * DVW) : 1;  // u-taps per step
   using V = Vec<R, CC>;
   extern __shared__ __align__(16) char smem[];
-  const uint32_t u = blockIdx.x;
-  if (u >= P.gunit_start[P.n_bins]) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
-  const GridLds L = grid_lds_layout<R, NPOL, DVW>(ov, S, Tu, Tv);
+  const int nw = blockDim.x >> 6;  // wavefronts (units) per workgroup
+  const GridLds L = grid_lds_layout<R, NPOL, DVW>(ov, S, Tu, Tv, nw);
   R *tab = (R *)(smem + L.tab);
   double *nrm = (double *)(smem + L.nrm);
-  R *acc = (R *)(smem + L.acc);
-  R *st_d = (R *)(smem + L.st_d);    // [record][C]
-  R *st_cu = (R *)(smem + L.st_cu);  // [record][DVW]
-  R *st_cv = (R *)(smem + L.st_cv);  // [record][DVW]
-  int *st_pos = (int *)(smem + L.st_pos);
-  double *red = (double *)(smem + L.red);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  R *acc = (R *)(smem + L.acc + wave * L.wave_bytes);
+  R *st_d = (R *)(smem + L.st_d + wave * L.wave_bytes);  // [record][C]
 
   const bool psf = (mode & AV_GRID_PSF) != 0;
   const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
   const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
   const int half = ov / 2 + 2;
 
-  for (int i = tid; i < L.ntab; i += kGridThreads)
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  if ((uint32_t)blockIdx.x * nw >= n_units) return;  // whole workgroup idle
+  for (int i = tid; i < L.ntab; i += blockDim.x)
     tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
-  for (int i = tid; i < L.nnrm; i += kGridThreads) nrm[i] = P.nrm[i];
+  for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
+  __syncthreads();  // the only workgroup barrier: tables ready
+  const uint32_t u = blockIdx.x * nw + wave;
+  if (u >= n_units) return;
   const int cells = Tu * Tv;
 
   // update lanes: (u-tap group dq, v-tap dv, chunk h)
@@ -101,12 +130,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   const int dv = (lane / H) % DVW;
   const int dq = lane / (H * DVW);
   const bool lane_on = dq < DQ && dv < S;
-  // this wavefront's band of tile rows
-  const int band_lo = (wave * Tu) / kGridWaves, band_hi = ((wave + 1) * Tu) / kGridWaves;
-  // flush layout: a thread owns one column of the tile when tile_v divides
-  // the workgroup, so the flush loops need no division
-  const bool rowwise = (kGridThreads % Tv) == 0;
-  const int fcol = tid % Tv, frow = tid / Tv, fstep = kGridThreads / Tv;
+  // flush layout: a lane owns one column when tile_v divides 64
+  const bool rowwise = (64 % Tv) == 0;
+  const int fcol = lane % Tv, frow = lane / Tv, fstep = 64 / Tv;
 
   const uint4 unit = P.gunits[u];
   const int64_t b = unit.x;
@@ -119,9 +145,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
   if (cnt == 0) {
-    if (tid < NPOL) P.upart[(size_t)u * 4 + tid] = 0.0;
+    if (lane < NPOL) P.upart[(size_t)u * 4 + lane] = 0.0;
     if (!accumulate) {
-      for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+      for (int idx = lane; idx < NPOL * cells; idx += 64) {
         int p = idx / cells, c = idx - p * cells;
         int gu = u0 + c / Tv, gv = v0 + c % Tv;
         if (gu < P.n_u && gv < P.n_v) {
@@ -137,10 +163,10 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
     uint4 *a4 = (uint4 *)acc;
     const int nbytes = cells * C * (int)sizeof(R);
     const int n16 = nbytes / 16;
-    for (int i = tid; i < n16; i += kGridThreads) a4[i] = make_uint4(0u, 0u, 0u, 0u);
-    for (int i = n16 * 16 / (int)sizeof(R) + tid; i < cells * C; i += kGridThreads) acc[i] = (R)0;
+    for (int i = lane; i < n16; i += 64) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = n16 * 16 / (int)sizeof(R) + lane; i < cells * C; i += 64) acc[i] = (R)0;
   }
-  // compensated mode: per-thread running sums of the components it flushes
+  // compensated mode: per-lane running sums of the components it flushes
   constexpr int kCpt = COMP ? 32 : 1;
   R csum[kCpt], ccmp[kCpt];
   if (COMP) {
@@ -151,104 +177,98 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
 #pragma unroll
   for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
 
+  // software pipeline: records of batch k+2 and sample data of batch k+1
+  // are in flight while batch k is applied
   const uint32_t end = first + cnt;
-  int round = 0;
-  for (uint32_t r0 = first; r0 < end; r0 += kGridThreads, ++round) {
-    __syncthreads();  // previous batch fully applied (and tile zeroed)
-    // ---- stage: one record per thread
-    const uint32_t r = r0 + tid;
-    int pos = -1;
-    if (r < end) {
-      const uint32_t g = P.rec_g[r];
-      const uint32_t item = P.rec_idx[r];
-      if (g != kInvalidGeo) {
-        int au, ou, avv, ovv;
-        unpack_geo(g, au, ou, avv, ovv);
-        const double norm = nrm[ou + half] * nrm[ovv + half];
-        const bool prim = r < prim_end;
-        R wv[NPOL];
-        Cplx<R> vv[NPOL];
+  uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
+  if (first + lane < end) {
+    g0 = P.rec_g[first + lane];
+    i0 = P.rec_idx[first + lane];
+  }
+  if (first + 64 + lane < end) {
+    g1 = P.rec_g[first + 64 + lane];
+    i1 = P.rec_idx[first + 64 + lane];
+  }
+  RawSample<R, NPOL> raw0;
+  fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
+  int batch = 0;
+  for (uint32_t base = first; base < end; base += 64, ++batch) {
+    // ---- stage this lane's record of the batch
+    uint32_t geo = kInvalidGeo;
+    if (g0 != kInvalidGeo) {
+      int au, ou, avv, ovv;
+      unpack_geo(g0, au, ou, avv, ovv);
+      const double norm = nrm[ou + half] * nrm[ovv + half];
+      const bool prim = base + lane < prim_end;
+      bool any = false;
+      R d[C];
 #pragma unroll
-        for (int p = 0; p < NPOL; ++p) {
-          if (!adjoint) wv[p] = weight[(size_t)item * NPOL + p];
-          if (!psf) vv[p] = ((const Cplx<R> *)vis)[(size_t)item * NPOL + p];
-        }
-        uint32_t fbits = 0;
-        if (flag && !adjoint) {
-          if (NPOL == 4)
-            fbits = *(const uint32_t *)(flag + (size_t)item * 4);
-          else
-            for (int p = 0; p < NPOL; ++p) fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
-        }
-        bool any = false;
-        R d[C];
-#pragma unroll
-        for (int p = 0; p < NPOL; ++p) {
-          R dre, dim;
-          if (adjoint) {
-            R inv = (R)(1.0 / norm);
-            dre = vv[p].re * inv;
-            dim = vv[p].im * inv;
-          } else {
-            R w = wv[p];
-            if ((fbits >> (8 * p)) & 0xFFu) w = (R)0;
-            if (psf) {
-              dre = w;
-              dim = (R)0;
-            } else {
-              dre = vv[p].re * w;
-              dim = vv[p].im * w;
-            }
-          }
-          // standard_grid.py:378: skip NaN or exactly-zero data
-          bool ok = !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0);
-          if (!ok) {
-            dre = (R)0;
+      for (int p = 0; p < NPOL; ++p) {
+        R dre, dim;
+        if (adjoint) {
+          const R inv = (R)(1.0 / norm);
+          dre = raw0.vv[p].re * inv;
+          dim = raw0.vv[p].im * inv;
+        } else {
+          R w = raw0.wv[p];
+          if ((raw0.fbits >> (8 * p)) & 0xFFu) w = (R)0;
+          if (psf) {
+            dre = w;
             dim = (R)0;
           } else {
-            any = true;
-            if (prim && !adjoint) sw[p] += (double)wv[p] * norm;
+            dre = raw0.vv[p].re * w;
+            dim = raw0.vv[p].im * w;
           }
-          d[2 * p] = dre;
-          d[2 * p + 1] = dim;
         }
-        if (any) {
-          pos = (au + kPosBias) | ((avv + kPosBias) << 16);
+        // standard_grid.py:378: skip NaN or exactly-zero data
+        const bool ok = !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0);
+        if (!ok) {
+          dre = (R)0;
+          dim = (R)0;
+        } else {
+          any = true;
+          if (prim && !adjoint) sw[p] += (double)raw0.wv[p] * norm;
+        }
+        d[2 * p] = dre;
+        d[2 * p + 1] = dim;
+      }
+      if (any) {
+        geo = g0;
 #pragma unroll
-          for (int k = 0; k < C; k += CC) {
-            V t;
+        for (int k = 0; k < C; k += CC) {
+          V t;
 #pragma unroll
-            for (int c = 0; c < CC; ++c) t.v[c] = d[k + c];
-            *(V *)(st_d + tid * C + k) = t;
-          }
-#pragma unroll
-          for (int t = 0; t < DVW; ++t) {
-            st_cu[tid * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
-            st_cv[tid * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
-          }
+          for (int c = 0; c < CC; ++c) t.v[c] = d[k + c];
+          *(V *)(st_d + lane * C + k) = t;
         }
       }
     }
-    st_pos[tid] = pos;
-    __syncthreads();
-    // ---- apply: every wavefront walks the staged samples, own band only
-    const int nst = (int)min((uint32_t)kGridThreads, end - r0);
+    // ---- issue the next loads
+    RawSample<R, NPOL> raw1;
+    fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
+    uint32_t g2 = kInvalidGeo, i2 = 0;
+    if (base + 128 + lane < end) {
+      g2 = P.rec_g[base + 128 + lane];
+      i2 = P.rec_idx[base + 128 + lane];
+    }
+    wave_lds_sync();
+    // ---- apply the staged samples one at a time
+    const int nst = (int)min((uint32_t)64, end - base);
     for (int j = 0; j < nst; ++j) {
-      const int pj = st_pos[j];
-      if (pj < 0) continue;
-      const int au = (pj & 0xFFFF) - kPosBias;
-      const int avv = (pj >> 16) - kPosBias;
-      const int rlo = max(band_lo, au - sc), rhi = min(band_hi, au - sc + S);
-      if (rlo >= rhi) continue;  // nothing in this band (wave-uniform)
+      const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)geo, j);
+      if (g == kInvalidGeo) continue;  // wave-uniform
+      int au, ou, avv, ovv;
+      unpack_geo(g, au, ou, avv, ovv);
+      const int rlo = max(0, au - sc), rhi = min(Tu, au - sc + S);
       const int col = avv - sc + dv;
-      if (!lane_on || col < 0 || col >= Tv) continue;
-      const R cv = st_cv[j * DVW + dv];
+      if (!lane_on || col < 0 || col >= Tv || rlo >= rhi) continue;
+      const R cv = tab[tab_index(ov, dv, sc, ovv)];
       const V dd = *(const V *)(st_d + j * C + h * CC);
       R x[CC];
 #pragma unroll
       for (int c = 0; c < CC; ++c) x[c] = dd.v[c] * cv;
       for (int row = rlo + dq; row < rhi; row += DQ) {
-        const R cu = st_cu[j * DVW + (row - (au - sc))];
+        const R cu = tab[tab_index(ov, row - au + sc, sc, ou)];
         V *cell = (V *)(acc + (row * Tv + col) * C + h * CC);
         V a = *cell;
 #pragma unroll
@@ -256,16 +276,16 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
         *cell = a;
       }
     }
-    if (COMP && ((round + 1) % kCompChunkRounds == 0) && r0 + kGridThreads < end) {
-      // fold the LDS partial sums into the per-thread compensated sums
-      __syncthreads();
+    wave_lds_sync();
+    if (COMP && ((batch + 1) % kCompChunkBatches == 0) && base + 64 < end) {
+      // fold the LDS partial sums into the per-lane compensated sums
 #pragma unroll
       for (int k = 0; k < kCpt; ++k) {
-        int idx = tid + k * kGridThreads;
+        const int idx = lane + k * 64;
         if (idx < C * cells) {
-          R x = acc[idx];
+          const R x = acc[idx];
           acc[idx] = (R)0;
-          R t = csum[k] + x;  // Neumaier
+          const R t = csum[k] + x;  // Neumaier
           if (fabs(csum[k]) >= fabs(x))
             ccmp[k] += (csum[k] - t) + x;
           else
@@ -273,19 +293,24 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
           csum[k] = t;
         }
       }
+      wave_lds_sync();
     }
+    raw0 = raw1;
+    g0 = g1;
+    i0 = i1;
+    g1 = g2;
+    i1 = i2;
   }
-  __syncthreads();
 
   // ---- flush the tile
   const bool multi = (fl & kUnitMulti) != 0;
   if (COMP) {
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
-      int idx = tid + k * kGridThreads;
+      const int idx = lane + k * 64;
       if (idx < C * cells) {
-        R x = acc[idx];
-        R t = csum[k] + x;
+        const R x = acc[idx];
+        const R t = csum[k] + x;
         if (fabs(csum[k]) >= fabs(x))
           ccmp[k] += (csum[k] - t) + x;
         else
@@ -293,10 +318,10 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
         acc[idx] = t + ccmp[k];
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   auto flush_cell = [&](int p, int c, int gu, int gv) {
-    Cplx<R> val = *(const Cplx<R> *)(acc + c * C + 2 * p);
+    const Cplx<R> val = *(const Cplx<R> *)(acc + c * C + 2 * p);
     R *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
     if (multi) {
       if (val.re != (R)0) atomicAdd(dst, val.re);
@@ -317,7 +342,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
         if (gu < P.n_u && gv < P.n_v) flush_cell(p, uu * Tv + fcol, gu, gv);
       }
   } else {
-    for (int idx = tid; idx < NPOL * cells; idx += kGridThreads) {
+    for (int idx = lane; idx < NPOL * cells; idx += 64) {
       const int p = idx / cells, c = idx - p * cells;
       const int gu = u0 + c / Tv, gv = v0 + c % Tv;
       if (gu < P.n_u && gv < P.n_v) flush_cell(p, c, gu, gv);
@@ -328,13 +353,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   for (int p = 0; p < NPOL; ++p) {
     double sv = adjoint ? 0.0 : sw[p];
     for (int off = 32; off >= 1; off >>= 1) sv += __shfl_xor(sv, off);
-    if (lane == 0) red[wave * NPOL + p] = sv;
-  }
-  __syncthreads();
-  if (tid < NPOL) {
-    double sv = 0.0;
-    for (int w = 0; w < kGridWaves; ++w) sv += red[w * NPOL + tid];
-    P.upart[(size_t)u * 4 + tid] = sv;
+    if (lane == 0) P.upart[(size_t)u * 4 + p] = sv;
   }
 }
 
@@ -402,11 +421,14 @@ template <typename R, int NPOL, int DVW, bool COMP>
 static int launch_grid(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
                        const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s) {
   PlanDev &d = plan->d;
-  GridLds L = grid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v);
   auto kfn = k_grid_tiles<R, NPOL, DVW, COMP>;
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
-  if (d.tile_u < kGridWaves) return fail(AV_ERR_UNSUPPORTED, "av_grid: tile_u must be at least 4");
+  // as many wavefronts per workgroup as LDS holds tiles for, up to four
+  int nw = kGridMaxWaves;
+  GridLds L = grid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v, nw);
+  while (nw > 1 && (size_t)L.total > prop.lds_per_block)
+    L = grid_lds_layout<R, NPOL, DVW>(d.ov, d.support, d.tile_u, d.tile_v, --nw);
   if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_grid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
@@ -414,8 +436,9 @@ static int launch_grid(av_plan *plan, const void *vis, const void *weight, const
                              L.total));
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  // one workgroup per unit; the plan's capacity bounds the unit count
-  hipLaunchKernelGGL(kfn, dim3((unsigned)d.max_gunits), dim3(kGridThreads), L.total, s, d,
+  // one wavefront per unit; the plan's capacity bounds the unit count
+  const unsigned nwg = (unsigned)((d.max_gunits + nw - 1) / nw);
+  hipLaunchKernelGGL(kfn, dim3(nwg), dim3(64 * nw), L.total, s, d,
                      (const R *)vis, (const R *)weight, flag, cgk, (R *)grid, mode);
   AV_LAUNCH_CHECK("k_grid_tiles");
   if (!(mode & AV_GRID_ADJOINT)) {
@@ -436,10 +459,8 @@ static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, con
   if (comp) {
     if (sizeof(R) != 4)
       return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode is for FP32 grids");
-    if (2 * NPOL * d.tile_u * d.tile_v > 32 * kGridThreads)
-      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode needs tile area <= 4096/npol");
-    if (d.tile_u < kGridWaves)
-      return fail(AV_ERR_UNSUPPORTED, "av_grid: tile_u must be at least 4");
+    if (2 * NPOL * d.tile_u * d.tile_v > 32 * 64)
+      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode needs tile area <= 1024/npol");
   }
   if (d.support <= 8) {
     if (comp) return launch_grid<R, NPOL, 8, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);

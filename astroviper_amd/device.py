@@ -107,7 +107,7 @@ class GridPlan:
     """
 
     def __init__(self, n_rows: int, n_chan: int, n_img_chan: int, n_uv, cell,
-                 support: int = 7, oversampling: int = 100, tile=(32, 32),
+                 support: int = 7, oversampling: int = 100, tile=(16, 16),
                  geometry: str = "reference", unit_cap: int = 0, device=None):
         self.device = require_gpu(device)
         self.n_rows, self.n_chan, self.n_img_chan = int(n_rows), int(n_chan), int(n_img_chan)

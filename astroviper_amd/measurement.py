@@ -32,7 +32,7 @@ class MeasurementOperator:
     def __init__(self, uvw: torch.Tensor, freq: torch.Tensor, n_uv, cell, n_pol: int = 1,
                  chan_map: Optional[torch.Tensor] = None, n_img_chan: int = 1,
                  support: int = 7, oversampling: int = 100, dtype=torch.complex64,
-                 tile=(32, 32), device=None):
+                 tile=(16, 16), device=None):
         self.device = _dev.require_gpu(device)
         self.n_u, self.n_v = int(n_uv[0]), int(n_uv[1])
         self.n_pol = int(n_pol)

@@ -18,7 +18,7 @@ C_LIGHT = 299792458.0  # astropy.constants.c
 
 
 def _degrid_to_device(uvw, flag, freq, grid, scale, offset, support, sampling, cgk_dev,
-                      chanmap, polmap, values_dev, incremental, tile=(32, 32)):
+                      chanmap, polmap, values_dev, incremental, tile=(16, 16)):
     """Degrid host uvw [nt, nb, 3] / flag [nt, nb, nf, npol] against the grid
     into values_dev [rows, nf, npol] on the GPU."""
     dev = _dev.require_gpu()

@@ -40,7 +40,7 @@ def _cgk_device(cgk_1D, oversampling, support, dev):
 
 def grid_to_device(vis_data, uvw, weight, flag, freq_chan, chan_map, n_img_chan, n_uv,
                    delta_lm, support, oversampling, do_psf, prec, cgk_dev,
-                   grid_dev=None, sumwt_dev=None, tile=(32, 32)):
+                   grid_dev=None, sumwt_dev=None, tile=(16, 16)):
     """Grid host arrays [nt, nb, nchan, npol] on the GPU, accumulating into
     grid_dev [n_img_chan, npol, n_u, n_v] / sumwt_dev when given."""
     dev = _dev.require_gpu()

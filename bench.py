@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["cube", "continuum"], default="cube")
-    ap.add_argument("--tile", type=int, nargs=2, default=[32, 32])
+    ap.add_argument("--tile", type=int, nargs=2, default=[16, 16])
     ap.add_argument("--unit-cap", type=int, default=0)
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

@@ -164,11 +164,15 @@ def test_compensated_fp32_is_closer(gpu, oracle):
     gref, _ = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_psf=True)
     g1, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True)
     g2, _, _ = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True,
-                           compensated=True, unit_cap=100000)
+                           compensated=True, unit_cap=100000, tile=(16, 16))
     e1, e2 = rel(g1.cpu().numpy(), gref), rel(g2.cpu().numpy(), gref)
     assert e2 < TOL32 and e1 < TOL32
     print(f"FP32 grid error: plain {e1:.3e}, compensated {e2:.3e}")
     assert e2 <= e1 * 1.2
+    # the compensated sums live in registers: 32 components per lane
+    with pytest.raises(NotImplementedError):
+        device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 32, psf=True,
+                    compensated=True, tile=(64, 64))
 
 
 @pytest.mark.parametrize("tile,support,npol,cap", [
