@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cmath>
 #include <string>
 
@@ -35,6 +36,7 @@ constexpr int kPosBias = 16;   // tile-relative positions are stored + 16
 constexpr int kOffBias = 128;  // kernel offsets are stored + 128
 constexpr int kMaxSupport = 16;
 constexpr int kMaxPol = 4;
+constexpr int kMaxDevices = 64;
 
 // Packed geometry of one sample relative to a tile origin:
 // bits 0-7 u position + 16, 8-15 u offset + 128, 16-23 v position + 16,
@@ -169,6 +171,27 @@ struct DeviceProps {
   size_t lds_per_block;
 };
 const DeviceProps &device_props();
+
+// Raise a kernel's dynamic-LDS limit to `bytes` the first time a launch needs
+// it on a device.  Later launches make no runtime call, so they can also be
+// recorded under stream capture once the kernel has run eagerly.
+template <auto Kernel>
+int ensure_lds_limit(int bytes, const char *what) {
+  static std::atomic<int> limit[kMaxDevices];
+  int dev = 0;
+  AV_HIP(hipGetDevice(&dev));
+  if (dev >= 0 && dev < kMaxDevices && limit[dev].load(std::memory_order_relaxed) >= bytes)
+    return AV_OK;
+  hipError_t e = hipFuncSetAttribute((const void *)Kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (dev >= 0 && dev < kMaxDevices) {
+    int cur = limit[dev].load(std::memory_order_relaxed);
+    while (cur < bytes && !limit[dev].compare_exchange_weak(cur, bytes)) {
+    }
+  }
+  return AV_OK;
+}
 
 }  // namespace av
 

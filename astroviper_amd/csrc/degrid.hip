@@ -188,6 +188,189 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core degridding.  For a group of 16 records k of one unit,
+//   T[k][(v, c)] = sum_u wu_k(u) * G[u][(v, c)]     (product over region rows)
+//   V_k[c]       = sum_v wv_k(v) * T[k][(v, c)] / norm_k.
+// The first step runs on v_mfma_*_16x16x4 with the records as the 16 result
+// rows, four region rows per step (lane l supplies A[l%16][l/16], the
+// u-weight of record l%16 at region row 4s + l/16) and the region columns,
+// cell-major, in 16-wide blocks read from LDS.  The second step is a
+// per-lane weighted sum over the blocks followed by a DPP reduction over the
+// lanes holding other cells of the same component.  One more block, of ones,
+// gives sum_u wu_k(u); the kernel-weight sum is then formed with exactly the
+// operations applied to the data, so a constant grid degrids to that
+// constant exactly (the reference sums wt*g and wt side by side).
+
+template <typename R>
+struct DegridMfma;
+// Result layout (lane l, register r): the FP32 instruction holds row
+// 4*(l/16) + r, the FP64 one row l/16 + 4*r; both hold column l%16.
+template <>
+struct DegridMfma<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lk, int r) { return 4 * lk + r; }
+};
+template <>
+struct DegridMfma<double> {
+  typedef double V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lk, int r) { return lk + 4 * r; }
+};
+
+struct DegridMfmaLds {
+  int tab, reg, total, ntab, halo, ru, rup, rv, rvp, rs;
+};
+
+// Region rows padded to a multiple of 4 (one matrix step), columns to NB
+// blocks of 16 components.
+template <typename R, int NPOL, int NB>
+__host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support, int tu, int tv) {
+  DegridMfmaLds L;
+  const int sc = support / 2;
+  L.ntab = ov * (sc + 1);
+  L.halo = sc + 1;
+  L.ru = tu + 2 * L.halo;
+  L.rup = (L.ru + 3) & ~3;
+  L.rv = tv + 2 * L.halo;
+  L.rvp = NB * 16 / (2 * NPOL);
+  L.rs = NB * 16;  // components per region row
+  L.tab = 0;
+  L.reg = (L.ntab * (int)sizeof(R) + 15) & ~15;
+  L.total = L.reg + L.rup * L.rs * (int)sizeof(R);
+  return L;
+}
+
+template <int NPOL>
+__host__ __device__ constexpr int degrid_mfma_blocks_needed(int tv, int support) {
+  return ((tv + 2 * (support / 2 + 1)) * 2 * NPOL + 15) / 16;
+}
+
+template <typename R, int NPOL, int NB>
+__global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
+    PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
+    const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
+    int mode) {
+  using MO = DegridMfma<R>;
+  using MV = typename MO::V;
+  constexpr int C = 2 * NPOL;
+  constexpr int CPB = 16 / C;  // cells per 16-column block
+  extern __shared__ __align__(16) char smem[];
+  const uint32_t u = blockIdx.x;
+  if (u >= P.dunit_start[P.n_bins]) return;
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const int Tu = P.tile_u, Tv = P.tile_v;
+  const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(ov, S, Tu, Tv);
+  R *tab = (R *)(smem + L.tab);
+  R *reg = (R *)(smem + L.reg);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const int Hh = L.halo, RS = L.rs;
+
+  for (int i = tid; i < L.ntab; i += kDegridThreads)
+    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+
+  const uint4 unit = P.dunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, cnt = unit.z;
+  const int ic = (int)(b % P.n_img_chan);
+  const int64_t tile = b / P.n_img_chan;
+  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+  // ---- tile + halo into LDS, cell-major in visibility-polarisation order,
+  // zero outside the grid and in the padding; one wavefront per (pol, row)
+  for (int row = wave; row < NPOL * L.rup; row += kDegridWaves) {
+    const int p = row / L.rup, ru = row - p * L.rup;
+    int pg = pol_map ? pol_map[p] : p % npol_g;
+    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);
+    const int gu = u0 - Hh + ru;
+    const bool urow = ru < L.ru && gu >= 0 && gu < P.n_u;
+    const Cplx2<R> *src =
+        (const Cplx2<R> *)(gplane + pg * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2);
+    for (int rv = lane; rv < L.rvp; rv += 64) {
+      const int gv = v0 - Hh + rv;
+      Cplx2<R> val{(R)0, (R)0};
+      if (urow && rv < L.rv && gv >= 0 && gv < P.n_v) val = src[gv];
+      *(Cplx2<R> *)(reg + ru * RS + rv * C + 2 * p) = val;
+    }
+  }
+  __syncthreads();
+
+  const int li = lane & 15, lk = lane >> 4;
+  const uint32_t end = first + cnt;
+  const uint32_t ngroups = (cnt + 15) / 16;
+  const int ksteps = L.rup / 4;
+  for (uint32_t g = wave; g < ngroups; g += kDegridWaves) {
+    // lane l holds record l%16 of the group (the A-operand record)
+    const uint32_t r = first + g * 16 + li;
+    uint32_t geo = kInvalidGeo, item = 0, fbits = 0;
+    if (r < end) {
+      geo = P.rec_d[r];
+      item = P.rec_idx[r];
+      if (geo != kInvalidGeo && flag) {
+        if (NPOL == 4)
+          fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+        else
+          for (int p = 0; p < NPOL; ++p) fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+      }
+    }
+    int au = -1000, ou = 0, avv = -1000, ovv = 0;
+    if (geo != kInvalidGeo) unpack_geo(geo, au, ou, avv, ovv);
+    const int urow0 = au + Hh - sc;  // region row of the first u-tap
+
+    MV T[NB], T1 = MV{0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
+    for (int s = 0; s < ksteps; ++s) {
+      const int row = 4 * s + lk;
+      const int t = row - urow0;
+      const R a = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ou)] : (R)0;
+      const R *rp = reg + row * RS + li;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, rp[16 * k], T[k]);
+      T1 = MO::mma(a, (R)1, T1);
+    }
+    // ---- weighted sums over v: lane holds records MO::row(l/16, q), cell
+    // k*CPB + li/C, component li%C of every block k
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rec = MO::row(lk, q);  // record slot in the group
+      const int avq = __shfl(avv, rec), ovq = __shfl(ovv, rec);
+      const int vcol0 = avq + Hh - sc;  // region column of the first v-tap
+      R part = (R)0, npart = (R)0;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const int t = k * CPB + li / C - vcol0;
+        const R wv = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ovq)] : (R)0;
+        part += T[k][q] * wv;
+        npart += T1[q] * wv;
+      }
+      part = sum_over_dv<C, CPB>(part);
+      npart = sum_over_dv<C, CPB>(npart);
+      const uint32_t gq = (uint32_t)__shfl((int)geo, rec);
+      const uint32_t itq = (uint32_t)__shfl((int)item, rec);
+      const uint32_t fq = (uint32_t)__shfl((int)fbits, rec);
+      if (li >= 16 - C && gq != kInvalidGeo) {
+        const int c = li - (16 - C);
+        const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
+        R *dst = vis_out + (size_t)itq * C + c;
+        if (incremental) {
+          if (!flagged) *dst += part / npart;
+        } else {
+          *dst = flagged ? (R)0 : part / npart;
+        }
+      }
+    }
+  }
+}
+
 // non-incremental mode: zero the samples no unit visits (off-grid, NaN uvw,
 // channel outside the grid), read from the plan's degrid-validity bitmask
 template <typename R>
@@ -215,8 +398,10 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
   if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_degrid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
-  AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             L.total));
+  {
+    const int rc_lds = ensure_lds_limit<k_degrid_tiles<R, NPOL, DVW>>(L.total, "av_degrid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
   if (!(mode & AV_DEGRID_INCREMENTAL) && d.n_items > 0) {
@@ -231,10 +416,50 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
   return AV_OK;
 }
 
+template <typename R, int NPOL, int NB>
+static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
+                              const uint8_t *flag, const double *cgk, void *vis_out, int mode,
+                              hipStream_t s) {
+  PlanDev &d = plan->d;
+  const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(d.ov, d.support, d.tile_u, d.tile_v);
+  auto kfn = k_degrid_mfma<R, NPOL, NB>;
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
+  if ((size_t)L.total > prop.lds_per_block)
+    return fail(AV_ERR_UNSUPPORTED, "av_degrid: tile too large for LDS (" +
+                                        std::to_string(L.total) + " bytes)");
+  {
+    const int rc_lds = ensure_lds_limit<k_degrid_mfma<R, NPOL, NB>>(L.total, "av_degrid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
+  int rc = prepare_tables(d, cgk, s);
+  if (rc) return rc;
+  if (!(mode & AV_DEGRID_INCREMENTAL) && d.n_items > 0) {
+    int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero_invalid<R>, dim3((unsigned)blocks), dim3(256), 0, s, d,
+                       (R *)vis_out, NPOL);
+    AV_LAUNCH_CHECK("k_zero_invalid");
+  }
+  hipLaunchKernelGGL(kfn, dim3((unsigned)d.max_dunits), dim3(kDegridThreads), L.total, s, d,
+                     (const R *)grid, npol_g, pol_map, flag, cgk, (R *)vis_out, mode);
+  AV_LAUNCH_CHECK("k_degrid_mfma");
+  return AV_OK;
+}
+
 template <typename R, int NPOL>
 static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
                            const uint8_t *flag, const double *cgk, void *vis_out, int mode,
                            hipStream_t s) {
+  // regions of at most 16 column blocks run on the matrix cores
+  const int nb = degrid_mfma_blocks_needed<NPOL>(plan->d.tile_v, plan->d.support);
+  if (nb <= 4)
+    return launch_degrid_mfma<R, NPOL, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+  if (nb <= 8)
+    return launch_degrid_mfma<R, NPOL, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+  if (nb <= 12)
+    return launch_degrid_mfma<R, NPOL, 12>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+  if (nb <= 16)
+    return launch_degrid_mfma<R, NPOL, 16>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   if (plan->d.support <= 8)
     return launch_degrid<R, NPOL, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   return launch_degrid<R, NPOL, 16>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);

@@ -357,34 +357,346 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   }
 }
 
-// sums of weights per (image channel, polarisation) from the unit partials,
-// gathered in LDS (FP64 LDS atomics) and added to HBM once per workgroup
-__global__ __launch_bounds__(256) void k_sumwt_reduce(PlanDev P, int npol, double *sumwt,
-                                                     int use_lds) {
-  extern __shared__ double sacc[];
-  const int nplanes = P.n_img_chan * npol;
-  const uint32_t n_units = P.gunit_start[P.n_bins];
-  if (use_lds) {
-    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) sacc[i] = 0.0;
-    __syncthreads();
+// ---------------------------------------------------------------------------
+// Matrix-core gridder.  The kernel is separable, so the update of one tile by
+// a batch of records k is a sum of rank-1 products,
+//   tile[u][c][v] += sum_k wu_k(u) * (wv_k(v) * d_k[c]),
+// a (tile_u x K) by (K x C*tile_v) matrix product whose K dimension runs over
+// records.  v_mfma_*_16x16x4 consumes four records per step: lane l supplies
+// A[i = l%16][k = l/16] = wu_k(row i) and, for every 16-column block,
+// B[k = l/16][j = l%16] = wv_k(column j) * d_k[c]; the tile accumulates in the
+// result registers (lane l holds column l%16 of four rows; MfmaOp::row).
+// Rows and columns outside a record's footprint get weight 0, so a record
+// costs the same dense blocks wherever it lies in the tile, and no LDS
+// read-modify-write is needed at all.  A unit is one wavefront, as above.
+
+template <typename R>
+struct MfmaOp;
+// Result layout (lane l, register r): the FP32 instruction holds row
+// 4*(l/16) + r, the FP64 one row l/16 + 4*r; both hold column l%16.
+template <>
+struct MfmaOp<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_units;
-       u += gridDim.x * blockDim.x) {
-    const int ic = (int)(P.gunits[u].x % P.n_img_chan);
-    for (int p = 0; p < npol; ++p) {
-      const double v = P.upart[(size_t)u * 4 + p];
-      if (v != 0.0) {
-        if (use_lds)
-          atomicAdd(&sacc[ic * npol + p], v);
-        else
-          atomicAdd(&sumwt[ic * npol + p], v);
+  __device__ static int row(int lk, int r) { return 4 * lk + r; }
+};
+template <>
+struct MfmaOp<double> {
+  typedef double V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lk, int r) { return lk + 4 * r; }
+};
+
+struct MfmaLds {
+  int tab, nrm, st_d, st_geo, wave_bytes, total, ntab, nnrm;
+};
+
+template <typename R, int NPOL>
+__host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support) {
+  MfmaLds L;
+  const int sc = support / 2;
+  L.ntab = ov * (sc + 1);
+  L.nnrm = 2 * (ov / 2 + 2) + 1;
+  int o = 0;
+  L.tab = o;
+  o += (L.ntab * (int)sizeof(R) + 15) & ~15;
+  L.nrm = o;
+  o += (L.nnrm * (int)sizeof(double) + 15) & ~15;
+  L.st_d = o;                           // per wave: [64][2*NPOL] R
+  L.st_geo = 64 * 2 * NPOL * (int)sizeof(R);  // offset inside the wave region
+  L.wave_bytes = L.st_geo + 64 * 4;
+  L.total = o + kGridMaxWaves * L.wave_bytes;
+  return L;
+}
+
+template <typename R, int NPOL, int MB, int VB, bool COMP>
+__global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
+                                                            const R *__restrict__ weight,
+                                                            const uint8_t *__restrict__ flag,
+                                                            const double *__restrict__ cgk,
+                                                            R *__restrict__ grid, int mode) {
+  using MO = MfmaOp<R>;
+  using MV = typename MO::V;
+  constexpr int C = 2 * NPOL;
+  extern __shared__ __align__(16) char smem[];
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const MfmaLds L = mfma_lds_layout<R, NPOL>(ov, S);
+  R *tab = (R *)(smem + L.tab);
+  double *nrm = (double *)(smem + L.nrm);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char *wbase = smem + L.st_d + wave * L.wave_bytes;
+  R *st_d = (R *)wbase;                           // [record][C]
+  uint32_t *st_geo = (uint32_t *)(wbase + L.st_geo);  // [record]
+
+  const bool psf = (mode & AV_GRID_PSF) != 0;
+  const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
+  const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
+  const int half = ov / 2 + 2;
+
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  if ((uint32_t)blockIdx.x * kGridMaxWaves >= n_units) return;  // whole workgroup idle
+  for (int i = tid; i < L.ntab; i += blockDim.x)
+    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+  for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
+  __syncthreads();  // the only workgroup barrier: tables ready
+  const uint32_t u = blockIdx.x * kGridMaxWaves + wave;
+  if (u >= n_units) return;
+
+  const int Tu = 16 * MB, Tv = 16 * VB;
+  const uint4 unit = P.gunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
+  const int ic = (int)(b % P.n_img_chan);
+  const int64_t tile = b / P.n_img_chan;
+  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
+  R *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+  const int li = lane & 15, lk = lane >> 4;
+
+  if (cnt == 0) {
+    if (lane < NPOL) P.upart[(size_t)u * 4 + lane] = 0.0;
+    if (!accumulate) {
+      for (int idx = lane; idx < NPOL * Tu * Tv; idx += 64) {
+        const int p = idx / (Tu * Tv), c = idx - p * (Tu * Tv);
+        const int gu = u0 + c / Tv, gv = v0 + c % Tv;
+        if (gu < P.n_u && gv < P.n_v) {
+          Cplx<R> z{(R)0, (R)0};
+          *(Cplx<R> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) = z;
+        }
       }
     }
+    return;
   }
-  if (use_lds) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < nplanes; i += blockDim.x)
-      if (sacc[i] != 0.0) atomicAdd(&sumwt[i], sacc[i]);
+
+  MV acc[MB][C][VB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int vb = 0; vb < VB; ++vb) acc[m][c][vb] = MV{0, 0, 0, 0};
+  // compensated mode: running (sum, compensation) pairs of the accumulators
+  constexpr int kNc = COMP ? MB * C * VB * 4 : 1;
+  R csum[kNc], ccmp[kNc];
+  if (COMP) {
+#pragma unroll
+    for (int k = 0; k < kNc; ++k) csum[k] = ccmp[k] = (R)0;
+  }
+  auto fold = [&](bool reset) {
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int vb = 0; vb < VB; ++vb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int k = ((m * C + c) * VB + vb) * 4 + r;
+            const R x = acc[m][c][vb][r];
+            const R t = csum[k] + x;  // Neumaier
+            if (fabs(csum[k]) >= fabs(x))
+              ccmp[k] += (csum[k] - t) + x;
+            else
+              ccmp[k] += (x - t) + csum[k];
+            csum[k] = t;
+            acc[m][c][vb][r] = reset ? (R)0 : t + ccmp[k];
+          }
+  };
+  double sw[NPOL];
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+
+  const uint32_t end = first + cnt;
+  uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
+  if (first + lane < end) {
+    g0 = P.rec_g[first + lane];
+    i0 = P.rec_idx[first + lane];
+  }
+  if (first + 64 + lane < end) {
+    g1 = P.rec_g[first + 64 + lane];
+    i1 = P.rec_idx[first + 64 + lane];
+  }
+  RawSample<R, NPOL> raw0;
+  fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
+  int batch = 0;
+  for (uint32_t base = first; base < end; base += 64, ++batch) {
+    // ---- stage this lane's record: data (zeros when skipped) and geometry
+    uint32_t geo = kInvalidGeo;
+    R d[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) d[k] = (R)0;
+    if (g0 != kInvalidGeo) {
+      int au, ou, avv, ovv;
+      unpack_geo(g0, au, ou, avv, ovv);
+      const double norm = nrm[ou + half] * nrm[ovv + half];
+      const bool prim = base + lane < prim_end;
+      bool any = false;
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        R dre, dim;
+        if (adjoint) {
+          const R inv = (R)(1.0 / norm);
+          dre = raw0.vv[p].re * inv;
+          dim = raw0.vv[p].im * inv;
+        } else {
+          R w = raw0.wv[p];
+          if ((raw0.fbits >> (8 * p)) & 0xFFu) w = (R)0;
+          if (psf) {
+            dre = w;
+            dim = (R)0;
+          } else {
+            dre = raw0.vv[p].re * w;
+            dim = raw0.vv[p].im * w;
+          }
+        }
+        // standard_grid.py:378: skip NaN or exactly-zero data
+        const bool ok = !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0);
+        if (ok) {
+          any = true;
+          d[2 * p] = dre;
+          d[2 * p + 1] = dim;
+          if (prim && !adjoint) sw[p] += (double)raw0.wv[p] * norm;
+        }
+      }
+      if (any) geo = g0;
+    }
+#pragma unroll
+    for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {
+      Vec<R, CellLayout<R, NPOL>::kCC> t;
+#pragma unroll
+      for (int c = 0; c < CellLayout<R, NPOL>::kCC; ++c) t.v[c] = d[k + c];
+      *(Vec<R, CellLayout<R, NPOL>::kCC> *)(st_d + lane * C + k) = t;
+    }
+    st_geo[lane] = geo;
+    // ---- issue the next loads
+    RawSample<R, NPOL> raw1;
+    fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
+    uint32_t g2 = kInvalidGeo, i2 = 0;
+    if (base + 128 + lane < end) {
+      g2 = P.rec_g[base + 128 + lane];
+      i2 = P.rec_idx[base + 128 + lane];
+    }
+    wave_lds_sync();
+    // ---- four records per matrix-core step
+    const int nst = (int)min((uint32_t)64, end - base);
+    for (int s4 = 0; s4 < nst; s4 += 4) {
+      const int rec = s4 + lk;
+      const uint32_t g = st_geo[rec];
+      R a[MB], wv[VB];
+      if (g != kInvalidGeo) {
+        int au, ou, avv, ovv;
+        unpack_geo(g, au, ou, avv, ovv);
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+          const int t = 16 * m + li - au + sc;
+          a[m] = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ou)] : (R)0;
+        }
+#pragma unroll
+        for (int vb = 0; vb < VB; ++vb) {
+          const int t = 16 * vb + li - avv + sc;
+          wv[vb] = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < MB; ++m) a[m] = (R)0;
+#pragma unroll
+        for (int vb = 0; vb < VB; ++vb) wv[vb] = (R)0;
+      }
+      R dd[C];
+#pragma unroll
+      for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {
+        const Vec<R, CellLayout<R, NPOL>::kCC> t =
+            *(const Vec<R, CellLayout<R, NPOL>::kCC> *)(st_d + rec * C + k);
+#pragma unroll
+        for (int c = 0; c < CellLayout<R, NPOL>::kCC; ++c) dd[k + c] = t.v[c];
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (psf && (c & 1)) continue;  // PSF data are real
+#pragma unroll
+        for (int vb = 0; vb < VB; ++vb) {
+          const R bv = wv[vb] * dd[c];
+#pragma unroll
+          for (int m = 0; m < MB; ++m) acc[m][c][vb] = MO::mma(a[m], bv, acc[m][c][vb]);
+        }
+      }
+    }
+    wave_lds_sync();
+    if (COMP && ((batch + 1) % kCompChunkBatches == 0) && base + 64 < end) fold(true);
+    raw0 = raw1;
+    g0 = g1;
+    i0 = i1;
+    g1 = g2;
+    i1 = i2;
+  }
+  if (COMP) fold(false);
+
+  // ---- flush: lane holds rows MO::row(lane/16, r), column lane%16 of every block
+  const bool multi = (fl & kUnitMulti) != 0;
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int vb = 0; vb < VB; ++vb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gu = u0 + 16 * m + MO::row(lk, r), gv = v0 + 16 * vb + li;
+        if (gu >= P.n_u || gv >= P.n_v) continue;
+#pragma unroll
+        for (int p = 0; p < NPOL; ++p) {
+          const R re = acc[m][2 * p][vb][r], im = acc[m][2 * p + 1][vb][r];
+          R *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+          if (multi) {
+            if (re != (R)0) atomicAdd(dst, re);
+            if (im != (R)0) atomicAdd(dst + 1, im);
+          } else if (accumulate) {
+            Cplx<R> old = *(Cplx<R> *)dst;
+            old.re += re;
+            old.im += im;
+            *(Cplx<R> *)dst = old;
+          } else {
+            *(Cplx<R> *)dst = Cplx<R>{re, im};
+          }
+        }
+      }
+  // ---- sums of weights (FP64): this unit's partial per polarisation
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) {
+    double sv = adjoint ? 0.0 : sw[p];
+    for (int off = 32; off >= 1; off >>= 1) sv += __shfl_xor(sv, off);
+    if (lane == 0) P.upart[(size_t)u * 4 + p] = sv;
+  }
+}
+
+// sums of weights per (image channel, polarisation) from the unit partials:
+// one workgroup per image channel walks the channel's bins in tile order and
+// reduces in a fixed order, so repeated calls give bit-identical sums
+__global__ __launch_bounds__(256) void k_sumwt_reduce(PlanDev P, int npol, double *sumwt) {
+  __shared__ double red[4][kMaxPol];
+  const int ic = blockIdx.x;
+  const int64_t ntiles = (int64_t)P.ntu * P.ntv;
+  double acc[kMaxPol] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+    const int64_t b = t * P.n_img_chan + ic;
+    for (uint32_t u = P.gunit_start[b]; u < P.gunit_start[b + 1]; ++u)
+      for (int p = 0; p < npol; ++p) acc[p] += P.upart[(size_t)u * 4 + p];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int p = 0; p < npol; ++p) {
+    double v = acc[p];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[wave][p] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < npol) {
+    const int p = threadIdx.x;
+    double v = red[0][p];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v += red[w][p];
+    sumwt[ic * npol + p] += v;
   }
 }
 
@@ -432,8 +744,10 @@ static int launch_grid(av_plan *plan, const void *vis, const void *weight, const
   if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_grid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
-  AV_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             L.total));
+  {
+    const int rc_lds = ensure_lds_limit<k_grid_tiles<R, NPOL, DVW, COMP>>(L.total, "av_grid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
   // one wavefront per unit; the plan's capacity bounds the unit count
@@ -442,13 +756,72 @@ static int launch_grid(av_plan *plan, const void *vis, const void *weight, const
                      (const R *)vis, (const R *)weight, flag, cgk, (R *)grid, mode);
   AV_LAUNCH_CHECK("k_grid_tiles");
   if (!(mode & AV_GRID_ADJOINT)) {
-    const int nplanes = d.n_img_chan * NPOL;
-    const int use_lds = nplanes * 8 <= 65536;
-    hipLaunchKernelGGL(k_sumwt_reduce, dim3(prop.num_cus), dim3(256),
-                       use_lds ? nplanes * 8 : 0, s, d, NPOL, sumwt, use_lds);
+    hipLaunchKernelGGL(k_sumwt_reduce, dim3((unsigned)d.n_img_chan), dim3(256), 0, s, d, NPOL,
+                       sumwt);
     AV_LAUNCH_CHECK("k_sumwt_reduce");
   }
   return AV_OK;
+}
+
+template <typename R, int NPOL, int MB, int VB, bool COMP>
+static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
+                            const uint8_t *flag, const double *cgk, void *grid, double *sumwt,
+                            int mode, hipStream_t s) {
+  PlanDev &d = plan->d;
+  auto kfn = k_grid_mfma<R, NPOL, MB, VB, COMP>;
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
+  const MfmaLds L = mfma_lds_layout<R, NPOL>(d.ov, d.support);
+  {
+    const int rc_lds = ensure_lds_limit<k_grid_mfma<R, NPOL, MB, VB, COMP>>(L.total, "av_grid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
+  int rc = prepare_tables(d, cgk, s);
+  if (rc) return rc;
+  const unsigned nwg = (unsigned)((d.max_gunits + kGridMaxWaves - 1) / kGridMaxWaves);
+  hipLaunchKernelGGL(kfn, dim3(nwg), dim3(kGridThreads), L.total, s, d, (const R *)vis,
+                     (const R *)weight, flag, cgk, (R *)grid, mode);
+  AV_LAUNCH_CHECK("k_grid_mfma");
+  if (!(mode & AV_GRID_ADJOINT)) {
+    hipLaunchKernelGGL(k_sumwt_reduce, dim3((unsigned)d.n_img_chan), dim3(256), 0, s, d, NPOL,
+                       sumwt);
+    AV_LAUNCH_CHECK("k_sumwt_reduce");
+  }
+  return AV_OK;
+}
+
+// Result blocks a matrix-core unit may hold in registers: 16 for FP32 (64
+// registers), 8 for FP64 and for the compensated mode.
+template <typename R, bool COMP>
+constexpr int mfma_max_blocks() {
+  return (sizeof(R) == 8 || COMP) ? 8 : 16;
+}
+
+template <typename R, int NPOL, int MB, int VB, bool COMP>
+static int try_mfma(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                    const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s,
+                    bool &done) {
+  if constexpr (MB * 2 * NPOL * VB <= mfma_max_blocks<R, COMP>()) {
+    if (plan->d.tile_u == 16 * MB && plan->d.tile_v == 16 * VB) {
+      done = true;
+      return launch_grid_mfma<R, NPOL, MB, VB, COMP>(plan, vis, weight, flag, cgk, grid, sumwt,
+                                                     mode, s);
+    }
+  }
+  return AV_OK;
+}
+
+template <typename R, int NPOL, bool COMP>
+static int grid_mfma_shapes(av_plan *plan, const void *vis, const void *weight,
+                            const uint8_t *flag, const double *cgk, void *grid, double *sumwt,
+                            int mode, hipStream_t s, bool &done) {
+  int rc = AV_OK;
+#define AV_TRY(MB, VB)                                                                       \
+  if (!done) rc = try_mfma<R, NPOL, MB, VB, COMP>(plan, vis, weight, flag, cgk, grid, sumwt, \
+                                                  mode, s, done);
+  AV_TRY(1, 1) AV_TRY(1, 2) AV_TRY(2, 1) AV_TRY(2, 2) AV_TRY(1, 4) AV_TRY(2, 4)
+#undef AV_TRY
+  return rc;
 }
 
 template <typename R, int NPOL>
@@ -456,17 +829,34 @@ static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, con
                          const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s) {
   const bool comp = (mode & AV_GRID_COMPENSATED) != 0;
   const PlanDev &d = plan->d;
-  if (comp) {
-    if (sizeof(R) != 4)
-      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode is for FP32 grids");
-    if (2 * NPOL * d.tile_u * d.tile_v > 32 * 64)
-      return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode needs tile area <= 1024/npol");
+  if (comp && sizeof(R) != 4)
+    return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode is for FP32 grids");
+  // tiles of 16 x 16 multiples whose result blocks fit in registers run on
+  // the matrix cores; other tile shapes use the LDS tile kernel
+  bool done = false;
+  int rc;
+  if constexpr (sizeof(R) == 4) {
+    if (comp)
+      rc = grid_mfma_shapes<R, NPOL, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s,
+                                           done);
+    else
+      rc = grid_mfma_shapes<R, NPOL, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s,
+                                            done);
+  } else {
+    rc = grid_mfma_shapes<R, NPOL, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s, done);
   }
-  if (d.support <= 8) {
-    if (comp) return launch_grid<R, NPOL, 8, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+  if (done) return rc;
+  if (comp && 2 * NPOL * d.tile_u * d.tile_v > 32 * 64)
+    return fail(AV_ERR_UNSUPPORTED, "av_grid: compensated mode needs tile area <= 1024/npol");
+  if constexpr (sizeof(R) == 4) {
+    if (comp) {
+      if (d.support <= 8)
+        return launch_grid<R, NPOL, 8, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+      return launch_grid<R, NPOL, 16, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
+    }
+  }
+  if (d.support <= 8)
     return launch_grid<R, NPOL, 8, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
-  }
-  if (comp) return launch_grid<R, NPOL, 16, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
   return launch_grid<R, NPOL, 16, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
 }
 

@@ -255,6 +255,27 @@ class GridPlan:
         return vis_out
 
 
+# ------------------------------------------------------ kernel choice --
+def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = False) -> str:
+    """Name of the tile kernel av_grid runs for this plan shape: the
+    matrix-core kernel for tiles of 16 x 16 multiples whose result blocks
+    ((tile_u/16) * 2*n_pol * (tile_v/16)) fit its register budget (16 blocks
+    for FP32, 8 for FP64 and for compensated sums), else the LDS kernel."""
+    tu, tv = int(tile[0]), int(tile[1])
+    limit = 8 if (fp64 or compensated) else 16
+    if tu in (16, 32) and tv in (16, 32, 64) and (tu // 16) * 2 * n_pol * (tv // 16) <= limit:
+        return "k_grid_mfma"
+    return "k_grid_tiles"
+
+
+def degrid_kernel_name(tile, n_pol: int, support: int = 7) -> str:
+    """Name of the kernel av_degrid runs: the matrix-core kernel when the
+    tile plus its halo of support//2 + 1 pixels spans at most 16 blocks of
+    16 components per row, else the LDS kernel."""
+    blocks = ((int(tile[1]) + 2 * (support // 2 + 1)) * 2 * n_pol + 15) // 16
+    return "k_degrid_mfma" if blocks <= 16 else "k_degrid_tiles"
+
+
 # --------------------------------------------------------- dirty image --
 def dirty_image(grid: torch.Tensor, sumwt: torch.Tensor, support: int = 7,
                 oversampling: int = 100, out_dtype: torch.dtype = torch.float64,

@@ -13,6 +13,10 @@ The adjoint is A^H y = (F^H D^H y) / t with F^H = fftshift . (N^2 ifft2)
 . ifftshift and D^H the gridding kernel weighting each sample by 1/norm
 (av_grid with AV_GRID_ADJOINT on a plan whose gridding geometry is the
 degridder's), so grid and degrid are exact transposes of each other.
+In FP32 the grid and degrid kernels run in FP32 (the adjoint with
+compensated sums) while the taper and FFT run in FP64: the taper division
+reaches 1/grdsf(1)^2 = 6e4 at the image corners and would otherwise
+magnify FP32 FFT rounding.
 `MeasurementFunction` wraps A as a torch.autograd.Function whose backward is
 that adjoint (the real part for a real image).
 """
@@ -39,6 +43,9 @@ class MeasurementOperator:
         self.n_img_chan = int(n_img_chan)
         self.cdtype = dtype
         self.code = _dev._real_code(dtype)
+        self.fft_dtype = torch.complex128
+        self.fcode = _dev._real_code(self.fft_dtype)
+        self.compensated = self.code == _dev._real_code(torch.complex64)
         rows = uvw.shape[0]
         nchan = freq.numel()
         self.plan = _dev.GridPlan(rows, nchan, n_img_chan, n_uv, cell, support, oversampling,
@@ -49,11 +56,12 @@ class MeasurementOperator:
         self.shape = (self.n_img_chan, self.n_pol, self.n_u, self.n_v)
         self.n_samples = self.plan.n_samples
 
-    def _pass(self, src, dst, shift_u, shift_v, taper, scale, src_real, dst_real):
+    def _pass(self, src, dst, shift_u, shift_v, taper, scale, src_real, dst_real,
+              src_code, dst_code):
         planes = src.numel() // (self.n_u * self.n_v)
         call("av_shift_taper", _dev._ptr(src), _dev._ptr(dst), planes, self.n_u, self.n_v,
              shift_u, shift_v, _dev._ptr(self.tu), _dev._ptr(self.tv), taper, float(scale),
-             self.code, int(src_real), self.code, int(dst_real), _dev._stream())
+             src_code, int(src_real), dst_code, int(dst_real), _dev._stream())
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
         """Model visibilities [samples, n_pol] of image x (real or complex)."""
@@ -61,25 +69,29 @@ class MeasurementOperator:
         real = not x.is_complex()
         rdt = _dev.real_dtype(self.code)
         x = x.to(rdt if real else self.cdtype)
-        w = torch.empty(self.shape, dtype=self.cdtype, device=self.device)
+        w = torch.empty(self.shape, dtype=self.fft_dtype, device=self.device)
         # ifftshift, dividing by the taper at the source pixel
-        self._pass(x, w, self.n_u // 2, self.n_v // 2, 1, 1.0, real, False)
+        self._pass(x, w, self.n_u // 2, self.n_v // 2, 1, 1.0, real, False, self.code,
+                   self.fcode)
         torch.fft.fft2(w, dim=(-2, -1), out=w)
-        g = torch.empty_like(w)
-        self._pass(w, g, self.n_u - self.n_u // 2, self.n_v - self.n_v // 2, 0, 1.0, False, False)
+        g = torch.empty(self.shape, dtype=self.cdtype, device=self.device)
+        self._pass(w, g, self.n_u - self.n_u // 2, self.n_v - self.n_v // 2, 0, 1.0, False, False,
+                   self.fcode, self.code)
         return self.plan.degrid(g, n_pol=self.n_pol)
 
     def adjoint(self, y: torch.Tensor, real: bool = False) -> torch.Tensor:
         """A^H y for visibilities y [samples, n_pol]; real part if `real`."""
         y = y.reshape(self.n_samples, self.n_pol).to(self.cdtype).contiguous()
-        g, _ = self.plan.grid(y, None, adjoint=True, n_pol=self.n_pol)
-        w = torch.empty_like(g)
-        self._pass(g, w, self.n_u // 2, self.n_v // 2, 0, 1.0, False, False)
+        g, _ = self.plan.grid(y, None, adjoint=True, n_pol=self.n_pol,
+                              compensated=self.compensated)
+        w = torch.empty(self.shape, dtype=self.fft_dtype, device=self.device)
+        self._pass(g, w, self.n_u // 2, self.n_v // 2, 0, 1.0, False, False, self.code,
+                   self.fcode)
         torch.fft.ifft2(w, dim=(-2, -1), norm="forward", out=w)  # N^2 * ifft2
         out = torch.empty(self.shape, dtype=_dev.real_dtype(self.code) if real else self.cdtype,
                           device=self.device)
         self._pass(w, out, self.n_u - self.n_u // 2, self.n_v - self.n_v // 2, 2, 1.0, False,
-                   real)
+                   real, self.fcode, self.code)
         return out
 
     def apply(self, x: torch.Tensor) -> torch.Tensor:

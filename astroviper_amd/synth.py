@@ -169,3 +169,100 @@ def device_visibilities(obs: Observation, sky: SkyConfig, flag_frac: float = 0.0
     else:
         flag = torch.zeros((rows, nf, npol), dtype=torch.uint8, device=dev)
     return uvw, vis, w, flag
+
+
+def uneven_plane_sets(n_planes: int = 256, n_uv: int = 512, counts=(1e4, 2e6),
+                      flag_range=(0.0, 0.6), supports=(5, 7, 9), n_pol: int = 4,
+                      seed: int = 4, device="cuda", array: Optional[ArrayConfig] = None,
+                      sky: Optional[SkyConfig] = None, dtype=None, chunk: int = 1 << 24):
+    """Batched uneven workload: `n_planes` (channel, polarisation) planes of
+    n_uv x n_uv.  Plane p belongs to channel p // n_pol and uses support
+    supports[p % len(supports)]; its unflagged count is log-uniform in
+    `counts`, and a fraction U(flag_range) of its samples is flagged on top.
+    Each sample is one baseline (64-antenna array) at a random hour angle in
+    the array's range; visibilities are the point-source DFT plus noise,
+    weights U(0.5, 1.5).  The cell puts the longest baseline at the highest
+    frequency at 0.95 of the grid's half width.
+
+    Returns ([PlaneSet per support], (cell_u, cell_v), info)."""
+    import torch
+
+    from .batch import PlaneSet
+
+    arr = array or ArrayConfig(n_ant=64, seed=seed)
+    sky = sky or SkyConfig()
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device)
+    cdtype = dtype or torch.complex64
+    rdtype = torch.float32 if cdtype == torch.complex64 else torch.float64
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed) * 104729 + 3)
+
+    enu = antenna_enu(arr.n_ant, arr.radius_m, rng)
+    pairs = baseline_pairs(arr.n_ant)
+    lat = math.radians(arr.lat_deg)
+    e, n, up = enu[:, 0], enu[:, 1], enu[:, 2]
+    x = -math.sin(lat) * n + math.cos(lat) * up
+    y = e
+    z = math.cos(lat) * n + math.sin(lat) * up
+    bl = np.stack([x[pairs[:, 1]] - x[pairs[:, 0]], y[pairs[:, 1]] - y[pairs[:, 0]],
+                   z[pairs[:, 1]] - z[pairs[:, 0]]], axis=1)
+    n_chan = (n_planes + n_pol - 1) // n_pol
+    freqs = np.linspace(arr.freq_lo, arr.freq_hi, n_chan)
+    plane_freq = freqs[np.arange(n_planes) // n_pol]
+    off = math.radians(sky.max_offset_deg)
+    lm = rng.uniform(-off, off, size=(sky.n_src, 2))
+    flux = rng.uniform(sky.flux[0], sky.flux[1], size=sky.n_src)
+    fflag = rng.uniform(flag_range[0], flag_range[1], size=n_planes)
+    n_after = 10.0 ** rng.uniform(math.log10(counts[0]), math.log10(counts[1]), size=n_planes)
+    n_tot = np.maximum(1, np.round(n_after / (1.0 - fflag))).astype(np.int64)
+    bmax = float(np.max(np.linalg.norm(bl, axis=1)))
+    cell = 0.95 / (2.0 * bmax * arr.freq_hi / C_LIGHT)
+    dec = math.radians(arr.dec_deg)
+    sd, cd = math.sin(dec), math.cos(dec)
+    ha0, ha1 = (h * math.pi / 12.0 for h in arr.ha_range_h)
+    blt = torch.from_numpy(bl).to(dev)
+
+    sets = []
+    unflagged = 0
+    for s in sorted(set(supports)):
+        planes = [p for p in range(n_planes) if supports[p % len(supports)] == s]
+        cnt = torch.from_numpy(n_tot[planes]).to(dev)
+        total = int(cnt.sum().item())
+        plane = torch.repeat_interleave(torch.arange(len(planes), device=dev, dtype=torch.int32),
+                                        cnt)
+        pf = torch.from_numpy(plane_freq[planes]).to(dev)
+        pflag = torch.from_numpy(fflag[planes]).to(dev)
+        uvw = torch.empty((total, 3), dtype=torch.float64, device=dev)
+        vis = torch.empty((total, 1), dtype=cdtype, device=dev)
+        flag = torch.empty((total, 1), dtype=torch.uint8, device=dev)
+        for a in range(0, total, chunk):
+            b = min(total, a + chunk)
+            k = torch.randint(0, bl.shape[0], (b - a,), generator=gen, device=dev)
+            h = ha0 + (ha1 - ha0) * torch.rand((b - a,), generator=gen, device=dev,
+                                               dtype=torch.float64)
+            lx, ly, lz = blt[k, 0], blt[k, 1], blt[k, 2]
+            sh, ch = torch.sin(h), torch.cos(h)
+            uu = sh * lx + ch * ly
+            vv = -sd * ch * lx + sd * sh * ly + cd * lz
+            ww = cd * ch * lx - cd * sh * ly + sd * lz
+            uvw[a:b] = torch.stack([uu, vv, ww], dim=1)
+            f = pf[plane[a:b].long()] / C_LIGHT
+            acc = torch.zeros((b - a,), dtype=torch.complex128, device=dev)
+            for (l, m), fl in zip(lm, flux):
+                ph = (-2.0 * math.pi) * (uu * float(l) + vv * float(m)) * f
+                acc += float(fl) * torch.polar(torch.ones_like(ph), ph)
+            noise = torch.randn((b - a, 2), generator=gen, device=dev,
+                                dtype=torch.float64) * sky.noise
+            acc += torch.view_as_complex(noise)
+            vis[a:b, 0] = acc.to(cdtype)
+            fr = torch.rand((b - a,), generator=gen, device=dev, dtype=torch.float64)
+            flag[a:b, 0] = (fr < pflag[plane[a:b].long()]).to(torch.uint8)
+        weight = torch.empty((total, 1), dtype=rdtype, device=dev)
+        weight.uniform_(0.5, 1.5, generator=gen)
+        unflagged += total - int(flag.sum().item())
+        sets.append(PlaneSet(support=s, uvw=uvw, plane=plane, freq=pf, vis=vis, weight=weight,
+                             flag=flag))
+    info = dict(unflagged=unflagged, planes=n_planes, n_uv=n_uv, seed=seed,
+                plane_counts=n_tot.tolist())
+    return sets, (-cell, cell), info

@@ -47,6 +47,26 @@ def cpu_model():
     return platform.processor()
 
 
+def measured_traffic(kernel, visibilities, n_uv, mode, tile):
+    """HBM bytes per launch of `kernel` on this workload, as measured with
+    rocprofv3 counters (FETCH_SIZE + WRITE_SIZE, each scaled by the ratio of
+    a copy of known size to what the same counter reads for it) and recorded
+    in profiles/traffic.json; None when no measurement of this exact
+    workload and tile is recorded there."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for row in table.get("measurements", []):
+        if (row.get("kernel") == kernel and row.get("visibilities") == visibilities
+                and row.get("n_uv") == list(n_uv) and row.get("mode") == mode
+                and row.get("tile") == list(tile)):
+            return row.get("bytes_per_launch")
+    return None
+
+
 def cpu_baseline(obs, uvw_d, vis_d, w_d, flag_d, n_uv, cell, cube, budget_s=12.0):
     """Single-threaded FP64 oracle grid+degrid on a bounded sample: the first
     rows (time-major) in the first 4 channels, grown until the pair takes a
@@ -103,6 +123,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--time-samples", type=int, default=1000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="after the timed steps, copy a BYTES-sized device buffer once "
+                         "(the reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -111,7 +134,8 @@ def main():
     import torch
 
     from astroviper_amd import synth
-    from astroviper_amd.device import GridPlan, degrid_plain, grid_plain
+    from astroviper_amd.device import (GridPlan, degrid_kernel_name, degrid_plain,
+                                       grid_kernel_name, grid_plain)
 
     dev = torch.device("cuda", 0)
     cfg = synth.CONFIG2
@@ -183,9 +207,12 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     tdom, bdom = kern[dom]
     achieved = bdom / (tdom / 1e3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_grid_tiles" if dom == "grid" else "k_degrid_tiles",
+    names = {"grid": grid_kernel_name(args.tile, npol, compensated=args.compensated),
+             "degrid": degrid_kernel_name(args.tile, npol, 7)}
+    roofline = {"bound": "hbm", "kernel": names[dom],
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                "traffic": measured_traffic(names[dom], V, n_uv, args.mode, args.tile),
                 "algorithmic_bytes": bdom, "avg_ms": round(tdom, 3)}
 
     extra = {"plan_ms": round(t_plan, 3), "grid_ms": round(t_grid, 3),
@@ -235,6 +262,14 @@ def main():
         }
         log(f"bench: errors vs FP64 {extra['max_rel_err_vs_fp64']}")
         del g64, m64
+
+    if args.calibrate_copy > 0:
+        src = torch.empty(args.calibrate_copy // 4, dtype=torch.float32, device=dev)
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu:
