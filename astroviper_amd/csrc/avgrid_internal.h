@@ -45,6 +45,12 @@ __host__ __device__ inline uint32_t pack_geo(int au, int ou, int av_, int ov) {
   return (uint32_t)(au + kPosBias) | ((uint32_t)(ou + kOffBias) << 8) |
          ((uint32_t)(av_ + kPosBias) << 16) | ((uint32_t)(ov + kOffBias) << 24);
 }
+// A valid geometry with both centres at +239 (far outside any tile and
+// region) and zero kernel offsets: substituting it for kInvalidGeo gives
+// every tap weight zero through in-bounds table reads, with no branch.
+constexpr uint32_t kFarGeo = 0x80FF80FFu;
+__device__ inline uint32_t geo_or_far(uint32_t g) { return g == kInvalidGeo ? kFarGeo : g; }
+
 __device__ inline void unpack_geo(uint32_t g, int &au, int &ou, int &av_, int &ov) {
   au = (int)(g & 0xFFu) - kPosBias;
   ou = (int)((g >> 8) & 0xFFu) - kOffBias;

@@ -18,6 +18,54 @@ namespace av {
 
 constexpr int kDegridThreads = 256;
 constexpr int kDegridWaves = kDegridThreads / 64;
+constexpr int kRegionBatch = 8;  // grid cells each thread has in flight per pass
+
+// Copy the region [u0 - halo, u0 - halo + rows) x [v0 - halo, v0 - halo +
+// cols) of every visibility polarisation's grid plane (through pol_map) into
+// LDS, zero outside the grid and beyond (ru_real, rv_real).  The cells are
+// numbered polarisation-major, then row, then column, so consecutive threads
+// read consecutive cells of a row; each thread issues kRegionBatch loads
+// before it waits and writes them with store(p, row, col, value).
+template <typename R, int NPOL, typename Store>
+__device__ inline void load_region(const R *gplane, size_t pstride, int npol_g,
+                                   const int32_t *pol_map, int n_u, int n_v, int u_lo, int v_lo,
+                                   int rows, int cols, int ru_real, int rv_real, Store store) {
+  int pg[NPOL];
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) {
+    int q = pol_map ? pol_map[p] : p % npol_g;
+    pg[p] = q < 0 ? 0 : (q >= npol_g ? npol_g - 1 : q);
+  }
+  const int per_pol = rows * cols;
+  const int total = NPOL * per_pol;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kRegionBatch * kDegridThreads) {
+    Cplx2<R> val[kRegionBatch];
+#pragma unroll
+    for (int k = 0; k < kRegionBatch; ++k) {
+      const int e = e0 + k * kDegridThreads;
+      val[k] = Cplx2<R>{(R)0, (R)0};
+      if (e < total) {
+        const int p = e / per_pol, rc = e - p * per_pol;
+        const int ru = rc / cols, rv = rc - ru * cols;
+        const int gu = u_lo + ru, gv = v_lo + rv;
+        int pq = pg[0];
+#pragma unroll
+        for (int q = 1; q < NPOL; ++q) pq = p == q ? pg[q] : pq;
+        if (ru < ru_real && rv < rv_real && gu >= 0 && gu < n_u && gv >= 0 && gv < n_v)
+          val[k] = *(const Cplx2<R> *)(gplane + pq * pstride + ((size_t)gu * n_v + gv) * 2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRegionBatch; ++k) {
+      const int e = e0 + k * kDegridThreads;
+      if (e < total) {
+        const int p = e / per_pol, rc = e - p * per_pol;
+        const int ru = rc / cols, rv = rc - ru * cols;
+        store(p, ru, rv, val[k]);
+      }
+    }
+  }
+}
 
 struct DegridLds {
   int tab, reg, st_cu, st_cv, st_pos, st_item, total;
@@ -86,21 +134,11 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
   // ---- tile + halo into LDS in visibility-polarisation order (zero outside
-  // the grid); one wavefront per (polarisation, region row)
-  for (int row = wave; row < NPOL * L.ru; row += kDegridWaves) {
-    const int p = row / L.ru, ru = row - p * L.ru;
-    int pg = pol_map ? pol_map[p] : p % npol_g;
-    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);
-    const int gu = u0 - Hh + ru;
-    const bool urow = gu >= 0 && gu < P.n_u;
-    const Cplx2<R> *src = (const Cplx2<R> *)(gplane + pg * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2);
-    for (int rv = lane; rv < RV; rv += 64) {
-      const int gv = v0 - Hh + rv;
-      Cplx2<R> val{(R)0, (R)0};
-      if (urow && gv >= 0 && gv < P.n_v) val = src[gv];
-      *(Cplx2<R> *)(reg + (ru * RV + rv) * C + 2 * p) = val;
-    }
-  }
+  // the grid), cell-major
+  load_region<R, NPOL>(gplane, pstride, npol_g, pol_map, P.n_u, P.n_v, u0 - Hh, v0 - Hh, L.ru,
+                       RV, L.ru, RV, [&](int p, int ru, int rv, const Cplx2<R> &val) {
+                         *(Cplx2<R> *)(reg + (ru * RV + rv) * C + 2 * p) = val;
+                       });
   __syncthreads();
 
   // lanes: (sample slot s, v-tap dv, chunk h)
@@ -224,11 +262,15 @@ struct DegridMfma<double> {
 };
 
 struct DegridMfmaLds {
-  int tab, reg, total, ntab, halo, ru, rup, rv, rvp, rs;
+  int tab, reg, wvt, total, ntab, halo, ru, rup, rv, rvp, rs;
 };
 
 // Region rows padded to a multiple of 4 (one matrix step), columns to NB
-// blocks of 16 components.
+// blocks of 16 components.  Within a row, component n of the region is
+// stored at (n % 16) * NB + n / 16, so the NB B-operands of lane l (column
+// l % 16 of every block) are contiguous and load as 16-byte reads; rows are
+// padded by 16 bytes so that the four rows of one step fall in different
+// banks.
 template <typename R, int NPOL, int NB>
 __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support, int tu, int tv) {
   DegridMfmaLds L;
@@ -239,16 +281,41 @@ __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support,
   L.rup = (L.ru + 3) & ~3;
   L.rv = tv + 2 * L.halo;
   L.rvp = NB * 16 / (2 * NPOL);
-  L.rs = NB * 16;  // components per region row
+  L.rs = NB * 16 + 16 / (int)sizeof(R);  // elements per region row
   L.tab = 0;
   L.reg = (L.ntab * (int)sizeof(R) + 15) & ~15;
-  L.total = L.reg + L.rup * L.rs * (int)sizeof(R);
+  L.wvt = L.reg + L.rup * L.rs * (int)sizeof(R);  // per wave: [16 records][rvp]
+  L.total = L.wvt + kDegridWaves * 16 * L.rvp * (int)sizeof(R);
   return L;
 }
 
 template <int NPOL>
 __host__ __device__ constexpr int degrid_mfma_blocks_needed(int tv, int support) {
   return ((tv + 2 * (support / 2 + 1)) * 2 * NPOL + 15) / 16;
+}
+
+// The 16 records of one group: lane l loads record l % 16.
+struct DegridGroup {
+  uint32_t geo, item, fbits;
+};
+
+template <int NPOL>
+__device__ inline DegridGroup load_group(const PlanDev &P, uint32_t first, uint32_t end,
+                                         uint32_t g, int li, const uint8_t *flag) {
+  DegridGroup G{kInvalidGeo, 0u, 0u};
+  const uint32_t r = first + g * 16 + li;
+  if (r < end) {
+    G.geo = P.rec_d[r];
+    G.item = P.rec_idx[r];
+    if (G.geo != kInvalidGeo && flag) {
+      if (NPOL == 4)
+        G.fbits = *(const uint32_t *)(flag + (size_t)G.item * 4);
+      else
+        for (int p = 0; p < NPOL; ++p)
+          G.fbits |= (uint32_t)flag[(size_t)G.item * NPOL + p] << (8 * p);
+    }
+  }
+  return G;
 }
 
 template <typename R, int NPOL, int NB>
@@ -259,7 +326,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   using MO = DegridMfma<R>;
   using MV = typename MO::V;
   constexpr int C = 2 * NPOL;
-  constexpr int CPB = 16 / C;  // cells per 16-column block
+  constexpr int CPB = 16 / C;               // cells per 16-column block
+  constexpr int VL = 16 / (int)sizeof(R);   // elements per 16-byte read
   extern __shared__ __align__(16) char smem[];
   const uint32_t u = blockIdx.x;
   if (u >= P.dunit_start[P.n_bins]) return;
@@ -269,94 +337,86 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  R *wvt = (R *)(smem + L.wvt) + wave * 16 * L.rvp;
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
   const int Hh = L.halo, RS = L.rs;
-
-  for (int i = tid; i < L.ntab; i += kDegridThreads)
-    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+  const int li = lane & 15, lk = lane >> 4;
 
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, cnt = unit.z;
+  const uint32_t end = first + cnt;
+  const uint32_t ngroups = (cnt + 15) / 16;
+  // this wavefront's first group of records, in flight during the region load
+  DegridGroup cur = load_group<NPOL>(P, first, end, wave, li, flag);
+
+  for (int i = tid; i < L.ntab; i += kDegridThreads)
+    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
   const int ic = (int)(b % P.n_img_chan);
   const int64_t tile = b / P.n_img_chan;
   const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
   const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
-  // ---- tile + halo into LDS, cell-major in visibility-polarisation order,
-  // zero outside the grid and in the padding; one wavefront per (pol, row)
-  for (int row = wave; row < NPOL * L.rup; row += kDegridWaves) {
-    const int p = row / L.rup, ru = row - p * L.rup;
-    int pg = pol_map ? pol_map[p] : p % npol_g;
-    pg = pg < 0 ? 0 : (pg >= npol_g ? npol_g - 1 : pg);
-    const int gu = u0 - Hh + ru;
-    const bool urow = ru < L.ru && gu >= 0 && gu < P.n_u;
-    const Cplx2<R> *src =
-        (const Cplx2<R> *)(gplane + pg * pstride + (size_t)(urow ? gu : 0) * P.n_v * 2);
-    for (int rv = lane; rv < L.rvp; rv += 64) {
-      const int gv = v0 - Hh + rv;
-      Cplx2<R> val{(R)0, (R)0};
-      if (urow && rv < L.rv && gv >= 0 && gv < P.n_v) val = src[gv];
-      *(Cplx2<R> *)(reg + ru * RS + rv * C + 2 * p) = val;
-    }
-  }
+  // ---- tile + halo into LDS in visibility-polarisation order, zero outside
+  // the grid and in the padding
+  load_region<R, NPOL>(gplane, pstride, npol_g, pol_map, P.n_u, P.n_v, u0 - Hh, v0 - Hh, L.rup,
+                       L.rvp, L.ru, L.rv, [&](int p, int ru, int rv, const Cplx2<R> &val) {
+                         const int n = rv * C + 2 * p;  // n, n + 1 share a 16-column block
+                         R *dst = reg + ru * RS + (n & 15) * NB + (n >> 4);
+                         dst[0] = val.re;
+                         dst[NB] = val.im;
+                       });
   __syncthreads();
 
-  const int li = lane & 15, lk = lane >> 4;
-  const uint32_t end = first + cnt;
-  const uint32_t ngroups = (cnt + 15) / 16;
   const int ksteps = L.rup / 4;
   for (uint32_t g = wave; g < ngroups; g += kDegridWaves) {
-    // lane l holds record l%16 of the group (the A-operand record)
-    const uint32_t r = first + g * 16 + li;
-    uint32_t geo = kInvalidGeo, item = 0, fbits = 0;
-    if (r < end) {
-      geo = P.rec_d[r];
-      item = P.rec_idx[r];
-      if (geo != kInvalidGeo && flag) {
-        if (NPOL == 4)
-          fbits = *(const uint32_t *)(flag + (size_t)item * 4);
-        else
-          for (int p = 0; p < NPOL; ++p) fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
-      }
-    }
-    int au = -1000, ou = 0, avv = -1000, ovv = 0;
-    if (geo != kInvalidGeo) unpack_geo(geo, au, ou, avv, ovv);
+    const DegridGroup nxt = load_group<NPOL>(P, first, end, g + kDegridWaves, li, flag);
+    int au, ou, avv, ovv;
+    unpack_geo(geo_or_far(cur.geo), au, ou, avv, ovv);
     const int urow0 = au + Hh - sc;  // region row of the first u-tap
+    // v-weights of record li at every region column (lanes of one record
+    // split the columns)
+    const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
+    for (int j = lk; j < L.rvp; j += 4) wvt[li * L.rvp + j] = tap_weight(tab, ov, j - vcol0, S, sc, ovv);
 
     MV T[NB], T1 = MV{0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
     for (int s = 0; s < ksteps; ++s) {
       const int row = 4 * s + lk;
-      const int t = row - urow0;
-      const R a = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ou)] : (R)0;
-      const R *rp = reg + row * RS + li;
+      const R a = tap_weight(tab, ov, row - urow0, S, sc, ou);
+      const R *rp = reg + row * RS + li * NB;
+      R bv[NB];
 #pragma unroll
-      for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, rp[16 * k], T[k]);
+      for (int k = 0; k < NB; k += VL) {
+        const Vec<R, VL> t = *(const Vec<R, VL> *)(rp + k);
+#pragma unroll
+        for (int e = 0; e < VL; ++e) bv[k + e] = t.v[e];
+      }
+#pragma unroll
+      for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], T[k]);
       T1 = MO::mma(a, (R)1, T1);
     }
+    wave_lds_sync();
     // ---- weighted sums over v: lane holds records MO::row(l/16, q), cell
     // k*CPB + li/C, component li%C of every block k
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int rec = MO::row(lk, q);  // record slot in the group
-      const int avq = __shfl(avv, rec), ovq = __shfl(ovv, rec);
-      const int vcol0 = avq + Hh - sc;  // region column of the first v-tap
+      const uint32_t gq = (uint32_t)__shfl((int)cur.geo, rec);
+      const R *wrow = wvt + rec * L.rvp + li / C;
       R part = (R)0, npart = (R)0;
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
-        const int t = k * CPB + li / C - vcol0;
-        const R wv = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ovq)] : (R)0;
+        const R wv = wrow[k * CPB];
         part += T[k][q] * wv;
         npart += T1[q] * wv;
       }
       part = sum_over_dv<C, CPB>(part);
       npart = sum_over_dv<C, CPB>(npart);
-      const uint32_t gq = (uint32_t)__shfl((int)geo, rec);
-      const uint32_t itq = (uint32_t)__shfl((int)item, rec);
-      const uint32_t fq = (uint32_t)__shfl((int)fbits, rec);
+      const uint32_t itq = (uint32_t)__shfl((int)cur.item, rec);
+      const uint32_t fq = (uint32_t)__shfl((int)cur.fbits, rec);
       if (li >= 16 - C && gq != kInvalidGeo) {
         const int c = li - (16 - C);
         const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
@@ -368,6 +428,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
         }
       }
     }
+    wave_lds_sync();  // the next group rewrites this wavefront's weights
+    cur = nxt;
   }
 }
 

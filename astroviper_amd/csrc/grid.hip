@@ -413,7 +413,7 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support) {
   return L;
 }
 
-template <typename R, int NPOL, int MB, int VB, bool COMP>
+template <typename R, int NPOL, int MB, int VB, bool COMP, bool PSF>
 __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
                                                             const R *__restrict__ weight,
                                                             const uint8_t *__restrict__ flag,
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
   R *st_d = (R *)wbase;                           // [record][C]
   uint32_t *st_geo = (uint32_t *)(wbase + L.st_geo);  // [record]
 
-  const bool psf = (mode & AV_GRID_PSF) != 0;
+  constexpr bool psf = PSF;  // PSF data are real: the imaginary blocks are skipped
   const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
   const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
   const int half = ov / 2 + 2;
@@ -586,27 +586,16 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
     const int nst = (int)min((uint32_t)64, end - base);
     for (int s4 = 0; s4 < nst; s4 += 4) {
       const int rec = s4 + lk;
-      const uint32_t g = st_geo[rec];
+      // skipped records become a far-away geometry: all their weights are 0
+      const uint32_t g = geo_or_far(st_geo[rec]);
+      int au, ou, avv, ovv;
+      unpack_geo(g, au, ou, avv, ovv);
       R a[MB], wv[VB];
-      if (g != kInvalidGeo) {
-        int au, ou, avv, ovv;
-        unpack_geo(g, au, ou, avv, ovv);
 #pragma unroll
-        for (int m = 0; m < MB; ++m) {
-          const int t = 16 * m + li - au + sc;
-          a[m] = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ou)] : (R)0;
-        }
+      for (int m = 0; m < MB; ++m) a[m] = tap_weight(tab, ov, 16 * m + li - au + sc, S, sc, ou);
 #pragma unroll
-        for (int vb = 0; vb < VB; ++vb) {
-          const int t = 16 * vb + li - avv + sc;
-          wv[vb] = (t >= 0 && t < S) ? tab[tab_index(ov, t, sc, ovv)] : (R)0;
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < MB; ++m) a[m] = (R)0;
-#pragma unroll
-        for (int vb = 0; vb < VB; ++vb) wv[vb] = (R)0;
-      }
+      for (int vb = 0; vb < VB; ++vb)
+        wv[vb] = tap_weight(tab, ov, 16 * vb + li - avv + sc, S, sc, ovv);
       R dd[C];
 #pragma unroll
       for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {
@@ -617,7 +606,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
       }
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        if (psf && (c & 1)) continue;  // PSF data are real
+        if constexpr (PSF) {
+          if (c & 1) continue;
+        }
 #pragma unroll
         for (int vb = 0; vb < VB; ++vb) {
           const R bv = wv[vb] * dd[c];
@@ -763,17 +754,17 @@ static int launch_grid(av_plan *plan, const void *vis, const void *weight, const
   return AV_OK;
 }
 
-template <typename R, int NPOL, int MB, int VB, bool COMP>
+template <typename R, int NPOL, int MB, int VB, bool COMP, bool PSF>
 static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
                             const uint8_t *flag, const double *cgk, void *grid, double *sumwt,
                             int mode, hipStream_t s) {
   PlanDev &d = plan->d;
-  auto kfn = k_grid_mfma<R, NPOL, MB, VB, COMP>;
+  auto kfn = k_grid_mfma<R, NPOL, MB, VB, COMP, PSF>;
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
   const MfmaLds L = mfma_lds_layout<R, NPOL>(d.ov, d.support);
   {
-    const int rc_lds = ensure_lds_limit<k_grid_mfma<R, NPOL, MB, VB, COMP>>(L.total, "av_grid: hipFuncSetAttribute");
+    const int rc_lds = ensure_lds_limit<k_grid_mfma<R, NPOL, MB, VB, COMP, PSF>>(L.total, "av_grid: hipFuncSetAttribute");
     if (rc_lds) return rc_lds;
   }
   int rc = prepare_tables(d, cgk, s);
@@ -804,8 +795,11 @@ static int try_mfma(av_plan *plan, const void *vis, const void *weight, const ui
   if constexpr (MB * 2 * NPOL * VB <= mfma_max_blocks<R, COMP>()) {
     if (plan->d.tile_u == 16 * MB && plan->d.tile_v == 16 * VB) {
       done = true;
-      return launch_grid_mfma<R, NPOL, MB, VB, COMP>(plan, vis, weight, flag, cgk, grid, sumwt,
-                                                     mode, s);
+      if (mode & AV_GRID_PSF)
+        return launch_grid_mfma<R, NPOL, MB, VB, COMP, true>(plan, vis, weight, flag, cgk, grid,
+                                                             sumwt, mode, s);
+      return launch_grid_mfma<R, NPOL, MB, VB, COMP, false>(plan, vis, weight, flag, cgk, grid,
+                                                            sumwt, mode, s);
     }
   }
   return AV_OK;

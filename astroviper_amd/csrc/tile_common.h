@@ -88,4 +88,17 @@ __device__ inline int tab_index(int ov, int tap, int sc, int off) {
   return k < 0 ? -k : k;
 }
 
+// Kernel weight of tap t at offset off, or 0 when t is outside [0, S).  The
+// table read uses the clamped tap, so it is always in bounds, and the result
+// is multiplied by a 0/1 mask: every lane issues the read, and the reads of
+// several taps can be in flight together instead of each sitting behind a
+// branch.
+template <typename R>
+__device__ inline R tap_weight(const R *tab, int ov, int t, int S, int sc, int off) {
+  const int tc = t < 0 ? 0 : (t >= S ? S - 1 : t);
+  const R w = tab[tab_index(ov, tc, sc, off)];
+  const R in = (t >= 0 && t < S) ? (R)1 : (R)0;
+  return w * in;
+}
+
 }  // namespace av

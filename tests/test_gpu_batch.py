@@ -81,14 +81,15 @@ def test_graph_replays_match_eager(gpu):
         # eager work between replays
         x = torch.randn(1 << 16, device=gpu).sum().item()
         assert np.isfinite(x)
-        # bins split over several units add their tiles with float atomics,
-        # so grids (and the model visibilities degridded from them) agree to
-        # FP32 rounding; sums of weights are reduced in a fixed order
+        # binning fills each bin through atomic cursors, so the order of the
+        # records in a bin (and with it the order of the FP32 tile sums and
+        # the FP64 weight sums) can differ from run to run: grids and model
+        # visibilities agree to FP32 rounding, sums of weights to FP64
         for g, r in zip(bg.grids, ref_g):
             assert torch.isfinite(g).all(), k
             assert rel(g.cpu().numpy(), r.cpu().numpy()) < 1e-6, k
         for w, r in zip(bg.sumwts, ref_w):
-            assert torch.equal(w, r), k
+            assert rel(w.cpu().numpy(), r.cpu().numpy()) < 1e-12, k
         for m, r in zip(bg.models, ref_m):
             assert torch.isfinite(m).all(), k
             assert rel(m.cpu().numpy(), r.cpu().numpy()) < 1e-6, k
