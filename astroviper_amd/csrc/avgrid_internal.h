@@ -37,6 +37,9 @@ constexpr int kOffBias = 128;  // kernel offsets are stored + 128
 constexpr int kMaxSupport = 16;
 constexpr int kMaxPol = 4;
 constexpr int kMaxDevices = 64;
+constexpr int kChunkThreads = 1024;       // threads of a chunked-binning workgroup
+constexpr int kMaxChunkTiles = 16384;     // tiles whose counters fit in 64 KB of LDS
+constexpr int kMaxRowsPerChunk = 31 * kChunkThreads;
 
 // Packed geometry of one sample relative to a tile origin:
 // bits 0-7 u position + 16, 8-15 u offset + 128, 16-23 v position + 16,
@@ -145,12 +148,19 @@ struct PlanDev {
   uint32_t *gunit_start, *dunit_start;  // [n_bins + 1]
   uint32_t *scan_tmp;               // block sums
   // records (SoA)
-  uint32_t *rec_idx, *rec_g, *rec_d;  // [max_records]
+  uint2 *rec;                       // [max_records] {sample, packed grid geometry}
+  uint32_t *rec_d;                  // [max_records] packed degrid geometry (primaries only)
   // work units: x = bin, y = first record, z = count, w = flags
   uint4 *gunits, *dunits;
-  // validity bitmasks, one bit per sample: grid-valid, degrid-valid
+  // validity bitmasks, one bit per sample: grid-valid, degrid-valid; bit
+  // (chan, row) is bit row % 64 of word chan * mask_words_per_chan + row / 64
+  // (chan 0 when rows carry their own channel)
   unsigned long long *gmask, *dmask;  // [n_words]
-  int64_t n_words;
+  int64_t n_words, mask_words_per_chan;
+  // chunked binning: a chunk is rows_per_chunk rows of one channel
+  int32_t chunked;                  // chunked binning possible for this plan
+  int32_t rows_per_chunk, chunks_per_chan;
+  uint32_t *chunk_hist;             // [n_chan * chunks_per_chan][ntu * ntv]
   // per-call tables: FP32 kernel table, FP64 per-axis kernel sums by offset
   float *tab32;                     // [ov * (support/2 + 1)]
   double *nrm;                      // [2 * (ov/2 + 2) + 1]

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
         int au, ou, avv, ovv;
         unpack_geo(g, au, ou, avv, ovv);
         pos = (au + kPosBias) | ((avv + kPosBias) << 16);
-        st_item[lane] = P.rec_idx[r];
+        st_item[lane] = P.rec[r].x;
 #pragma unroll
         for (int t = 0; t < DVW; ++t) {
           st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
@@ -306,7 +306,7 @@ __device__ inline DegridGroup load_group(const PlanDev &P, uint32_t first, uint3
   const uint32_t r = first + g * 16 + li;
   if (r < end) {
     G.geo = P.rec_d[r];
-    G.item = P.rec_idx[r];
+    G.item = P.rec[r].x;
     if (G.geo != kInvalidGeo && flag) {
       if (NPOL == 4)
         G.fbits = *(const uint32_t *)(flag + (size_t)G.item * 4);
@@ -440,8 +440,13 @@ __global__ void k_zero_invalid(PlanDev P, R *vis_out, int npol) {
   const int64_t n = P.n_items;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned long long m = P.dmask[i >> 6];
-    if (!((m >> (i & 63)) & 1ull)) {
+    int64_t row = i, chan = 0;
+    if (!P.row_chan) {
+      row = i / P.n_chan;
+      chan = i - row * P.n_chan;
+    }
+    const unsigned long long m = P.dmask[chan * P.mask_words_per_chan + (row >> 6)];
+    if (!((m >> (row & 63)) & 1ull)) {
       R *o = vis_out + (size_t)i * npol * 2;
       for (int k = 0; k < 2 * npol; ++k) o[k] = (R)0;
     }

@@ -182,12 +182,14 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   const uint32_t end = first + cnt;
   uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
   if (first + lane < end) {
-    g0 = P.rec_g[first + lane];
-    i0 = P.rec_idx[first + lane];
+    const uint2 r = P.rec[first + lane];
+    i0 = r.x;
+    g0 = r.y;
   }
   if (first + 64 + lane < end) {
-    g1 = P.rec_g[first + 64 + lane];
-    i1 = P.rec_idx[first + 64 + lane];
+    const uint2 r = P.rec[first + 64 + lane];
+    i1 = r.x;
+    g1 = r.y;
   }
   RawSample<R, NPOL> raw0;
   fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
@@ -248,8 +250,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
     fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
     uint32_t g2 = kInvalidGeo, i2 = 0;
     if (base + 128 + lane < end) {
-      g2 = P.rec_g[base + 128 + lane];
-      i2 = P.rec_idx[base + 128 + lane];
+      const uint2 r = P.rec[base + 128 + lane];
+      i2 = r.x;
+      g2 = r.y;
     }
     wave_lds_sync();
     // ---- apply the staged samples one at a time
@@ -514,12 +517,14 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
   const uint32_t end = first + cnt;
   uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
   if (first + lane < end) {
-    g0 = P.rec_g[first + lane];
-    i0 = P.rec_idx[first + lane];
+    const uint2 r = P.rec[first + lane];
+    i0 = r.x;
+    g0 = r.y;
   }
   if (first + 64 + lane < end) {
-    g1 = P.rec_g[first + 64 + lane];
-    i1 = P.rec_idx[first + 64 + lane];
+    const uint2 r = P.rec[first + 64 + lane];
+    i1 = r.x;
+    g1 = r.y;
   }
   RawSample<R, NPOL> raw0;
   fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
@@ -578,8 +583,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
     fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
     uint32_t g2 = kInvalidGeo, i2 = 0;
     if (base + 128 + lane < end) {
-      g2 = P.rec_g[base + 128 + lane];
-      i2 = P.rec_idx[base + 128 + lane];
+      const uint2 r = P.rec[base + 128 + lane];
+      i2 = r.x;
+      g2 = r.y;
     }
     wave_lds_sync();
     // ---- four records per matrix-core step

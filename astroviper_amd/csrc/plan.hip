@@ -9,6 +9,14 @@
 // by exactly one owner with no halo exchange.  Within a bin, primary records
 // come first.  Bins are ordered tile-major, channel-minor, so neighbouring
 // channels of the same rows are processed close together in time.
+//
+// Binning is a counting sort.  The counts come from chunks (one channel, a
+// range of rows) gathered in LDS when a channel's tiles fit there, else from
+// global atomics (per-row channels, very many tiles).  The records are then
+// written with one cursor per bin, the samples taken in order so that the
+// records a wavefront writes land in neighbouring bins.  Validity bitmasks are channel-major: bit (chan, row) is
+// bit row % 64 of word chan * ceil(n_rows / 64) + row / 64 (chan 0 when
+// every row carries its own channel).
 #include <algorithm>
 
 #include "avgrid_internal.h"
@@ -23,14 +31,11 @@ struct ItemGeo {
   int lu, lv, fu, fv;  // degrid centre and kernel offsets
 };
 
-__device__ inline void item_geometry(const PlanDev &P, int64_t item, ItemGeo &g) {
-  if (P.row_chan) {
-    g.row = item;
-    g.chan = P.row_chan[item];
-  } else {
-    g.row = item / P.n_chan;
-    g.chan = (int)(item - g.row * P.n_chan);
-  }
+// Geometry of the sample of row `row` at channel `chan` (the row's own
+// channel when rows carry one).
+__device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, ItemGeo &g) {
+  g.row = row;
+  g.chan = P.row_chan ? P.row_chan[row] : chan;
   bool chan_ok = g.chan >= 0 && g.chan < P.n_chan;
   int ch = chan_ok ? g.chan : 0;
   g.ic = P.chan_map[ch];
@@ -110,24 +115,37 @@ __device__ inline RecPlan record_plan(const PlanDev &P, const ItemGeo &g) {
   return r;
 }
 
-// Pass 1: count records per bin.
+__device__ inline int64_t item_index(const PlanDev &P, int64_t row, int chan) {
+  return P.row_chan ? row : row * P.n_chan + chan;
+}
+
+// The validity bits of a wavefront's 64 consecutive rows of one channel
+// (rows from a multiple of 64): one mask word, written by lane 0.
+__device__ inline void write_masks(const PlanDev &P, int64_t row, int chan, bool gvalid,
+                                   bool dvalid) {
+  const unsigned long long gm = __ballot(gvalid), dm = __ballot(dvalid);
+  const int64_t first = row - __lane_id();
+  if (__lane_id() == 0 && first < P.n_rows) {
+    const int64_t w = (P.row_chan ? 0 : (int64_t)chan * P.mask_words_per_chan) + first / 64;
+    P.gmask[w] = gm;
+    P.dmask[w] = dm;
+  }
+}
+
+// Global-atomic binning, pass 1: count records per bin.  Threads walk the
+// (channel, row) space with rows padded to a multiple of 64 per channel.
 __global__ __launch_bounds__(256) void k_count(PlanDev P) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t nitems = P.n_items;
-  // keep whole waves iterating together so the ballots see every lane
-  int64_t nround = ((nitems + stride - 1) / stride) * stride;
-  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < nround; it += stride) {
-    bool live = it < nitems;
+  const int64_t rows_pad = P.mask_words_per_chan * 64;
+  const int nch = P.row_chan ? 1 : P.n_chan;
+  const int64_t total = rows_pad * nch;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;  // a multiple of 64
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < total; it += stride) {
+    const int chan = (int)(it / rows_pad);
+    const int64_t row = it - (int64_t)chan * rows_pad;
     ItemGeo g;
     g.gvalid = g.dvalid = false;
-    if (live) item_geometry(P, it, g);
-    // a wave covers 64 consecutive samples starting at a multiple of 64
-    const unsigned long long gm = __ballot(g.gvalid), dm = __ballot(g.dvalid);
-    const int64_t first = it - __lane_id();
-    if (__lane_id() == 0 && first < nitems) {
-      P.gmask[first >> 6] = gm;
-      P.dmask[first >> 6] = dm;
-    }
+    if (row < P.n_rows) sample_geometry(P, row, chan, g);
+    write_masks(P, row, chan, g.gvalid, g.dvalid);
     if (!(g.gvalid || g.dvalid)) continue;
     const RecPlan rp = record_plan(P, g);
     if (rp.gprim) {
@@ -154,41 +172,113 @@ __global__ void k_cursors(PlanDev P) {
   P.cur_dup[b] = s + P.cnt_prim[b];
 }
 
-// Pass 2: write the records.
+// One 8-byte store per record; the degrid geometry is stored for primary
+// records only (degrid units cover the primary records of a bin).
+__device__ inline void write_records(const PlanDev &P, const ItemGeo &g, const RecPlan &rp,
+                                     int64_t item, uint32_t slot, int a, int b, bool primary) {
+  const int u0 = a * P.tile_u, v0 = b * P.tile_v;
+  P.rec[slot] = make_uint2((uint32_t)item, pack_geo(g.iu - u0, g.ou, g.iv - v0, g.ov));
+  if (primary)
+    P.rec_d[slot] = rp.share ? pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv) : kInvalidGeo;
+}
+
+__device__ inline void write_degrid_record(const PlanDev &P, const ItemGeo &g, const RecPlan &rp,
+                                           int64_t item, uint32_t slot) {
+  const int u0 = rp.du_t * P.tile_u, v0 = rp.dv_t * P.tile_v;
+  P.rec[slot] = make_uint2((uint32_t)item, kInvalidGeo);
+  P.rec_d[slot] = pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv);
+}
+
+// Binning pass 2: write the records, taking slots from the bins' cursors.
+// Threads walk the samples in order, so a wavefront covers the channels of
+// one or a few rows: their bins are neighbours (bins are channel-minor) and
+// so are the records they write.
 __global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t nitems = P.n_items;
-  int64_t nround = ((nitems + stride - 1) / stride) * stride;
-  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < nround; it += stride) {
-    bool live = it < nitems;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < P.n_items;
+       item += stride) {
+    int64_t row = item;
+    int chan = 0;
+    if (!P.row_chan) {
+      row = item / P.n_chan;
+      chan = (int)(item - row * P.n_chan);
+    }
     ItemGeo g;
-    g.gvalid = g.dvalid = false;
-    if (live) item_geometry(P, it, g);
+    sample_geometry(P, row, chan, g);
     if (!(g.gvalid || g.dvalid)) continue;
     const RecPlan rp = record_plan(P, g);
     if (rp.gprim) {
-      const int u0 = rp.tu * P.tile_u, v0 = rp.tv * P.tile_v;
       const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.tu, rp.tv, g.ic)], 1u);
-      P.rec_idx[slot] = (uint32_t)it;
-      P.rec_g[slot] = pack_geo(g.iu - u0, g.ou, g.iv - v0, g.ov);
-      P.rec_d[slot] = rp.share ? pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv) : kInvalidGeo;
+      write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
       for (int a = rp.tu0; a <= rp.tu1; ++a)
         for (int b = rp.tv0; b <= rp.tv1; ++b)
           if (a != rp.tu || b != rp.tv) {
             const uint32_t s2 = atomicAdd(&P.cur_dup[bin_of(P, a, b, g.ic)], 1u);
-            P.rec_idx[s2] = (uint32_t)it;
-            P.rec_g[s2] = pack_geo(g.iu - a * P.tile_u, g.ou, g.iv - b * P.tile_v, g.ov);
-            P.rec_d[s2] = kInvalidGeo;
+            write_records(P, g, rp, item, s2, a, b, false);
           }
     }
     if (rp.dsep) {
-      const int u0 = rp.du_t * P.tile_u, v0 = rp.dv_t * P.tile_v;
       const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.du_t, rp.dv_t, g.ic)], 1u);
-      P.rec_idx[slot] = (uint32_t)it;
-      P.rec_g[slot] = kInvalidGeo;
-      P.rec_d[slot] = pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv);
+      write_degrid_record(P, g, rp, item, slot);
     }
   }
+}
+
+// ---- chunked binning ------------------------------------------------------
+// A chunk is rows [r0, r0 + rows_per_chunk) of one channel.  Per tile it
+// holds (primaries) | (duplicates << 16): a row adds at most two primaries
+// and one duplicate to a tile, and rows_per_chunk <= 31744, so neither half
+// overflows.
+
+// Chunked pass 1: per-tile counts of one chunk, gathered in LDS.
+__global__ __launch_bounds__(kChunkThreads) void k_count_chunks(PlanDev P) {
+  extern __shared__ uint32_t h[];  // [n_tiles]
+  const int64_t w = blockIdx.x;
+  const int chan = (int)(w / P.chunks_per_chan);
+  const int64_t r0 = (w - (int64_t)chan * P.chunks_per_chan) * P.rows_per_chunk;
+  const int ntiles = P.ntu * P.ntv;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  for (int64_t row = r0 + threadIdx.x; row < r0 + P.rows_per_chunk; row += blockDim.x) {
+    ItemGeo g;
+    g.gvalid = g.dvalid = false;
+    if (row < P.n_rows) sample_geometry(P, row, chan, g);
+    write_masks(P, row, chan, g.gvalid, g.dvalid);
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    if (rp.gprim) {
+      atomicAdd(&h[rp.tu * P.ntv + rp.tv], 1u);
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) atomicAdd(&h[a * P.ntv + b], 1u << 16);
+    }
+    if (rp.dsep) atomicAdd(&h[rp.du_t * P.ntv + rp.dv_t], 1u);
+  }
+  __syncthreads();
+  uint32_t *dst = P.chunk_hist + w * ntiles;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) dst[i] = h[i];
+}
+
+// Chunked pass 2: the counts of every bin, summed over the chunks of the
+// channels that map to its image channel.
+__global__ void k_chunk_totals(PlanDev P) {
+  const int ntiles = P.ntu * P.ntv;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)P.n_img_chan * ntiles) return;
+  const int ic = (int)(id / ntiles);
+  const int t = (int)(id - (int64_t)ic * ntiles);
+  uint32_t sp = 0, sd = 0;
+  for (int c = 0; c < P.n_chan; ++c) {
+    if (P.chan_map[c] != ic) continue;
+    for (int k = 0; k < P.chunks_per_chan; ++k) {
+      const uint32_t hv = P.chunk_hist[((int64_t)c * P.chunks_per_chan + k) * ntiles + t];
+      sp += hv & 0xFFFFu;
+      sd += hv >> 16;
+    }
+  }
+  const int64_t b = (int64_t)t * P.n_img_chan + ic;
+  P.cnt_prim[b] = sp;
+  P.cnt_all[b] = sp + sd;
 }
 
 // units per bin: grid units cover every bin (an empty bin still owns its
@@ -326,6 +416,16 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
 
   const int64_t nb = d.n_bins;
   int64_t scan_blocks = (std::max<int64_t>(nb, n_items) + 4095) / 4096 + 1;
+  d.mask_words_per_chan = (n_rows + 63) / 64;
+  const int64_t mask_words = d.mask_words_per_chan * n_chan + 1;
+  // chunked binning when a channel's tiles fit in LDS
+  const int64_t ntiles = (int64_t)d.ntu * d.ntv;
+  d.chunked = ntiles <= kMaxChunkTiles && n_rows > 0;
+  d.rows_per_chunk = (int32_t)std::min<int64_t>(
+      kMaxRowsPerChunk, std::max<int64_t>(kChunkThreads,
+                                          (n_rows + kChunkThreads - 1) / kChunkThreads * kChunkThreads));
+  d.chunks_per_chan = (int32_t)((n_rows + d.rows_per_chunk - 1) / d.rows_per_chunk);
+  const int64_t chunk_cells = d.chunked ? (int64_t)n_chan * d.chunks_per_chan * ntiles : 0;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -337,10 +437,10 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_call = take(4 * nb), o_cprim = take(4 * nb), o_bstart = take(4 * (nb + 1)),
          o_curp = take(4 * nb), o_curd = take(4 * nb), o_gus = take(4 * (nb + 1)),
          o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
-         o_ridx = take(4 * d.max_records), o_rg = take(4 * d.max_records),
-         o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
+         o_rec = take(8 * d.max_records), o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
          o_dun = take(sizeof(uint4) * d.max_dunits),
-         o_gm = take(8 * ((n_items + 63) / 64 + 1)), o_dm = take(8 * ((n_items + 63) / 64 + 1)),
+         o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
+         o_chh = take(4 * chunk_cells),
          o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
          o_nrm = take(sizeof(double) * (2 * (oversampling / 2 + 2) + 1)),
          o_upart = take(sizeof(double) * 4 * d.max_gunits), o_ctr = take(4 * 16);
@@ -366,14 +466,14 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.gunit_start = (uint32_t *)(c + o_gus);
   d.dunit_start = (uint32_t *)(c + o_dus);
   d.scan_tmp = (uint32_t *)(c + o_scan);
-  d.rec_idx = (uint32_t *)(c + o_ridx);
-  d.rec_g = (uint32_t *)(c + o_rg);
+  d.rec = (uint2 *)(c + o_rec);
   d.rec_d = (uint32_t *)(c + o_rd);
   d.gunits = (uint4 *)(c + o_gun);
   d.dunits = (uint4 *)(c + o_dun);
   d.gmask = (unsigned long long *)(c + o_gm);
   d.dmask = (unsigned long long *)(c + o_dm);
-  d.n_words = (n_items + 63) / 64;
+  d.n_words = mask_words - 1;
+  d.chunk_hist = chunk_cells ? (uint32_t *)(c + o_chh) : nullptr;
   d.tab32 = (float *)(c + o_tab);
   d.nrm = (double *)(c + o_nrm);
   d.upart = (double *)(c + o_upart);
@@ -401,23 +501,43 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   d.row_chan = (int32_t *)row_chan;
   if (row_chan) d.n_items = d.n_rows;  // one sample per row
   else d.n_items = d.n_rows * (int64_t)d.n_chan;
-  d.n_words = (d.n_items + 63) / 64;
+  d.n_words = d.mask_words_per_chan * (row_chan ? 1 : d.n_chan);
   hipLaunchKernelGGL(k_chan_setup, dim3((d.n_chan + 255) / 256), dim3(256), 0, s, d, freq,
                      chan_map);
   AV_LAUNCH_CHECK("k_chan_setup");
   int rc;
-  if ((rc = launch_fill_u32(d.cnt_all, 0u, d.n_bins, s))) return rc;
-  if ((rc = launch_fill_u32(d.cnt_prim, 0u, d.n_bins, s))) return rc;
-  int64_t blocks = (d.n_items + 255) / 256;
-  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
-  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(256), 0, s, d);
-  AV_LAUNCH_CHECK("k_count");
+  const unsigned bb = (unsigned)((d.n_bins + 255) / 256);
+  // pass 1: counts per bin
+  if (d.chunked && !row_chan) {
+    const int64_t nchunks = (int64_t)d.n_chan * d.chunks_per_chan;
+    const int lds = (int)(sizeof(uint32_t) * d.ntu * d.ntv);
+    if ((rc = ensure_lds_limit<k_count_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
+      return rc;
+    hipLaunchKernelGGL(k_count_chunks, dim3((unsigned)nchunks), dim3(kChunkThreads), lds, s, d);
+    AV_LAUNCH_CHECK("k_count_chunks");
+    const int64_t pairs = (int64_t)d.n_img_chan * d.ntu * d.ntv;
+    hipLaunchKernelGGL(k_chunk_totals, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s,
+                       d);
+    AV_LAUNCH_CHECK("k_chunk_totals");
+  } else {
+    if ((rc = launch_fill_u32(d.cnt_all, 0u, d.n_bins, s))) return rc;
+    if ((rc = launch_fill_u32(d.cnt_prim, 0u, d.n_bins, s))) return rc;
+    const int64_t span = d.mask_words_per_chan * 64 * (row_chan ? 1 : d.n_chan);
+    int64_t blocks = (span + 255) / 256;
+    blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
+    hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_count");
+  }
+  // pass 2: bin offsets, cursors, records
   if ((rc = scan_exclusive_u32(d.cnt_all, d.bin_start, d.n_bins, d.scan_tmp, s))) return rc;
-  unsigned bb = (unsigned)((d.n_bins + 255) / 256);
   hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
   AV_LAUNCH_CHECK("k_cursors");
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, s, d);
-  AV_LAUNCH_CHECK("k_scatter");
+  {
+    int64_t blocks = (d.n_items + 255) / 256;
+    blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_scatter");
+  }
   // cursors are free again: reuse them for the per-bin unit counts
   hipLaunchKernelGGL(k_unit_counts, dim3(bb), dim3(256), 0, s, d, d.cur_prim, d.cur_dup);
   AV_LAUNCH_CHECK("k_unit_counts");

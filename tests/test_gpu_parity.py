@@ -198,6 +198,32 @@ def test_grid_shapes_supports_units(gpu, oracle, tile, support, npol, cap):
             assert rel(sw.cpu().numpy(), swref) < 1e-12
 
 
+@pytest.mark.parametrize("n,tile", [(2080, (16, 16)), (256, (16, 16))])
+def test_binning_paths(gpu, oracle, n, tile):
+    """Plans with more tiles than fit the chunked binning's LDS counters
+    (130 x 130 > 16384) take the global-atomic path; smaller ones the
+    chunked path (the 27-antenna data have 35100 rows: two chunks per
+    channel).  Both must give the oracle's grid, sums of weights and
+    degridded samples."""
+    uvw, freq, vis, w, flag, _, _ = c1_data(flag_frac=0.2, seed=4)
+    maxuv = np.max(np.hypot(uvw[..., 0], uvw[..., 1]))
+    c = 0.45 * C / freq[-1] / maxuv  # every sample within 0.45 n of the centre
+    cell = (-c, c)
+    w = np.where(flag, 0.0, w)
+    for cube in (False, True):
+        gref, swref = oracle.standard_grid(vis, uvw, w, freq, (n, n), cell,
+                                           chan_mode="cube" if cube else "continuum")
+        g, sw, plan = device_grid(gpu, vis, uvw, w, None, freq, (n, n), cell, cube, 64,
+                                  tile=tile)
+        assert rel(g.cpu().numpy(), gref) < 1e-12, cube
+        assert rel(sw.cpu().numpy(), swref) < 1e-12, cube
+        st = plan.stats()
+        assert st["grid_valid"] > 0.5 * st["samples"]
+        m = plan.degrid(g).cpu().numpy()
+        mref = oracle.degrid(uvw, flag, freq, gref, cell)
+        assert rel(m.reshape(mref.shape)[~flag], mref[~flag]) < 1e-10
+
+
 def test_grid_exact_ties(gpu, oracle):
     # f = c and cell = 1/N make every position an exact half-integer
     n = 64
