@@ -30,38 +30,43 @@ template <typename R, int NPOL, typename Store>
 __device__ inline void load_region(const R *gplane, size_t pstride, int npol_g,
                                    const int32_t *pol_map, int n_u, int n_v, int u_lo, int v_lo,
                                    int rows, int cols, int ru_real, int rv_real, Store store) {
-  int pg[NPOL];
+  constexpr int KB = kRegionBatch / NPOL > 0 ? kRegionBatch / NPOL : 1;  // cells per pol
+  const R *src[NPOL];
 #pragma unroll
   for (int p = 0; p < NPOL; ++p) {
     int q = pol_map ? pol_map[p] : p % npol_g;
-    pg[p] = q < 0 ? 0 : (q >= npol_g ? npol_g - 1 : q);
+    q = q < 0 ? 0 : (q >= npol_g ? npol_g - 1 : q);
+    src[p] = gplane + q * pstride;
   }
+  // e / cols by multiply-high, exact for e < 2^16 and 2 <= cols (a region is
+  // at least 6 cells wide: tile >= support >= 2 plus a halo of >= 2 per side)
   const int per_pol = rows * cols;
-  const int total = NPOL * per_pol;
-  for (int e0 = threadIdx.x; e0 < total; e0 += kRegionBatch * kDegridThreads) {
-    Cplx2<R> val[kRegionBatch];
+  const uint32_t magic = 0xFFFFFFFFu / (uint32_t)cols + 1u;
+  // the whole real region inside the grid: no per-cell bounds test
+  const bool inside = u_lo >= 0 && v_lo >= 0 && u_lo + ru_real <= n_u && v_lo + rv_real <= n_v;
+  for (int e0 = threadIdx.x; e0 < per_pol; e0 += KB * kDegridThreads) {
+    Cplx2<R> val[NPOL][KB];
+    int ru[KB], rv[KB];
 #pragma unroll
-    for (int k = 0; k < kRegionBatch; ++k) {
+    for (int k = 0; k < KB; ++k) {
       const int e = e0 + k * kDegridThreads;
-      val[k] = Cplx2<R>{(R)0, (R)0};
-      if (e < total) {
-        const int p = e / per_pol, rc = e - p * per_pol;
-        const int ru = rc / cols, rv = rc - ru * cols;
-        const int gu = u_lo + ru, gv = v_lo + rv;
-        int pq = pg[0];
+      ru[k] = (int)__umulhi((uint32_t)e, magic);
+      rv[k] = e - ru[k] * cols;
+      const int gu = u_lo + ru[k], gv = v_lo + rv[k];
+      const bool ok = e < per_pol && ru[k] < ru_real && rv[k] < rv_real &&
+                      (inside || (gu >= 0 && gu < n_u && gv >= 0 && gv < n_v));
+      const size_t off = ok ? ((size_t)gu * n_v + gv) * 2 : 0;
 #pragma unroll
-        for (int q = 1; q < NPOL; ++q) pq = p == q ? pg[q] : pq;
-        if (ru < ru_real && rv < rv_real && gu >= 0 && gu < n_u && gv >= 0 && gv < n_v)
-          val[k] = *(const Cplx2<R> *)(gplane + pq * pstride + ((size_t)gu * n_v + gv) * 2);
+      for (int p = 0; p < NPOL; ++p) {
+        val[p][k] = Cplx2<R>{(R)0, (R)0};
+        if (ok) val[p][k] = *(const Cplx2<R> *)(src[p] + off);
       }
     }
 #pragma unroll
-    for (int k = 0; k < kRegionBatch; ++k) {
-      const int e = e0 + k * kDegridThreads;
-      if (e < total) {
-        const int p = e / per_pol, rc = e - p * per_pol;
-        const int ru = rc / cols, rv = rc - ru * cols;
-        store(p, ru, rv, val[k]);
+    for (int k = 0; k < KB; ++k) {
+      if (e0 + k * kDegridThreads < per_pol) {
+#pragma unroll
+        for (int p = 0; p < NPOL; ++p) store(p, ru[k], rv[k], val[p][k]);
       }
     }
   }
@@ -282,10 +287,10 @@ __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support,
   L.rv = tv + 2 * L.halo;
   L.rvp = NB * 16 / (2 * NPOL);
   L.rs = NB * 16 + 16 / (int)sizeof(R);  // elements per region row
-  L.tab = 0;
-  L.reg = (L.ntab * (int)sizeof(R) + 15) & ~15;
-  L.wvt = L.reg + L.rup * L.rs * (int)sizeof(R);  // per wave: [16 records][rvp]
-  L.total = L.wvt + kDegridWaves * 16 * L.rvp * (int)sizeof(R);
+  L.tab = 0;  // the zero-padded weight table (WTab)
+  L.reg = (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
+  L.wvt = L.reg + L.rup * L.rs * (int)sizeof(R);  // per wave: [16 records][rvp + 1]
+  L.total = L.wvt + kDegridWaves * 16 * (L.rvp + 1) * (int)sizeof(R);
   return L;
 }
 
@@ -337,7 +342,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  R *wvt = (R *)(smem + L.wvt) + wave * 16 * L.rvp;
+  const int WS = L.rvp + 1;  // odd row stride: the 4 records of a step use different banks
+  R *wvt = (R *)(smem + L.wvt) + wave * 16 * WS;
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
   const int Hh = L.halo, RS = L.rs;
   const int li = lane & 15, lk = lane >> 4;
@@ -350,8 +356,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   // this wavefront's first group of records, in flight during the region load
   DegridGroup cur = load_group<NPOL>(P, first, end, wave, li, flag);
 
-  for (int i = tid; i < L.ntab; i += kDegridThreads)
-    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+  const WTab W(ov, S);
+  fill_wtab(tab, W, ov, sc, L.ntab, P.tab32, cgk);
   const int ic = (int)(b % P.n_img_chan);
   const int64_t tile = b / P.n_img_chan;
   const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
@@ -378,14 +384,14 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
     // v-weights of record li at every region column (lanes of one record
     // split the columns)
     const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
-    for (int j = lk; j < L.rvp; j += 4) wvt[li * L.rvp + j] = tap_weight(tab, ov, j - vcol0, S, sc, ovv);
+    for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
 
     MV T[NB], T1 = MV{0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
     for (int s = 0; s < ksteps; ++s) {
       const int row = 4 * s + lk;
-      const R a = tap_weight(tab, ov, row - urow0, S, sc, ou);
+      const R a = tab[W.index(row - urow0, ou)];
       const R *rp = reg + row * RS + li * NB;
       R bv[NB];
 #pragma unroll
@@ -405,13 +411,15 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
     for (int q = 0; q < 4; ++q) {
       const int rec = MO::row(lk, q);  // record slot in the group
       const uint32_t gq = (uint32_t)__shfl((int)cur.geo, rec);
-      const R *wrow = wvt + rec * L.rvp + li / C;
+      const R *wrow = wvt + rec * WS + li / C;
       R part = (R)0, npart = (R)0;
 #pragma unroll
+      // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
+      // and each step of part is then exactly g times the same step of npart
       for (int k = 0; k < NB; ++k) {
         const R wv = wrow[k * CPB];
-        part += T[k][q] * wv;
-        npart += T1[q] * wv;
+        part = fma(T[k][q], wv, part);
+        npart = fma(T1[q], wv, npart);
       }
       part = sum_over_dv<C, CPB>(part);
       npart = sum_over_dv<C, CPB>(npart);

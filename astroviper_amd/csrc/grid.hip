@@ -405,8 +405,8 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support) {
   L.ntab = ov * (sc + 1);
   L.nnrm = 2 * (ov / 2 + 2) + 1;
   int o = 0;
-  L.tab = o;
-  o += (L.ntab * (int)sizeof(R) + 15) & ~15;
+  L.tab = o;  // the zero-padded weight table (WTab)
+  o += (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
   L.nrm = o;
   o += (L.nnrm * (int)sizeof(double) + 15) & ~15;
   L.st_d = o;                           // per wave: [64][2*NPOL] R
@@ -442,8 +442,8 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
 
   const uint32_t n_units = P.gunit_start[P.n_bins];
   if ((uint32_t)blockIdx.x * kGridMaxWaves >= n_units) return;  // whole workgroup idle
-  for (int i = tid; i < L.ntab; i += blockDim.x)
-    tab[i] = sizeof(R) == 4 ? (R)P.tab32[i] : (R)cgk[i];
+  const WTab W(ov, S);
+  fill_wtab(tab, W, ov, sc, L.ntab, P.tab32, cgk);
   for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
   __syncthreads();  // the only workgroup barrier: tables ready
   const uint32_t u = blockIdx.x * kGridMaxWaves + wave;
@@ -598,10 +598,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
       unpack_geo(g, au, ou, avv, ovv);
       R a[MB], wv[VB];
 #pragma unroll
-      for (int m = 0; m < MB; ++m) a[m] = tap_weight(tab, ov, 16 * m + li - au + sc, S, sc, ou);
+      for (int m = 0; m < MB; ++m) a[m] = tab[W.index(16 * m + li - au + sc, ou)];
 #pragma unroll
-      for (int vb = 0; vb < VB; ++vb)
-        wv[vb] = tap_weight(tab, ov, 16 * vb + li - avv + sc, S, sc, ovv);
+      for (int vb = 0; vb < VB; ++vb) wv[vb] = tab[W.index(16 * vb + li - avv + sc, ovv)];
       R dd[C];
 #pragma unroll
       for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {

@@ -101,4 +101,35 @@ __device__ inline R tap_weight(const R *tab, int ov, int t, int S, int sc, int o
   return w * in;
 }
 
+// Zero-padded kernel-weight table.  Entry (t, off), for taps t in [-1, S] and
+// offsets off in [-half, half], holds tab[|ov*(t - sc) + off|] for
+// 0 <= t < S and 0 on the padding rows t = -1 and t = S, so clamping any tap
+// into [-1, S] gives its weight (or 0 outside the support) with one read.
+struct WTab {
+  int nw, half, S;
+  __host__ __device__ WTab(int ov, int support)
+      : nw(2 * (ov / 2 + 2) + 1), half(ov / 2 + 2), S(support) {}
+  __host__ __device__ int size() const { return (S + 2) * nw; }
+  __device__ int index(int t, int off) const {
+    const int tc = t < -1 ? -1 : (t > S ? S : t);
+    return (tc + 1) * nw + off + half;
+  }
+};
+
+// Fill a WTab from the half-kernel table (ntab entries of FP32 `tab32` or
+// FP64 `cgk`) with every thread of the workgroup.
+template <typename R>
+__device__ inline void fill_wtab(R *w, const WTab &W, int ov, int sc, int ntab,
+                                 const float *tab32, const double *cgk) {
+  for (int i = threadIdx.x; i < W.size(); i += blockDim.x) {
+    const int tc = i / W.nw - 1, off = i - (i / W.nw) * W.nw - W.half;
+    R v = (R)0;
+    if (tc >= 0 && tc < W.S) {
+      const int k = tab_index(ov, tc, sc, off);
+      if (k < ntab) v = sizeof(R) == 4 ? (R)tab32[k] : (R)cgk[k];
+    }
+    w[i] = v;
+  }
+}
+
 }  // namespace av
