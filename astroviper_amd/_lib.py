@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 3  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 4  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -30,6 +30,7 @@ AV_DEGRID_INCREMENTAL = 1
 
 AV_GEOM_REFERENCE = 0
 AV_GEOM_DEGRID = 1
+AV_PLAN_ROW_CHANNELS = 16
 
 _c = ctypes
 _P = _c.c_void_p

@@ -27,7 +27,7 @@ import torch
 
 from .device import GridPlan, require_gpu
 
-DEFAULT_TILES = ((16, 16), (16, 32), (32, 16), (32, 32))
+DEFAULT_TILES = ((16, 16), (16, 32), (32, 16), (32, 32), (16, 64))
 
 
 @dataclass
@@ -102,7 +102,7 @@ def autotune(ps: PlaneSet, n_uv, cell, oversampling: int = 100,
         if tile[0] < ps.support or tile[1] < ps.support:
             continue
         plan = GridPlan(ps.n_samples, ps.n_planes, ps.n_planes, n_uv, cell, ps.support,
-                        oversampling, tile=tile, device=dev)
+                        oversampling, tile=tile, device=dev, row_channels=True)
         try:
             times[tuple(tile)] = _time_pair(plan, ps, grid, sumwt, out, reps)
         except NotImplementedError:
@@ -140,7 +140,7 @@ class BatchedGridder:
             self.tiles[ps.support] = tuple(tile)
             self.plans.append(GridPlan(ps.n_samples, ps.n_planes, ps.n_planes, self.n_uv, cell,
                                        ps.support, oversampling, tile=tile,
-                                       device=self.device))
+                                       device=self.device, row_channels=True))
             self.grids.append(torch.empty((ps.n_planes, 1) + self.n_uv, dtype=ps.vis.dtype,
                                           device=self.device))
             self.sumwts.append(torch.empty((ps.n_planes, 1), dtype=torch.float64,

@@ -374,11 +374,13 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   if (tile_v == 0) tile_v = 32;
   if (tile_u < support || tile_v < support || tile_u > 128 || tile_v > 128)
     return fail(AV_ERR_INVALID, "av_plan_create: tiles must be in [support, 128]");
+  const bool row_channels = (geometry & AV_PLAN_ROW_CHANNELS) != 0;
+  geometry &= ~AV_PLAN_ROW_CHANNELS;
   if (geometry != AV_GEOM_REFERENCE && geometry != AV_GEOM_DEGRID)
     return fail(AV_ERR_INVALID, "av_plan_create: unknown geometry");
   if (unit_cap == 0) unit_cap = 8192;
   if (unit_cap < 64) return fail(AV_ERR_INVALID, "av_plan_create: unit_cap < 64");
-  int64_t n_items = n_rows * (int64_t)n_chan;
+  int64_t n_items = row_channels ? n_rows : n_rows * (int64_t)n_chan;
   if (n_items > ((int64_t)1 << 32) / 5 - 1)
     return fail(AV_ERR_UNSUPPORTED, "av_plan_create: more than 858993458 samples per plan");
 
@@ -401,6 +403,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.tile_u = tile_u;
   d.tile_v = tile_v;
   d.geometry = geometry;
+  d.row_channels = row_channels;
   d.unit_cap = unit_cap;
   d.ntu = (n_u + tile_u - 1) / tile_u;
   d.ntv = (n_v + tile_v - 1) / tile_v;
@@ -417,10 +420,10 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   const int64_t nb = d.n_bins;
   int64_t scan_blocks = (std::max<int64_t>(nb, n_items) + 4095) / 4096 + 1;
   d.mask_words_per_chan = (n_rows + 63) / 64;
-  const int64_t mask_words = d.mask_words_per_chan * n_chan + 1;
+  const int64_t mask_words = d.mask_words_per_chan * (row_channels ? 1 : n_chan) + 1;
   // chunked binning when a channel's tiles fit in LDS
   const int64_t ntiles = (int64_t)d.ntu * d.ntv;
-  d.chunked = ntiles <= kMaxChunkTiles && n_rows > 0;
+  d.chunked = !row_channels && ntiles <= kMaxChunkTiles && n_rows > 0;
   d.rows_per_chunk = (int32_t)std::min<int64_t>(
       kMaxRowsPerChunk, std::max<int64_t>(kChunkThreads,
                                           (n_rows + kChunkThreads - 1) / kChunkThreads * kChunkThreads));
@@ -495,6 +498,9 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
     return fail(AV_ERR_INVALID, "av_plan_set_uvw: NULL argument");
   if (row_chan && plan->d.n_chan < 1)
     return fail(AV_ERR_INVALID, "av_plan_set_uvw: row_chan needs n_chan >= 1");
+  if (plan->d.row_channels && !row_chan)
+    return fail(AV_ERR_INVALID,
+                "av_plan_set_uvw: the plan was created for per-row channels; row_chan is NULL");
   hipStream_t s = (hipStream_t)stream;
   PlanDev &d = plan->d;
   d.uvw = uvw;

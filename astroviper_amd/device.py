@@ -13,7 +13,7 @@ from typing import Optional, Tuple
 import torch
 
 from . import _lib
-from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_DEGRID,
+from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_DEGRID, AV_PLAN_ROW_CHANNELS,
                    AV_GEOM_REFERENCE, AV_GRID_ACCUMULATE, AV_GRID_ADJOINT,
                    AV_GRID_COMPENSATED, AV_GRID_PSF, call)
 
@@ -103,12 +103,16 @@ class GridPlan:
     ``support`` / ``oversampling`` the kernel table.  ``geometry`` is
     ``"reference"`` (gridding and degridding each use their reference
     rounding rule) or ``"degrid"`` (both use the degridder's, making
-    ``grid(adjoint=True)`` the exact adjoint of ``degrid``).
+    ``grid(adjoint=True)`` the exact adjoint of ``degrid``).  With
+    ``row_channels`` every row is one sample at its own channel (``row_chan``
+    in :meth:`set_uvw`), and the plan holds n_rows samples instead of
+    n_rows * n_chan.
     """
 
     def __init__(self, n_rows: int, n_chan: int, n_img_chan: int, n_uv, cell,
                  support: int = 7, oversampling: int = 100, tile=(16, 16),
-                 geometry: str = "reference", unit_cap: int = 0, device=None):
+                 geometry: str = "reference", unit_cap: int = 0, device=None,
+                 row_channels: bool = False):
         self.device = require_gpu(device)
         self.n_rows, self.n_chan, self.n_img_chan = int(n_rows), int(n_chan), int(n_img_chan)
         self.n_u, self.n_v = int(n_uv[0]), int(n_uv[1])
@@ -116,6 +120,8 @@ class GridPlan:
         self.support, self.oversampling = int(support), int(oversampling)
         self.tile = (int(tile[0]), int(tile[1]))
         geo = {"reference": AV_GEOM_REFERENCE, "degrid": AV_GEOM_DEGRID}[geometry]
+        if row_channels:
+            geo |= AV_PLAN_ROW_CHANNELS
         self.geometry = geometry
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -114,3 +114,66 @@ class MeasurementFunction(torch.autograd.Function):
     def backward(ctx, grad_y):
         g = ctx.op.adjoint(grad_y, real=ctx.real)
         return g.reshape(ctx.shape).to(ctx.dtype), None
+
+
+def _main(argv=None) -> int:
+    """Time the measurement operator on a sky of n x n pixels and about
+    `points` uvw samples from the 27-antenna, 4-channel generator (hour
+    angles extended until the sample count is reached)."""
+    import argparse
+    import json
+    import math
+
+    import numpy as np
+
+    from . import synth
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--points", type=int, default=10_000_000)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
+    args = ap.parse_args(argv)
+    dev = _dev.require_gpu()
+    uvw, freq, cell = synth.measurement_samples(args.points, args.n)
+    dtype = torch.complex64 if args.dtype == "fp32" else torch.complex128
+    rows = uvw.shape[0]
+    op = MeasurementOperator(torch.from_numpy(uvw).to(dev), torch.from_numpy(freq).to(dev),
+                             (args.n, args.n), cell, n_pol=1, dtype=dtype, device=dev)
+    sky = torch.from_numpy(synth.blob_sky(args.n, 50, np.random.default_rng(3))).to(dev)
+    x = sky.to(torch.float32 if args.dtype == "fp32" else torch.float64).requires_grad_(True)
+    y0 = op.forward(x.detach())
+
+    def fwd():
+        return op.forward(x.detach())
+
+    def grad():
+        x.grad = None
+        chi2 = (op.apply(x) - y0).abs().pow(2).sum()
+        chi2.backward()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.reps
+
+    tf = timed(fwd)
+    tg = timed(grad)
+    n_s = op.n_samples
+    print(json.dumps({"workload": f"{args.n}x{args.n} sky, {n_s} samples ({rows} rows x "
+                                  f"{freq.size} channels), support 7, {args.dtype}",
+                      "forward_ms": round(tf, 3), "forward_plus_backward_ms": round(tg, 3),
+                      "M_samples_per_s_forward": round(n_s / tf / 1e3, 1),
+                      "M_samples_per_s_forward_backward": round(n_s / tg / 1e3, 1)}),
+          flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())

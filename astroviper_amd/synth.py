@@ -266,3 +266,38 @@ def uneven_plane_sets(n_planes: int = 256, n_uv: int = 512, counts=(1e4, 2e6),
     info = dict(unflagged=unflagged, planes=n_planes, n_uv=n_uv, seed=seed,
                 plane_counts=n_tot.tolist())
     return sets, (-cell, cell), info
+
+
+def blob_sky(n: int, nblob: int, rng: np.random.Generator, sigma=(2.0, 10.0),
+             amp=(0.1, 1.0)) -> np.ndarray:
+    """n x n image of `nblob` Gaussian blobs, widths U(sigma) pixels,
+    amplitudes U(amp), centres uniform over the image."""
+    yy, xx = np.mgrid[0:n, 0:n]
+    img = np.zeros((n, n))
+    for _ in range(nblob):
+        cx, cy = rng.uniform(0, n, size=2)
+        s = rng.uniform(*sigma)
+        a = rng.uniform(*amp)
+        r0 = int(max(0, cy - 6 * s)), int(min(n, cy + 6 * s + 1))
+        c0 = int(max(0, cx - 6 * s)), int(min(n, cx + 6 * s + 1))
+        sub = (slice(r0[0], r0[1]), slice(c0[0], c0[1]))
+        img[sub] += a * np.exp(-((xx[sub] - cx) ** 2 + (yy[sub] - cy) ** 2) / (2 * s * s))
+    return img
+
+
+def measurement_samples(points: int, n: int, seed: int = 3):
+    """About `points` uvw samples for the measurement operator: the
+    27-antenna, 4-channel array (351 baselines) with its hour-angle range
+    sampled finely enough to reach `points` (rows x channels), truncated to
+    exactly `points` // 4 rows.  The cell puts the longest baseline at the
+    highest frequency at 0.45 of the grid's half width.  Returns
+    (uvw [rows, 3], freq [4], (cell_u, cell_v))."""
+    arr = ArrayConfig(seed=seed)
+    nb = arr.n_ant * (arr.n_ant - 1) // 2
+    rows = points // arr.n_chan
+    nt = (rows + nb - 1) // nb
+    obs = make_observation(ArrayConfig(**{**arr.__dict__, "n_time": nt}), SkyConfig())
+    uvw = np.ascontiguousarray(obs.uvw.reshape(-1, 3)[:rows])
+    maxuv = np.max(np.hypot(uvw[:, 0], uvw[:, 1]))
+    c = 0.45 * C_LIGHT / obs.freq[-1] / maxuv
+    return uvw, obs.freq, (-c, c)

@@ -44,10 +44,13 @@ extern "C" {
 #define AV_GEOM_DEGRID 1    /* both use dgrid_numba rounding: av_grid with
                                AV_GRID_ADJOINT is then the exact adjoint of
                                av_degrid */
+#define AV_PLAN_ROW_CHANNELS 16 /* or'ed into geometry: every row is one sample
+                                   at its own channel (row_chan), so the plan
+                                   holds n_rows samples, not n_rows * n_chan */
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 3
+#define AV_ABI_VERSION 4
 
 const char *av_last_error(void);
 int av_version(void);
@@ -91,11 +94,13 @@ int av_ps_outer(const double *k1, int32_t nrow, int32_t nfill,
  * and shared by av_grid and av_degrid.
  *
  * n_rows      rows (time x baseline) of uvw
- * n_chan      visibility channels per row (1 when row_chan is used)
+ * n_chan      visibility channels (entries of freq); with row_chan each row
+ *             is one sample at one of them
  * n_img_chan  grid channels; n_u, n_v grid axes 2 and 3
  * cell_u/v    pixelincr in radians (signed)
  * tile_u/v    tile shape in pixels (0 = default 32)
- * geometry    AV_GEOM_REFERENCE or AV_GEOM_DEGRID
+ * geometry    AV_GEOM_REFERENCE or AV_GEOM_DEGRID, optionally or'ed with
+ *             AV_PLAN_ROW_CHANNELS (then av_plan_set_uvw needs row_chan)
  * unit_cap    records per work unit (0 = default 8192)
  * The plan is device-resident; create it once per (shape, tiling). */
 typedef struct av_plan av_plan;

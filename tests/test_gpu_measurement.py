@@ -108,3 +108,67 @@ def test_chi2_gradient_against_fp64_oracle(gpu, oracle):
     err_in = np.max(np.abs(g - gref)[inner]) / den
     print(f"chi2 gradient FP32 vs FP64 oracle: full field {err:.3e}, taper >= 1e-2 {err_in:.3e}")
     assert err_in < 1e-4 and err < 1e-2
+
+
+def test_full_size_operator(gpu, oracle):
+    """The 1024 x 1024 sky (50 Gaussian blobs, sigma 2-10 px) and 1e7 uvw
+    samples of the 27-antenna generator: adjoint dot products (FP32 and
+    FP64), central finite differences on 32 pixels (FP64, step 1e-3) and the
+    FP32 chi-squared gradient against the FP64 oracle."""
+    from astroviper_amd import synth
+    from astroviper_amd.measurement import MeasurementOperator
+    n = 1024
+    uvw, freq, cell = synth.measurement_samples(10_000_000, n)
+    rng = np.random.default_rng(3)
+    sky = synth.blob_sky(n, 50, rng)
+    ops = {dt: MeasurementOperator(torch.from_numpy(uvw).to(gpu), torch.from_numpy(freq).to(gpu),
+                                   (n, n), cell, n_pol=1, dtype=dt, device=gpu)
+           for dt in (torch.complex64, torch.complex128)}
+    assert ops[torch.complex64].n_samples == 10_000_000
+    # adjoint dot products
+    g = torch.Generator(device=gpu)
+    g.manual_seed(31)
+    for dt, tol in ((torch.complex64, 1e-6), (torch.complex128, 1e-12)):
+        op = ops[dt]
+        x = torch.randn(op.shape, dtype=dt, device=gpu, generator=g)
+        y = torch.randn((op.n_samples, 1), dtype=dt, device=gpu, generator=g)
+        ax, ahy = op.forward(x), op.adjoint(y)
+        c128 = torch.complex128
+        lhs = torch.vdot(y.flatten().to(c128), ax.flatten().to(c128))
+        rhs = torch.vdot(ahy.flatten().to(c128), x.flatten().to(c128))
+        mis = ((lhs - rhs).abs() / (torch.linalg.norm(ax.to(c128)) *
+                                    torch.linalg.norm(y.to(c128)))).item()
+        print(f"full-size adjoint mismatch {dt}: {mis:.3e}")
+        assert mis < tol
+    # finite differences, FP64
+    op = ops[torch.complex128]
+    xs = torch.from_numpy(sky).to(gpu)
+    data = op.forward(xs * 1.1 + 0.01)
+    xg = xs.clone().requires_grad_(True)
+    (op.apply(xg) - data).abs().pow(2).sum().backward()
+    grad = xg.grad.cpu().numpy()
+    h = 1e-3
+    scale = max(1.0, np.abs(grad).max())
+    for (i, j) in rng.integers(64, n - 64, size=(32, 2)):
+        e = torch.zeros_like(xs)
+        e[i, j] = h
+        fp = (op.forward(xs + e) - data).abs().pow(2).sum().item()
+        fm = (op.forward(xs - e) - data).abs().pow(2).sum().item()
+        assert abs((fp - fm) / (2 * h) - grad[i, j]) <= 1e-6 * scale, (i, j)
+    # chi-squared gradient, FP32 operator against the FP64 oracle
+    truth = synth.blob_sky(n, 50, rng)
+    uvw3 = uvw.reshape(1, -1, 3)
+    d = oracle.measure(truth[None, None], uvw3, freq, cell)
+    op32 = ops[torch.complex64]
+    x32 = torch.from_numpy(sky).to(gpu, torch.float32).requires_grad_(True)
+    dt = torch.from_numpy(d.reshape(-1, 1)).to(gpu, torch.complex64)
+    (op32.apply(x32) - dt).abs().pow(2).sum().backward()
+    gg = x32.grad.double().cpu().numpy()
+    r = oracle.measure(sky[None, None], uvw3, freq, cell) - d
+    gref = 2 * np.real(oracle.measure_adjoint(r, uvw3, freq, (n, n), cell))[0, 0]
+    _, corr = oracle.create_prolate_spheroidal_kernel(100, 7, np.array([n, n]))
+    inner = corr >= 1e-2
+    den = np.max(np.abs(gref))
+    err, err_in = np.max(np.abs(gg - gref)) / den, np.max(np.abs(gg - gref)[inner]) / den
+    print(f"full-size chi2 gradient FP32 vs FP64 oracle: full {err:.3e}, taper >= 1e-2 {err_in:.3e}")
+    assert err_in < 1e-4 and err < 1e-2
