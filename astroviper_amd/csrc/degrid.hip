@@ -267,8 +267,13 @@ struct DegridMfma<double> {
 };
 
 struct DegridMfmaLds {
-  int tab, reg, wvt, total, ntab, halo, ru, rup, rv, rvp, rs;
+  int tab, reg, wvt, sgeo, sitem, scnt, total, ntab, halo, ru, rup, rv, rvp, rs;
 };
+
+// The records of a degrid unit are sorted by tile row in LDS, this many at a
+// time, so that each 16-record group covers a narrow band of region rows.
+constexpr int kSortCap = 1024;
+constexpr int kSortKeys = 136;  // rows -1 .. tile_u (<= 128) and one key for skipped records
 
 // Region rows padded to a multiple of 4 (one matrix step), columns to NB
 // blocks of 16 components.  Within a row, component n of the region is
@@ -290,7 +295,10 @@ __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support,
   L.tab = 0;  // the zero-padded weight table (WTab)
   L.reg = (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
   L.wvt = L.reg + L.rup * L.rs * (int)sizeof(R);  // per wave: [16 records][rvp + 1]
-  L.total = L.wvt + kDegridWaves * 16 * (L.rvp + 1) * (int)sizeof(R);
+  L.sgeo = (L.wvt + kDegridWaves * 16 * (L.rvp + 1) * (int)sizeof(R) + 15) & ~15;
+  L.sitem = L.sgeo + kSortCap * 4;
+  L.scnt = L.sitem + kSortCap * 4;
+  L.total = L.scnt + kSortKeys * 4;
   return L;
 }
 
@@ -299,29 +307,10 @@ __host__ __device__ constexpr int degrid_mfma_blocks_needed(int tv, int support)
   return ((tv + 2 * (support / 2 + 1)) * 2 * NPOL + 15) / 16;
 }
 
-// The 16 records of one group: lane l loads record l % 16.
+// The 16 records of one group (lane l holds record l % 16).
 struct DegridGroup {
   uint32_t geo, item, fbits;
 };
-
-template <int NPOL>
-__device__ inline DegridGroup load_group(const PlanDev &P, uint32_t first, uint32_t end,
-                                         uint32_t g, int li, const uint8_t *flag) {
-  DegridGroup G{kInvalidGeo, 0u, 0u};
-  const uint32_t r = first + g * 16 + li;
-  if (r < end) {
-    G.geo = P.rec_d[r];
-    G.item = P.rec[r].x;
-    if (G.geo != kInvalidGeo && flag) {
-      if (NPOL == 4)
-        G.fbits = *(const uint32_t *)(flag + (size_t)G.item * 4);
-      else
-        for (int p = 0; p < NPOL; ++p)
-          G.fbits |= (uint32_t)flag[(size_t)G.item * NPOL + p] << (8 * p);
-    }
-  }
-  return G;
-}
 
 template <typename R, int NPOL, int NB>
 __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
@@ -348,13 +337,34 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   const int Hh = L.halo, RS = L.rs;
   const int li = lane & 15, lk = lane >> 4;
 
+  uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
+  uint32_t *s_item = (uint32_t *)(smem + L.sitem);
+  uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
+
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, cnt = unit.z;
   const uint32_t end = first + cnt;
-  const uint32_t ngroups = (cnt + 15) / 16;
-  // this wavefront's first group of records, in flight during the region load
-  DegridGroup cur = load_group<NPOL>(P, first, end, wave, li, flag);
+  constexpr int HK = kSortCap / kDegridThreads;  // records per thread in a sort pass
+  const int nkeys = Tu + 3;
+  // sort key: tile row + 1 of the degrid centre (rows -1 .. Tu), Tu + 2 for
+  // a record without degrid geometry
+  auto key_of = [&](uint32_t g) {
+    const int k = (int)(g & 0xFFu) - kPosBias + 1;
+    return g == kInvalidGeo ? Tu + 2 : (k < 0 ? 0 : (k > Tu + 1 ? Tu + 1 : k));
+  };
+  // records of the first sort pass, in flight during the region load
+  uint32_t hg[HK], hi[HK];
+#pragma unroll
+  for (int k = 0; k < HK; ++k) {
+    const uint32_t r = first + tid + k * kDegridThreads;
+    hg[k] = kInvalidGeo;
+    hi[k] = 0;
+    if (r < end && tid + k * kDegridThreads < kSortCap) {
+      hg[k] = P.rec_d[r];
+      hi[k] = P.rec[r].x;
+    }
+  }
 
   const WTab W(ov, S);
   fill_wtab(tab, W, ov, sc, L.ntab, P.tab32, cgk);
@@ -373,71 +383,146 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
                          dst[0] = val.re;
                          dst[NB] = val.im;
                        });
-  __syncthreads();
 
-  const int ksteps = L.rup / 4;
-  for (uint32_t g = wave; g < ngroups; g += kDegridWaves) {
-    const DegridGroup nxt = load_group<NPOL>(P, first, end, g + kDegridWaves, li, flag);
-    int au, ou, avv, ovv;
-    unpack_geo(geo_or_far(cur.geo), au, ou, avv, ovv);
-    const int urow0 = au + Hh - sc;  // region row of the first u-tap
-    // v-weights of record li at every region column (lanes of one record
-    // split the columns)
-    const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
-    for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
-
-    MV T[NB], T1 = MV{0, 0, 0, 0};
+  for (uint32_t c0 = first; c0 < end; c0 += kSortCap) {
+    const int n = (int)min((uint32_t)kSortCap, end - c0);
+    if (c0 != first) {  // later passes of a large unit
 #pragma unroll
-    for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
-    for (int s = 0; s < ksteps; ++s) {
-      const int row = 4 * s + lk;
-      const R a = tab[W.index(row - urow0, ou)];
-      const R *rp = reg + row * RS + li * NB;
-      R bv[NB];
-#pragma unroll
-      for (int k = 0; k < NB; k += VL) {
-        const Vec<R, VL> t = *(const Vec<R, VL> *)(rp + k);
-#pragma unroll
-        for (int e = 0; e < VL; ++e) bv[k + e] = t.v[e];
+      for (int k = 0; k < HK; ++k) {
+        const int r = tid + k * kDegridThreads;
+        hg[k] = r < n ? P.rec_d[c0 + r] : kInvalidGeo;
+        hi[k] = r < n ? P.rec[c0 + r].x : 0u;
       }
-#pragma unroll
-      for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], T[k]);
-      T1 = MO::mma(a, (R)1, T1);
     }
-    wave_lds_sync();
-    // ---- weighted sums over v: lane holds records MO::row(l/16, q), cell
-    // k*CPB + li/C, component li%C of every block k
+    // ---- counting sort of the pass's records by key, in LDS
+    for (int i = tid; i < kSortKeys; i += kDegridThreads) s_cnt[i] = 0u;
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int rec = MO::row(lk, q);  // record slot in the group
-      const uint32_t gq = (uint32_t)__shfl((int)cur.geo, rec);
-      const R *wrow = wvt + rec * WS + li / C;
-      R part = (R)0, npart = (R)0;
+    for (int k = 0; k < HK; ++k)
+      if (tid + k * kDegridThreads < n) atomicAdd(&s_cnt[key_of(hg[k])], 1u);
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the counts: three keys per lane
+      uint32_t c3[3], t = 0;
 #pragma unroll
-      // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
-      // and each step of part is then exactly g times the same step of npart
-      for (int k = 0; k < NB; ++k) {
-        const R wv = wrow[k * CPB];
-        part = fma(T[k][q], wv, part);
-        npart = fma(T1[q], wv, npart);
+      for (int j = 0; j < 3; ++j) {
+        const int kk = 3 * lane + j;
+        c3[j] = kk < nkeys ? s_cnt[kk] : 0u;
+        t += c3[j];
       }
-      part = sum_over_dv<C, CPB>(part);
-      npart = sum_over_dv<C, CPB>(npart);
-      const uint32_t itq = (uint32_t)__shfl((int)cur.item, rec);
-      const uint32_t fq = (uint32_t)__shfl((int)cur.fbits, rec);
-      if (li >= 16 - C && gq != kInvalidGeo) {
-        const int c = li - (16 - C);
-        const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
-        R *dst = vis_out + (size_t)itq * C + c;
-        if (incremental) {
-          if (!flagged) *dst += part / npart;
-        } else {
-          *dst = flagged ? (R)0 : part / npart;
+      uint32_t incl = t;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+      }
+      uint32_t run = incl - t;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int kk = 3 * lane + j;
+        if (kk < nkeys) s_cnt[kk] = run;
+        run += c3[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < HK; ++k)
+      if (tid + k * kDegridThreads < n) {
+        const uint32_t pos = atomicAdd(&s_cnt[key_of(hg[k])], 1u);
+        s_geo[pos] = hg[k];
+        s_item[pos] = hi[k];
+      }
+    __syncthreads();
+
+    // ---- 16-record groups of the sorted pass
+    const int ngroups = (n + 15) / 16;
+    auto group_of = [&](int g) {
+      DegridGroup G{kInvalidGeo, 0u, 0u};
+      const int r = g * 16 + li;
+      if (g < ngroups && r < n) {
+        G.geo = s_geo[r];
+        G.item = s_item[r];
+        if (G.geo != kInvalidGeo && flag) {
+          if (NPOL == 4)
+            G.fbits = *(const uint32_t *)(flag + (size_t)G.item * 4);
+          else
+            for (int p = 0; p < NPOL; ++p)
+              G.fbits |= (uint32_t)flag[(size_t)G.item * NPOL + p] << (8 * p);
         }
       }
+      return G;
+    };
+    DegridGroup cur = group_of(wave);
+    for (int g = wave; g < ngroups; g += kDegridWaves) {
+      const DegridGroup nxt = group_of(g + kDegridWaves);
+      int au, ou, avv, ovv;
+      unpack_geo(geo_or_far(cur.geo), au, ou, avv, ovv);
+      const int urow0 = au + Hh - sc;  // region row of the first u-tap
+      // the group's band of region rows (records are sorted by row)
+      int rlo = cur.geo == kInvalidGeo ? 1 << 20 : urow0;
+      int rhi = cur.geo == kInvalidGeo ? -1 : urow0 + S - 1;
+      for (int off = 1; off < 16; off <<= 1) {
+        rlo = min(rlo, __shfl_xor(rlo, off));
+        rhi = max(rhi, __shfl_xor(rhi, off));
+      }
+      // v-weights of record li at every region column (lanes of one record
+      // split the columns)
+      const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
+      for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
+
+      MV T[NB], T1 = MV{0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
+      // row steps outside the band would only add zeros
+      for (int s4 = rlo >> 2; s4 <= (rhi >> 2); ++s4) {
+        const int row = 4 * s4 + lk;
+        const R a = tab[W.index(row - urow0, ou)];
+        const R *rp = reg + row * RS + li * NB;
+        R bv[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k += VL) {
+          const Vec<R, VL> t = *(const Vec<R, VL> *)(rp + k);
+#pragma unroll
+          for (int e = 0; e < VL; ++e) bv[k + e] = t.v[e];
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], T[k]);
+        T1 = MO::mma(a, (R)1, T1);
+      }
+      wave_lds_sync();
+      // ---- weighted sums over v: lane holds records MO::row(l/16, q), cell
+      // k*CPB + li/C, component li%C of every block k
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rec = MO::row(lk, q);  // record slot in the group
+        const uint32_t gq = (uint32_t)__shfl((int)cur.geo, rec);
+        const R *wrow = wvt + rec * WS + li / C;
+        R part = (R)0, npart = (R)0;
+        // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
+        // and each step of part is then exactly g times the same step of npart
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+          const R wv = wrow[k * CPB];
+          part = fma(T[k][q], wv, part);
+          npart = fma(T1[q], wv, npart);
+        }
+        part = sum_over_dv<C, CPB>(part);
+        npart = sum_over_dv<C, CPB>(npart);
+        const uint32_t itq = (uint32_t)__shfl((int)cur.item, rec);
+        const uint32_t fq = (uint32_t)__shfl((int)cur.fbits, rec);
+        if (li >= 16 - C && gq != kInvalidGeo) {
+          const int c = li - (16 - C);
+          const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
+          R *dst = vis_out + (size_t)itq * C + c;
+          if (incremental) {
+            if (!flagged) *dst += part / npart;
+          } else {
+            *dst = flagged ? (R)0 : part / npart;
+          }
+        }
+      }
+      wave_lds_sync();  // the next group rewrites this wavefront's weights
+      cur = nxt;
     }
-    wave_lds_sync();  // the next group rewrites this wavefront's weights
-    cur = nxt;
+    __syncthreads();  // the next pass rewrites the sorted records
   }
 }
 
