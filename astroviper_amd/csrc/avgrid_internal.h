@@ -165,6 +165,8 @@ struct PlanDev {
   // per-call tables: FP32 kernel table, FP64 per-axis kernel sums by offset
   float *tab32;                     // [ov * (support/2 + 1)]
   double *nrm;                      // [2 * (ov/2 + 2) + 1]
+  float *wtab32;                    // zero-padded weight table [(support + 2) * nrm size]
+  double *wtab64;                   // the same in FP64
   // per grid unit, per polarisation: sum of weight * norm (FP64)
   double *upart;                    // [max_gunits * 4]
   uint32_t *counters;               // scratch for statistics
@@ -185,7 +187,7 @@ int prepare_tables(const PlanDev &d, const double *cgk, hipStream_t s);
 struct DeviceProps {
   hipError_t error;
   int num_cus;
-  size_t lds_per_block;
+  size_t lds_per_block, lds_per_cu;
 };
 const DeviceProps &device_props();
 

@@ -39,6 +39,7 @@ const DeviceProps &device_props() {
     p.error = hipGetDeviceProperties(&prop, dev);
     p.num_cus = prop.multiProcessorCount;
     p.lds_per_block = prop.sharedMemPerBlock;
+    p.lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
     cache[dev] = p;
     have[dev] = p.error == hipSuccess;
     if (!have[dev]) return cache[dev];

@@ -312,6 +312,13 @@ struct DegridGroup {
   uint32_t geo, item, fbits;
 };
 
+// One workgroup per degrid unit.  The unit's records (first kSortCap; larger
+// units run further passes) are read before the tile region, so both loads
+// are in flight together; the records are then sorted by tile row in LDS,
+// so that the 16 records of a group span a narrow band of region rows and
+// only the row steps of that band are run (the others would add exact
+// zeros).  Without AV_DEGRID_INCREMENTAL no flags are read here: the
+// finishing pass zeroes flagged samples afterwards.
 template <typename R, int NPOL, int NB>
 __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
     PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
@@ -322,6 +329,7 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   constexpr int C = 2 * NPOL;
   constexpr int CPB = 16 / C;               // cells per 16-column block
   constexpr int VL = 16 / (int)sizeof(R);   // elements per 16-byte read
+  constexpr int HK = kSortCap / kDegridThreads;  // records per thread in a sort pass
   extern __shared__ __align__(16) char smem[];
   const uint32_t u = blockIdx.x;
   if (u >= P.dunit_start[P.n_bins]) return;
@@ -333,41 +341,34 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int WS = L.rvp + 1;  // odd row stride: the 4 records of a step use different banks
   R *wvt = (R *)(smem + L.wvt) + wave * 16 * WS;
-  const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
-  const int Hh = L.halo, RS = L.rs;
-  const int li = lane & 15, lk = lane >> 4;
-
   uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
   uint32_t *s_item = (uint32_t *)(smem + L.sitem);
   uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
+  const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const int Hh = L.halo, RS = L.rs;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nkeys = Tu + 3;
 
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
-  const uint32_t first = unit.y, cnt = unit.z;
-  const uint32_t end = first + cnt;
-  constexpr int HK = kSortCap / kDegridThreads;  // records per thread in a sort pass
-  const int nkeys = Tu + 3;
+  const uint32_t first = unit.y, end = unit.y + unit.z;
   // sort key: tile row + 1 of the degrid centre (rows -1 .. Tu), Tu + 2 for
   // a record without degrid geometry
   auto key_of = [&](uint32_t g) {
     const int k = (int)(g & 0xFFu) - kPosBias + 1;
     return g == kInvalidGeo ? Tu + 2 : (k < 0 ? 0 : (k > Tu + 1 ? Tu + 1 : k));
   };
-  // records of the first sort pass, in flight during the region load
-  uint32_t hg[HK], hi[HK];
+  // records of the first sort pass: in flight during the region load
+  uint32_t pg[HK], pi[HK];
 #pragma unroll
   for (int k = 0; k < HK; ++k) {
     const uint32_t r = first + tid + k * kDegridThreads;
-    hg[k] = kInvalidGeo;
-    hi[k] = 0;
-    if (r < end && tid + k * kDegridThreads < kSortCap) {
-      hg[k] = P.rec_d[r];
-      hi[k] = P.rec[r].x;
-    }
+    pg[k] = r < end ? P.rec_d[r] : kInvalidGeo;
+    pi[k] = r < end ? P.rec[r].x : 0u;
   }
 
   const WTab W(ov, S);
-  fill_wtab(tab, W, ov, sc, L.ntab, P.tab32, cgk);
+  copy_wtab(tab, W, P);
   const int ic = (int)(b % P.n_img_chan);
   const int64_t tile = b / P.n_img_chan;
   const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
@@ -390,8 +391,8 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
 #pragma unroll
       for (int k = 0; k < HK; ++k) {
         const int r = tid + k * kDegridThreads;
-        hg[k] = r < n ? P.rec_d[c0 + r] : kInvalidGeo;
-        hi[k] = r < n ? P.rec[c0 + r].x : 0u;
+        pg[k] = r < n ? P.rec_d[c0 + r] : kInvalidGeo;
+        pi[k] = r < n ? P.rec[c0 + r].x : 0u;
       }
     }
     // ---- counting sort of the pass's records by key, in LDS
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < HK; ++k)
-      if (tid + k * kDegridThreads < n) atomicAdd(&s_cnt[key_of(hg[k])], 1u);
+      if (tid + k * kDegridThreads < n) atomicAdd(&s_cnt[key_of(pg[k])], 1u);
     __syncthreads();
     if (wave == 0) {  // exclusive scan of the counts: three keys per lane
       uint32_t c3[3], t = 0;
@@ -426,46 +427,39 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
 #pragma unroll
     for (int k = 0; k < HK; ++k)
       if (tid + k * kDegridThreads < n) {
-        const uint32_t pos = atomicAdd(&s_cnt[key_of(hg[k])], 1u);
-        s_geo[pos] = hg[k];
-        s_item[pos] = hi[k];
+        const uint32_t pos = atomicAdd(&s_cnt[key_of(pg[k])], 1u);
+        s_geo[pos] = pg[k];
+        s_item[pos] = pi[k];
       }
     __syncthreads();
 
     // ---- 16-record groups of the sorted pass
     const int ngroups = (n + 15) / 16;
-    auto group_of = [&](int g) {
-      DegridGroup G{kInvalidGeo, 0u, 0u};
-      const int r = g * 16 + li;
-      if (g < ngroups && r < n) {
-        G.geo = s_geo[r];
-        G.item = s_item[r];
-        if (G.geo != kInvalidGeo && flag) {
-          if (NPOL == 4)
-            G.fbits = *(const uint32_t *)(flag + (size_t)G.item * 4);
-          else
-            for (int p = 0; p < NPOL; ++p)
-              G.fbits |= (uint32_t)flag[(size_t)G.item * NPOL + p] << (8 * p);
-        }
-      }
-      return G;
-    };
-    DegridGroup cur = group_of(wave);
     for (int g = wave; g < ngroups; g += kDegridWaves) {
-      const DegridGroup nxt = group_of(g + kDegridWaves);
+      const int r = g * 16 + li;
+      const uint32_t geo = r < n ? s_geo[r] : kInvalidGeo;
+      const uint32_t item = r < n ? s_item[r] : 0u;
+      uint32_t fbits = 0;
+      if (incremental && flag && geo != kInvalidGeo) {
+        if (NPOL == 4)
+          fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+        else
+          for (int p = 0; p < NPOL; ++p)
+            fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+      }
       int au, ou, avv, ovv;
-      unpack_geo(geo_or_far(cur.geo), au, ou, avv, ovv);
-      const int urow0 = au + Hh - sc;  // region row of the first u-tap
+      unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
+      const int urow0 = au + Hh - sc;   // region row of the first u-tap
+      const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
       // the group's band of region rows (records are sorted by row)
-      int rlo = cur.geo == kInvalidGeo ? 1 << 20 : urow0;
-      int rhi = cur.geo == kInvalidGeo ? -1 : urow0 + S - 1;
+      const bool valid = geo != kInvalidGeo;
+      int rlo = valid ? urow0 : 1 << 20, rhi = valid ? urow0 + S - 1 : -1;
       for (int off = 1; off < 16; off <<= 1) {
         rlo = min(rlo, __shfl_xor(rlo, off));
         rhi = max(rhi, __shfl_xor(rhi, off));
       }
       // v-weights of record li at every region column (lanes of one record
       // split the columns)
-      const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
       for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
 
       MV T[NB], T1 = MV{0, 0, 0, 0};
@@ -488,12 +482,12 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
         T1 = MO::mma(a, (R)1, T1);
       }
       wave_lds_sync();
-      // ---- weighted sums over v: lane holds records MO::row(l/16, q), cell
-      // k*CPB + li/C, component li%C of every block k
+      // ---- weighted sums over v: lane holds records MO::row(l/16, q),
+      // cell k*CPB + li/C, component li%C of every block k
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int rec = MO::row(lk, q);  // record slot in the group
-        const uint32_t gq = (uint32_t)__shfl((int)cur.geo, rec);
+        const uint32_t gq = (uint32_t)__shfl((int)geo, rec);
         const R *wrow = wvt + rec * WS + li / C;
         R part = (R)0, npart = (R)0;
         // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
@@ -506,23 +500,56 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
         }
         part = sum_over_dv<C, CPB>(part);
         npart = sum_over_dv<C, CPB>(npart);
-        const uint32_t itq = (uint32_t)__shfl((int)cur.item, rec);
-        const uint32_t fq = (uint32_t)__shfl((int)cur.fbits, rec);
+        const uint32_t itq = (uint32_t)__shfl((int)item, rec);
+        const uint32_t fq = (uint32_t)__shfl((int)fbits, rec);
         if (li >= 16 - C && gq != kInvalidGeo) {
           const int c = li - (16 - C);
-          const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
           R *dst = vis_out + (size_t)itq * C + c;
           if (incremental) {
-            if (!flagged) *dst += part / npart;
+            if (!((fq >> (8 * (c >> 1))) & 0xFFu)) *dst += part / npart;
           } else {
-            *dst = flagged ? (R)0 : part / npart;
+            *dst = part / npart;
           }
         }
       }
       wave_lds_sync();  // the next group rewrites this wavefront's weights
-      cur = nxt;
     }
     __syncthreads();  // the next pass rewrites the sorted records
+  }
+}
+
+// Finishing pass: zero the samples no degrid unit visits (off-grid, NaN uvw,
+// channel outside the grid) and, unless incremental, the flagged ones.  A
+// workgroup covers 64 rows x 64 channels: one validity word per channel and
+// contiguous flag reads along channels.
+template <typename R>
+__global__ __launch_bounds__(256) void k_degrid_finish(PlanDev P, const uint8_t *flag,
+                                                      R *vis_out, int npol, int incremental) {
+  const int nch = P.row_chan ? 1 : P.n_chan;
+  const int64_t ncb = (nch + 63) / 64;
+  const int64_t blk = blockIdx.x;
+  const int64_t rw = blk / ncb;                 // 64-row block
+  const int c = (int)((blk - rw * ncb) * 64) + (threadIdx.x & 63);
+  if (c >= nch) return;
+  const unsigned long long m = P.dmask[(int64_t)c * P.mask_words_per_chan + rw];
+  for (int k = threadIdx.x >> 6; k < 64; k += 4) {
+    const int64_t row = rw * 64 + k;
+    if (row >= P.n_rows) break;
+    const int64_t item = P.row_chan ? row : row * P.n_chan + c;
+    bool zero = !((m >> k) & 1ull);
+    if (!zero && !incremental && flag) {
+      // a sample's polarisations are flagged independently
+      R *o = vis_out + (size_t)item * npol * 2;
+      for (int p = 0; p < npol; ++p)
+        if (flag[(size_t)item * npol + p]) {
+          o[2 * p] = (R)0;
+          o[2 * p + 1] = (R)0;
+        }
+    }
+    if (zero && !incremental) {
+      R *o = vis_out + (size_t)item * npol * 2;
+      for (int q = 0; q < 2 * npol; ++q) o[q] = (R)0;
+    }
   }
 }
 
@@ -582,27 +609,29 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
                               hipStream_t s) {
   PlanDev &d = plan->d;
   const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(d.ov, d.support, d.tile_u, d.tile_v);
-  auto kfn = k_degrid_mfma<R, NPOL, NB>;
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
   if ((size_t)L.total > prop.lds_per_block)
     return fail(AV_ERR_UNSUPPORTED, "av_degrid: tile too large for LDS (" +
                                         std::to_string(L.total) + " bytes)");
   {
-    const int rc_lds = ensure_lds_limit<k_degrid_mfma<R, NPOL, NB>>(L.total, "av_degrid: hipFuncSetAttribute");
+    const int rc_lds =
+        ensure_lds_limit<k_degrid_mfma<R, NPOL, NB>>(L.total, "av_degrid: hipFuncSetAttribute");
     if (rc_lds) return rc_lds;
   }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  if (!(mode & AV_DEGRID_INCREMENTAL) && d.n_items > 0) {
-    int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_zero_invalid<R>, dim3((unsigned)blocks), dim3(256), 0, s, d,
-                       (R *)vis_out, NPOL);
-    AV_LAUNCH_CHECK("k_zero_invalid");
-  }
-  hipLaunchKernelGGL(kfn, dim3((unsigned)d.max_dunits), dim3(kDegridThreads), L.total, s, d,
-                     (const R *)grid, npol_g, pol_map, flag, cgk, (R *)vis_out, mode);
+  hipLaunchKernelGGL((k_degrid_mfma<R, NPOL, NB>), dim3((unsigned)d.max_dunits),
+                     dim3(kDegridThreads), L.total, s, d, (const R *)grid, npol_g, pol_map, flag,
+                     cgk, (R *)vis_out, mode);
   AV_LAUNCH_CHECK("k_degrid_mfma");
+  if (d.n_rows > 0 && !(mode & AV_DEGRID_INCREMENTAL)) {
+    const int64_t nch = d.row_chan ? 1 : d.n_chan;
+    const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
+    hipLaunchKernelGGL(k_degrid_finish<R>, dim3((unsigned)blocks), dim3(256), 0, s, d, flag,
+                       (R *)vis_out, NPOL, 0);
+    AV_LAUNCH_CHECK("k_degrid_finish");
+  }
   return AV_OK;
 }
 

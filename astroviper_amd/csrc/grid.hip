@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *
   const uint32_t n_units = P.gunit_start[P.n_bins];
   if ((uint32_t)blockIdx.x * kGridMaxWaves >= n_units) return;  // whole workgroup idle
   const WTab W(ov, S);
-  fill_wtab(tab, W, ov, sc, L.ntab, P.tab32, cgk);
+  copy_wtab(tab, W, P);
   for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
   __syncthreads();  // the only workgroup barrier: tables ready
   const uint32_t u = blockIdx.x * kGridMaxWaves + wave;

@@ -314,23 +314,38 @@ __global__ void k_units(PlanDev P) {
   }
 }
 
-// per-call tables: the FP32 kernel table and, for every kernel offset, the
-// FP64 sum of the support taps of one axis (norm = nrm[ou] * nrm[ov])
+// per-call tables: the FP32 kernel table; for every kernel offset, the FP64
+// sum of the support taps of one axis (norm = nrm[ou] * nrm[ov]); and the
+// zero-padded weight table by (tap, offset) in FP32 and FP64.  Offsets past
+// ov/2 (never produced by a sample) read 0 beyond the end of the table.
 __global__ void k_prepare_tables(PlanDev P, const double *cgk) {
   const int sc = P.support / 2, ov = P.ov;
   const int ntab = ov * (sc + 1), half = ov / 2 + 2, nnrm = 2 * half + 1;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ntab + nnrm;
+  const int nw = (P.support + 2) * nnrm;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ntab + nnrm + nw;
        i += gridDim.x * blockDim.x) {
     if (i < ntab) {
       P.tab32[i] = (float)cgk[i];
-    } else {
+    } else if (i < ntab + nnrm) {
       const int off = i - ntab - half;
       double s = 0.0;
       for (int t = 0; t < P.support; ++t) {
         int k = ov * (t - sc) + off;
-        s += cgk[k < 0 ? -k : k];
+        k = k < 0 ? -k : k;
+        s += k < ntab ? cgk[k] : 0.0;
       }
       P.nrm[i - ntab] = s;
+    } else {
+      const int j = i - ntab - nnrm;
+      const int tc = j / nnrm - 1, off = j - (j / nnrm) * nnrm - half;
+      double v = 0.0;
+      if (tc >= 0 && tc < P.support) {
+        int k = ov * (tc - sc) + off;
+        k = k < 0 ? -k : k;
+        v = k < ntab ? cgk[k] : 0.0;
+      }
+      P.wtab32[j] = (float)v;
+      P.wtab64[j] = v;
     }
   }
 }
@@ -446,6 +461,8 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_chh = take(4 * chunk_cells),
          o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
          o_nrm = take(sizeof(double) * (2 * (oversampling / 2 + 2) + 1)),
+         o_wt32 = take(sizeof(float) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
+         o_wt64 = take(sizeof(double) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
          o_upart = take(sizeof(double) * 4 * d.max_gunits), o_ctr = take(4 * 16);
   void *base = nullptr;
   hipError_t e = hipMalloc(&base, off);
@@ -479,6 +496,8 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.chunk_hist = chunk_cells ? (uint32_t *)(c + o_chh) : nullptr;
   d.tab32 = (float *)(c + o_tab);
   d.nrm = (double *)(c + o_nrm);
+  d.wtab32 = (float *)(c + o_wt32);
+  d.wtab64 = (double *)(c + o_wt64);
   d.upart = (double *)(c + o_upart);
   d.counters = (uint32_t *)(c + o_ctr);
   *out = p;

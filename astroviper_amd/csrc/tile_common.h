@@ -116,20 +116,12 @@ struct WTab {
   }
 };
 
-// Fill a WTab from the half-kernel table (ntab entries of FP32 `tab32` or
-// FP64 `cgk`) with every thread of the workgroup.
+// Copy the call's zero-padded weight table (built once per call by
+// prepare_tables) into LDS with every thread of the workgroup.
 template <typename R>
-__device__ inline void fill_wtab(R *w, const WTab &W, int ov, int sc, int ntab,
-                                 const float *tab32, const double *cgk) {
-  for (int i = threadIdx.x; i < W.size(); i += blockDim.x) {
-    const int tc = i / W.nw - 1, off = i - (i / W.nw) * W.nw - W.half;
-    R v = (R)0;
-    if (tc >= 0 && tc < W.S) {
-      const int k = tab_index(ov, tc, sc, off);
-      if (k < ntab) v = sizeof(R) == 4 ? (R)tab32[k] : (R)cgk[k];
-    }
-    w[i] = v;
-  }
+__device__ inline void copy_wtab(R *w, const WTab &W, const PlanDev &P) {
+  for (int i = threadIdx.x; i < W.size(); i += blockDim.x)
+    w[i] = sizeof(R) == 4 ? (R)P.wtab32[i] : (R)P.wtab64[i];
 }
 
 }  // namespace av
