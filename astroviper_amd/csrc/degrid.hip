@@ -319,8 +319,17 @@ struct DegridGroup {
 // only the row steps of that band are run (the others would add exact
 // zeros).  Without AV_DEGRID_INCREMENTAL no flags are read here: the
 // finishing pass zeroes flagged samples afterwards.
+// Workgroups per CU the FP32 kernels with up to 12 column blocks are compiled
+// for (at most 128 registers without spills, and four regions fit in LDS);
+// other variants keep the compiler's choice.
+template <typename R, int NB>
+constexpr int degrid_mfma_min_blocks() {
+  return (sizeof(R) == 4 && NB <= 12) ? 4 : 1;
+}
+
 template <typename R, int NPOL, int NB>
-__global__ __launch_bounds__(kDegridThreads) void k_degrid_mfma(
+__global__ __launch_bounds__(kDegridThreads, (degrid_mfma_min_blocks<R, NB>()))
+void k_degrid_mfma(
     PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
     const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
     int mode) {

@@ -416,8 +416,17 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support) {
   return L;
 }
 
+// Workgroups per CU the FP32 4-polarisation 16 x 16 kernels are compiled for
+// (94 registers, no spills: five wavefronts per SIMD); other variants keep
+// the compiler's choice.
+template <typename R, int NPOL, int MB, int VB, bool COMP>
+constexpr int grid_mfma_min_blocks() {
+  return (sizeof(R) == 4 && !COMP && MB * 2 * NPOL * VB <= 8) ? 5 : 1;
+}
+
 template <typename R, int NPOL, int MB, int VB, bool COMP, bool PSF>
-__global__ __launch_bounds__(kGridThreads) void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
+__global__ __launch_bounds__(kGridThreads, (grid_mfma_min_blocks<R, NPOL, MB, VB, COMP>()))
+void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
                                                             const R *__restrict__ weight,
                                                             const uint8_t *__restrict__ flag,
                                                             const double *__restrict__ cgk,
