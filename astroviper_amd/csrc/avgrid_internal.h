@@ -40,6 +40,7 @@ constexpr int kMaxDevices = 64;
 constexpr int kChunkThreads = 1024;       // threads of a chunked-binning workgroup
 constexpr int kMaxChunkTiles = 16384;     // tiles whose counters fit in 64 KB of LDS
 constexpr int kMaxRowsPerChunk = 31 * kChunkThreads;
+constexpr int kRowBlockRows = 8 * kChunkThreads;  // rows of a row-block binning workgroup
 
 // Packed geometry of one sample relative to a tile origin:
 // bits 0-7 u position + 16, 8-15 u offset + 128, 16-23 v position + 16,
@@ -162,6 +163,9 @@ struct PlanDev {
   int32_t chunked;                  // chunked binning possible for this plan
   int32_t rows_per_chunk, chunks_per_chan;
   uint32_t *chunk_hist;             // [n_chan * chunks_per_chan][ntu * ntv]
+  // row-block binning (rows carry their own channel): kRowBlockRows rows per
+  // workgroup, per-tile counters and cursors in LDS
+  int32_t row_blocks;
   // per-call tables: FP32 kernel table, FP64 per-axis kernel sums by offset
   float *tab32;                     // [ov * (support/2 + 1)]
   double *nrm;                      // [2 * (ov/2 + 2) + 1]

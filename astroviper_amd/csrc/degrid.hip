@@ -562,6 +562,32 @@ __global__ __launch_bounds__(256) void k_degrid_finish(PlanDev P, const uint8_t 
   }
 }
 
+// The same finishing pass for plans with few channels per row (rows that
+// carry their own channel included): one thread per sample, in sample
+// order, so the writes stay coalesced whatever the channel count.
+template <typename R>
+__global__ __launch_bounds__(256) void k_degrid_finish_items(PlanDev P, const uint8_t *flag,
+                                                            R *vis_out, int npol) {
+  const int nch = P.row_chan ? 1 : P.n_chan;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < P.n_items;
+       item += stride) {
+    int64_t row = item;
+    int c = 0;
+    if (nch > 1) {
+      row = item / nch;
+      c = (int)(item - row * nch);
+    }
+    const bool visited = (P.dmask[(int64_t)c * P.mask_words_per_chan + row / 64] >> (row & 63)) & 1ull;
+    R *o = vis_out + (size_t)item * npol * 2;
+    for (int p = 0; p < npol; ++p)
+      if (!visited || (flag && flag[(size_t)item * npol + p])) {
+        o[2 * p] = (R)0;
+        o[2 * p + 1] = (R)0;
+      }
+  }
+}
+
 // non-incremental mode: zero the samples no unit visits (off-grid, NaN uvw,
 // channel outside the grid), read from the plan's degrid-validity bitmask
 template <typename R>
@@ -636,10 +662,17 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
   AV_LAUNCH_CHECK("k_degrid_mfma");
   if (d.n_rows > 0 && !(mode & AV_DEGRID_INCREMENTAL)) {
     const int64_t nch = d.row_chan ? 1 : d.n_chan;
-    const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
-    hipLaunchKernelGGL(k_degrid_finish<R>, dim3((unsigned)blocks), dim3(256), 0, s, d, flag,
-                       (R *)vis_out, NPOL, 0);
-    AV_LAUNCH_CHECK("k_degrid_finish");
+    if (nch >= 32) {
+      const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
+      hipLaunchKernelGGL(k_degrid_finish<R>, dim3((unsigned)blocks), dim3(256), 0, s, d, flag,
+                         (R *)vis_out, NPOL, 0);
+      AV_LAUNCH_CHECK("k_degrid_finish");
+    } else {
+      const int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 16384);
+      hipLaunchKernelGGL(k_degrid_finish_items<R>, dim3((unsigned)blocks), dim3(256), 0, s, d,
+                         flag, (R *)vis_out, NPOL);
+      AV_LAUNCH_CHECK("k_degrid_finish_items");
+    }
   }
   return AV_OK;
 }

@@ -12,9 +12,11 @@
 //
 // Binning is a counting sort.  The counts come from chunks (one channel, a
 // range of rows) gathered in LDS when a channel's tiles fit there, else from
-// global atomics (per-row channels, very many tiles).  The records are then
-// written with one cursor per bin, the samples taken in order so that the
-// records a wavefront writes land in neighbouring bins.  Validity bitmasks are channel-major: bit (chan, row) is
+// global atomics (very many tiles).  The records are then written with one
+// cursor per bin, the samples taken in order so that the records a
+// wavefront writes land in neighbouring bins.  When rows carry their own
+// channel, blocks of consecutive rows count and place their records through
+// per-tile counters and cursors in LDS instead (row-block binning).  Validity bitmasks are channel-major: bit (chan, row) is
 // bit row % 64 of word chan * ceil(n_rows / 64) + row / 64 (chan 0 when
 // every row carries its own channel).
 #include <algorithm>
@@ -132,6 +134,23 @@ __device__ inline void write_masks(const PlanDev &P, int64_t row, int chan, bool
   }
 }
 
+// The records of one sample counted straight into the global bins.
+__device__ inline void count_global(const PlanDev &P, const ItemGeo &g, const RecPlan &rp) {
+  if (rp.gprim) {
+    const int64_t pb = bin_of(P, rp.tu, rp.tv, g.ic);
+    atomicAdd(&P.cnt_prim[pb], 1u);
+    atomicAdd(&P.cnt_all[pb], 1u);
+    for (int a = rp.tu0; a <= rp.tu1; ++a)
+      for (int b = rp.tv0; b <= rp.tv1; ++b)
+        if (a != rp.tu || b != rp.tv) atomicAdd(&P.cnt_all[bin_of(P, a, b, g.ic)], 1u);
+  }
+  if (rp.dsep) {
+    const int64_t db = bin_of(P, rp.du_t, rp.dv_t, g.ic);
+    atomicAdd(&P.cnt_prim[db], 1u);
+    atomicAdd(&P.cnt_all[db], 1u);
+  }
+}
+
 // Global-atomic binning, pass 1: count records per bin.  Threads walk the
 // (channel, row) space with rows padded to a multiple of 64 per channel.
 __global__ __launch_bounds__(256) void k_count(PlanDev P) {
@@ -147,20 +166,7 @@ __global__ __launch_bounds__(256) void k_count(PlanDev P) {
     if (row < P.n_rows) sample_geometry(P, row, chan, g);
     write_masks(P, row, chan, g.gvalid, g.dvalid);
     if (!(g.gvalid || g.dvalid)) continue;
-    const RecPlan rp = record_plan(P, g);
-    if (rp.gprim) {
-      int64_t pb = bin_of(P, rp.tu, rp.tv, g.ic);
-      atomicAdd(&P.cnt_prim[pb], 1u);
-      atomicAdd(&P.cnt_all[pb], 1u);
-      for (int a = rp.tu0; a <= rp.tu1; ++a)
-        for (int b = rp.tv0; b <= rp.tv1; ++b)
-          if (a != rp.tu || b != rp.tv) atomicAdd(&P.cnt_all[bin_of(P, a, b, g.ic)], 1u);
-    }
-    if (rp.dsep) {
-      int64_t db = bin_of(P, rp.du_t, rp.dv_t, g.ic);
-      atomicAdd(&P.cnt_prim[db], 1u);
-      atomicAdd(&P.cnt_all[db], 1u);
-    }
+    count_global(P, g, record_plan(P, g));
   }
 }
 
@@ -189,6 +195,25 @@ __device__ inline void write_degrid_record(const PlanDev &P, const ItemGeo &g, c
   P.rec_d[slot] = pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv);
 }
 
+// The records of one sample, their slots taken from the global bin cursors.
+__device__ inline void scatter_global(const PlanDev &P, const ItemGeo &g, const RecPlan &rp,
+                                      int64_t item) {
+  if (rp.gprim) {
+    const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.tu, rp.tv, g.ic)], 1u);
+    write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
+    for (int a = rp.tu0; a <= rp.tu1; ++a)
+      for (int b = rp.tv0; b <= rp.tv1; ++b)
+        if (a != rp.tu || b != rp.tv) {
+          const uint32_t s2 = atomicAdd(&P.cur_dup[bin_of(P, a, b, g.ic)], 1u);
+          write_records(P, g, rp, item, s2, a, b, false);
+        }
+  }
+  if (rp.dsep) {
+    const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.du_t, rp.dv_t, g.ic)], 1u);
+    write_degrid_record(P, g, rp, item, slot);
+  }
+}
+
 // Binning pass 2: write the records, taking slots from the bins' cursors.
 // Threads walk the samples in order, so a wavefront covers the channels of
 // one or a few rows: their bins are neighbours (bins are channel-minor) and
@@ -206,21 +231,7 @@ __global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
     ItemGeo g;
     sample_geometry(P, row, chan, g);
     if (!(g.gvalid || g.dvalid)) continue;
-    const RecPlan rp = record_plan(P, g);
-    if (rp.gprim) {
-      const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.tu, rp.tv, g.ic)], 1u);
-      write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
-      for (int a = rp.tu0; a <= rp.tu1; ++a)
-        for (int b = rp.tv0; b <= rp.tv1; ++b)
-          if (a != rp.tu || b != rp.tv) {
-            const uint32_t s2 = atomicAdd(&P.cur_dup[bin_of(P, a, b, g.ic)], 1u);
-            write_records(P, g, rp, item, s2, a, b, false);
-          }
-    }
-    if (rp.dsep) {
-      const uint32_t slot = atomicAdd(&P.cur_prim[bin_of(P, rp.du_t, rp.dv_t, g.ic)], 1u);
-      write_degrid_record(P, g, rp, item, slot);
-    }
+    scatter_global(P, g, record_plan(P, g), item);
   }
 }
 
@@ -279,6 +290,124 @@ __global__ void k_chunk_totals(PlanDev P) {
   const int64_t b = (int64_t)t * P.n_img_chan + ic;
   P.cnt_prim[b] = sp;
   P.cnt_all[b] = sp + sd;
+}
+
+// ---- row-block binning (rows carry their own channel) ---------------------
+// A workgroup takes kRowBlockRows consecutive rows.  Rows of one plane are
+// normally stored together, so the records of the block's "own" image
+// channel (that of its first row) are gathered per tile in LDS and reach the
+// global bins with one atomic per tile the block touches; the records of any
+// other image channel take the global path.  This keeps the hot central
+// tiles of a plane from being hit by every wavefront on the chip at once.
+
+// Image channel of a row, or -1 when the row's channel is out of range.
+__device__ inline int row_image_channel(const PlanDev &P, int64_t row) {
+  if (row >= P.n_rows) return -1;
+  const int c = P.row_chan[row];
+  if (c < 0 || c >= P.n_chan) return -1;
+  const int ic = P.chan_map[c];
+  return (ic >= 0 && ic < P.n_img_chan) ? ic : -1;
+}
+
+// Row-block pass 1: per-tile counts, (primaries) | (duplicates << 16); a row
+// adds at most two primaries and one duplicate to a tile and a block has
+// kRowBlockRows < 32768 rows, so neither half overflows.
+__global__ __launch_bounds__(kChunkThreads) void k_count_rows(PlanDev P) {
+  extern __shared__ uint32_t h[];  // [n_tiles]
+  const int64_t r0 = (int64_t)blockIdx.x * kRowBlockRows;
+  const int ntiles = P.ntu * P.ntv;
+  const int own = row_image_channel(P, r0);
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  for (int64_t row = r0 + threadIdx.x; row < r0 + kRowBlockRows; row += blockDim.x) {
+    ItemGeo g;
+    g.gvalid = g.dvalid = false;
+    if (row < P.n_rows) sample_geometry(P, row, 0, g);
+    write_masks(P, row, 0, g.gvalid, g.dvalid);
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    if (g.ic != own) {
+      count_global(P, g, rp);
+      continue;
+    }
+    if (rp.gprim) {
+      atomicAdd(&h[rp.tu * P.ntv + rp.tv], 1u);
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) atomicAdd(&h[a * P.ntv + b], 1u << 16);
+    }
+    if (rp.dsep) atomicAdd(&h[rp.du_t * P.ntv + rp.dv_t], 1u);
+  }
+  __syncthreads();
+  if (own < 0) return;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+    const uint32_t hv = h[i];
+    if (!hv) continue;
+    const int64_t b = (int64_t)i * P.n_img_chan + own;
+    const uint32_t np = hv & 0xFFFFu;
+    if (np) atomicAdd(&P.cnt_prim[b], np);
+    atomicAdd(&P.cnt_all[b], np + (hv >> 16));
+  }
+}
+
+// Row-block pass 2: the block counts its own-channel records per tile again,
+// reserves one run of slots per tile from each bin cursor (primaries and
+// duplicates), and hands out the slots of each run from LDS cursors.
+__global__ __launch_bounds__(kChunkThreads) void k_scatter_rows(PlanDev P) {
+  extern __shared__ uint32_t lds[];
+  const int ntiles = P.ntu * P.ntv;
+  uint32_t *cp = lds, *cd = lds + ntiles;  // [n_tiles] each
+  const int64_t r0 = (int64_t)blockIdx.x * kRowBlockRows;
+  const int64_t r1 = min(r0 + (int64_t)kRowBlockRows, P.n_rows);
+  const int own = row_image_channel(P, r0);
+  for (int i = threadIdx.x; i < 2 * ntiles; i += blockDim.x) lds[i] = 0u;
+  __syncthreads();
+  if (own >= 0) {
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+      ItemGeo g;
+      sample_geometry(P, row, 0, g);
+      if (!(g.gvalid || g.dvalid) || g.ic != own) continue;
+      const RecPlan rp = record_plan(P, g);
+      if (rp.gprim) {
+        atomicAdd(&cp[rp.tu * P.ntv + rp.tv], 1u);
+        for (int a = rp.tu0; a <= rp.tu1; ++a)
+          for (int b = rp.tv0; b <= rp.tv1; ++b)
+            if (a != rp.tu || b != rp.tv) atomicAdd(&cd[a * P.ntv + b], 1u);
+      }
+      if (rp.dsep) atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+      const int64_t b = (int64_t)i * P.n_img_chan + own;
+      if (cp[i]) cp[i] = atomicAdd(&P.cur_prim[b], cp[i]);
+      if (cd[i]) cd[i] = atomicAdd(&P.cur_dup[b], cd[i]);
+    }
+    __syncthreads();
+  }
+  for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+    ItemGeo g;
+    sample_geometry(P, row, 0, g);
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    if (g.ic != own) {
+      scatter_global(P, g, rp, row);
+      continue;
+    }
+    if (rp.gprim) {
+      const uint32_t slot = atomicAdd(&cp[rp.tu * P.ntv + rp.tv], 1u);
+      write_records(P, g, rp, row, slot, rp.tu, rp.tv, true);
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) {
+            const uint32_t s2 = atomicAdd(&cd[a * P.ntv + b], 1u);
+            write_records(P, g, rp, row, s2, a, b, false);
+          }
+    }
+    if (rp.dsep) {
+      const uint32_t slot = atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
+      write_degrid_record(P, g, rp, row, slot);
+    }
+  }
 }
 
 // units per bin: grid units cover every bin (an empty bin still owns its
@@ -532,8 +661,25 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   AV_LAUNCH_CHECK("k_chan_setup");
   int rc;
   const unsigned bb = (unsigned)((d.n_bins + 255) / 256);
+  // row-block binning needs a counter and a cursor per tile in LDS
+  const int64_t ntiles = (int64_t)d.ntu * d.ntv;
+  d.row_blocks = 0;
+  if (row_chan && d.n_rows > 0) {
+    const DeviceProps &prop = device_props();
+    if (prop.error) return hip_fail(prop.error, "av_plan_set_uvw: device properties");
+    d.row_blocks = (size_t)(2 * sizeof(uint32_t) * ntiles) <= prop.lds_per_block;
+  }
+  const unsigned row_grid = (unsigned)((d.n_rows + kRowBlockRows - 1) / kRowBlockRows);
   // pass 1: counts per bin
-  if (d.chunked && !row_chan) {
+  if (d.row_blocks) {
+    if ((rc = launch_fill_u32(d.cnt_all, 0u, d.n_bins, s))) return rc;
+    if ((rc = launch_fill_u32(d.cnt_prim, 0u, d.n_bins, s))) return rc;
+    const int lds = (int)(sizeof(uint32_t) * ntiles);
+    if ((rc = ensure_lds_limit<k_count_rows>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
+      return rc;
+    hipLaunchKernelGGL(k_count_rows, dim3(row_grid), dim3(kChunkThreads), lds, s, d);
+    AV_LAUNCH_CHECK("k_count_rows");
+  } else if (d.chunked && !row_chan) {
     const int64_t nchunks = (int64_t)d.n_chan * d.chunks_per_chan;
     const int lds = (int)(sizeof(uint32_t) * d.ntu * d.ntv);
     if ((rc = ensure_lds_limit<k_count_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
@@ -557,7 +703,13 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   if ((rc = scan_exclusive_u32(d.cnt_all, d.bin_start, d.n_bins, d.scan_tmp, s))) return rc;
   hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
   AV_LAUNCH_CHECK("k_cursors");
-  {
+  if (d.row_blocks) {
+    const int lds = (int)(2 * sizeof(uint32_t) * ntiles);
+    if ((rc = ensure_lds_limit<k_scatter_rows>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
+      return rc;
+    hipLaunchKernelGGL(k_scatter_rows, dim3(row_grid), dim3(kChunkThreads), lds, s, d);
+    AV_LAUNCH_CHECK("k_scatter_rows");
+  } else {
     int64_t blocks = (d.n_items + 255) / 256;
     blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, s, d);
