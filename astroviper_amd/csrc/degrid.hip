@@ -17,6 +17,7 @@
 namespace av {
 
 constexpr int kDegridThreads = 256;
+constexpr int kDegridXcdGroup = 16;  // matrix-core kernel: workgroups per XCD group
 constexpr int kDegridWaves = kDegridThreads / 64;
 constexpr int kRegionBatch = 8;  // grid cells each thread has in flight per pass
 
@@ -340,7 +341,9 @@ void k_degrid_mfma(
   constexpr int VL = 16 / (int)sizeof(R);   // elements per 16-byte read
   constexpr int HK = kSortCap / kDegridThreads;  // records per thread in a sort pass
   extern __shared__ __align__(16) char smem[];
-  const uint32_t u = blockIdx.x;
+  // groups of kDegridXcdGroup consecutive units (neighbouring channels of a
+  // tile) run under one L2
+  const uint32_t u = xcd_grouped_block(blockIdx.x, kDegridXcdGroup);
   if (u >= P.dunit_start[P.n_bins]) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
@@ -656,7 +659,8 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
   }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_degrid_mfma<R, NPOL, NB>), dim3((unsigned)d.max_dunits),
+  hipLaunchKernelGGL((k_degrid_mfma<R, NPOL, NB>),
+                     dim3((unsigned)xcd_grouped_grid(d.max_dunits, kDegridXcdGroup)),
                      dim3(kDegridThreads), L.total, s, d, (const R *)grid, npol_g, pol_map, flag,
                      cgk, (R *)vis_out, mode);
   AV_LAUNCH_CHECK("k_degrid_mfma");

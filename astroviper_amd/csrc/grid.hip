@@ -26,6 +26,7 @@ namespace av {
 constexpr int kGridThreads = 256;
 constexpr int kGridMaxWaves = kGridThreads / 64;
 constexpr int kCompChunkBatches = 4;  // compensated mode: fold every 4*64 records
+constexpr int kGridXcdGroup = 16;     // matrix-core kernel: workgroups per XCD group
 
 template <typename R>
 struct Cplx {
@@ -450,12 +451,14 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   const int half = ov / 2 + 2;
 
   const uint32_t n_units = P.gunit_start[P.n_bins];
-  if ((uint32_t)blockIdx.x * kGridMaxWaves >= n_units) return;  // whole workgroup idle
+  // 16 workgroups of 4 units: the 64 channels of a tile under one L2
+  const uint32_t blk = xcd_grouped_block(blockIdx.x, kGridXcdGroup);
+  if (blk * kGridMaxWaves >= n_units) return;  // whole workgroup idle
   const WTab W(ov, S);
   copy_wtab(tab, W, P);
   for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
   __syncthreads();  // the only workgroup barrier: tables ready
-  const uint32_t u = blockIdx.x * kGridMaxWaves + wave;
+  const uint32_t u = blk * kGridMaxWaves + wave;
   if (u >= n_units) return;
 
   const int Tu = 16 * MB, Tv = 16 * VB;
@@ -783,7 +786,8 @@ static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
   }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  const unsigned nwg = (unsigned)((d.max_gunits + kGridMaxWaves - 1) / kGridMaxWaves);
+  const unsigned nwg = (unsigned)xcd_grouped_grid(
+      (d.max_gunits + kGridMaxWaves - 1) / kGridMaxWaves, kGridXcdGroup);
   hipLaunchKernelGGL(kfn, dim3(nwg), dim3(kGridThreads), L.total, s, d, (const R *)vis,
                      (const R *)weight, flag, cgk, (R *)grid, mode);
   AV_LAUNCH_CHECK("k_grid_mfma");

@@ -82,6 +82,23 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups are dealt to the chip's 8 XCDs (each with its own L2) round
+// robin.  XcdGroups renumbers them so that each XCD receives whole groups of
+// G consecutive workgroups, the groups themselves dealt out round robin:
+// units of neighbouring channels (which share the cache lines of the
+// visibilities, weights and flags of a row) then run under one L2, while the
+// dense and sparse parts of the grid stay spread over all XCDs.  The grid
+// size must be a multiple of 8 * G; the map is then a permutation of
+// [0, gridDim.x).
+constexpr int kXcds = 8;
+__host__ __device__ inline uint32_t xcd_grouped_block(uint32_t b, uint32_t g) {
+  const uint32_t k = b / kXcds;  // position in this XCD's sequence
+  return ((k / g) * kXcds + b % kXcds) * g + k % g;
+}
+__host__ __device__ inline int64_t xcd_grouped_grid(int64_t blocks, int64_t g) {
+  return (blocks + kXcds * g - 1) / (kXcds * g) * (kXcds * g);
+}
+
 // |ov*(tap - sc) + off| into the half-kernel table (standard_grid.py:384-396)
 __device__ inline int tab_index(int ov, int tap, int sc, int off) {
   int k = ov * (tap - sc) + off;
