@@ -63,7 +63,11 @@ __host__ __device__ inline GridLds grid_lds_layout(int ov, int support, int tu, 
   return L;
 }
 
-// Raw per-record loads of one lane: visibilities, weights, flags.
+// Raw per-record loads of one lane: visibilities, weights, flags.  The loads
+// are unconditional -- a lane without a record passes an in-bounds sample
+// (its range's last record) and the caller discards the data -- so no branch
+// separates a load from its use and the loads stay in flight while the
+// previous batch is applied.
 template <typename R, int NPOL>
 struct RawSample {
   Cplx<R> vv[NPOL];
@@ -72,22 +76,39 @@ struct RawSample {
 };
 
 template <typename R, int NPOL>
-__device__ inline void fetch_raw(RawSample<R, NPOL> &s, uint32_t g, uint32_t item,
+__device__ inline void fetch_raw(RawSample<R, NPOL> &s, uint32_t item,
                                  const R *__restrict__ vis, const R *__restrict__ weight,
                                  const uint8_t *__restrict__ flag, bool psf, bool adjoint) {
   s.fbits = 0;
-  if (g == kInvalidGeo) return;
+  if (!adjoint) {
+    const Vec<R, NPOL> w = *(const Vec<R, NPOL> *)(weight + (size_t)item * NPOL);
 #pragma unroll
-  for (int p = 0; p < NPOL; ++p) {
-    if (!adjoint) s.wv[p] = weight[(size_t)item * NPOL + p];
-    if (!psf) s.vv[p] = ((const Cplx<R> *)vis)[(size_t)item * NPOL + p];
+    for (int p = 0; p < NPOL; ++p) s.wv[p] = w.v[p];
+  }
+  if (!psf) {
+    constexpr int K = CellLayout<R, NPOL>::kCC;  // components per 16-byte piece
+#pragma unroll
+    for (int k = 0; k < 2 * NPOL; k += K) {
+      const Vec<R, K> t = *(const Vec<R, K> *)(vis + (size_t)item * 2 * NPOL + k);
+#pragma unroll
+      for (int c = 0; c < K; c += 2) s.vv[(k + c) / 2] = Cplx<R>{t.v[c], t.v[c + 1]};
+    }
   }
   if (flag && !adjoint) {
     if (NPOL == 4)
       s.fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+    else if (NPOL == 2)
+      s.fbits = *(const uint16_t *)(flag + (size_t)item * 2);
     else
-      for (int p = 0; p < NPOL; ++p) s.fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+      s.fbits = flag[item];
   }
+}
+
+// One lane's record of the range ending at `end` (end > first): a lane past
+// the end reads the range's last record, which keeps the load unconditional;
+// the caller knows from base + lane < end whether the record is its own.
+__device__ inline uint2 load_record(const PlanDev &P, uint32_t idx, uint32_t end) {
+  return P.rec[min(idx, end - 1)];
 }
 
 template <typename R, int NPOL, int DVW, bool COMP>
@@ -181,23 +202,14 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   // software pipeline: records of batch k+2 and sample data of batch k+1
   // are in flight while batch k is applied
   const uint32_t end = first + cnt;
-  uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
-  if (first + lane < end) {
-    const uint2 r = P.rec[first + lane];
-    i0 = r.x;
-    g0 = r.y;
-  }
-  if (first + 64 + lane < end) {
-    const uint2 r = P.rec[first + 64 + lane];
-    i1 = r.x;
-    g1 = r.y;
-  }
+  uint2 r0 = load_record(P, first + lane, end), r1 = load_record(P, first + 64 + lane, end);
   RawSample<R, NPOL> raw0;
-  fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
+  fetch_raw<R, NPOL>(raw0, r0.x, vis, weight, flag, psf, adjoint);
   int batch = 0;
   for (uint32_t base = first; base < end; base += 64, ++batch) {
     // ---- stage this lane's record of the batch
     uint32_t geo = kInvalidGeo;
+    const uint32_t g0 = base + lane < end ? r0.y : kInvalidGeo;
     if (g0 != kInvalidGeo) {
       int au, ou, avv, ovv;
       unpack_geo(g0, au, ou, avv, ovv);
@@ -248,13 +260,8 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
     }
     // ---- issue the next loads
     RawSample<R, NPOL> raw1;
-    fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
-    uint32_t g2 = kInvalidGeo, i2 = 0;
-    if (base + 128 + lane < end) {
-      const uint2 r = P.rec[base + 128 + lane];
-      i2 = r.x;
-      g2 = r.y;
-    }
+    fetch_raw<R, NPOL>(raw1, r1.x, vis, weight, flag, psf, adjoint);
+    const uint2 r2 = load_record(P, base + 128 + lane, end);
     wave_lds_sync();
     // ---- apply the staged samples one at a time
     const int nst = (int)min((uint32_t)64, end - base);
@@ -300,10 +307,8 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
       wave_lds_sync();
     }
     raw0 = raw1;
-    g0 = g1;
-    i0 = i1;
-    g1 = g2;
-    i1 = i2;
+    r0 = r1;
+    r1 = r2;
   }
 
   // ---- flush the tile
@@ -527,19 +532,9 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
 
   const uint32_t end = first + cnt;
-  uint32_t g0 = kInvalidGeo, i0 = 0, g1 = kInvalidGeo, i1 = 0;
-  if (first + lane < end) {
-    const uint2 r = P.rec[first + lane];
-    i0 = r.x;
-    g0 = r.y;
-  }
-  if (first + 64 + lane < end) {
-    const uint2 r = P.rec[first + 64 + lane];
-    i1 = r.x;
-    g1 = r.y;
-  }
+  uint2 r0 = load_record(P, first + lane, end), r1 = load_record(P, first + 64 + lane, end);
   RawSample<R, NPOL> raw0;
-  fetch_raw<R, NPOL>(raw0, g0, i0, vis, weight, flag, psf, adjoint);
+  fetch_raw<R, NPOL>(raw0, r0.x, vis, weight, flag, psf, adjoint);
   int batch = 0;
   for (uint32_t base = first; base < end; base += 64, ++batch) {
     // ---- stage this lane's record: data (zeros when skipped) and geometry
@@ -547,6 +542,7 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
     R d[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) d[k] = (R)0;
+    const uint32_t g0 = base + lane < end ? r0.y : kInvalidGeo;
     if (g0 != kInvalidGeo) {
       int au, ou, avv, ovv;
       unpack_geo(g0, au, ou, avv, ovv);
@@ -592,13 +588,8 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
     st_geo[lane] = geo;
     // ---- issue the next loads
     RawSample<R, NPOL> raw1;
-    fetch_raw<R, NPOL>(raw1, g1, i1, vis, weight, flag, psf, adjoint);
-    uint32_t g2 = kInvalidGeo, i2 = 0;
-    if (base + 128 + lane < end) {
-      const uint2 r = P.rec[base + 128 + lane];
-      i2 = r.x;
-      g2 = r.y;
-    }
+    fetch_raw<R, NPOL>(raw1, r1.x, vis, weight, flag, psf, adjoint);
+    const uint2 r2 = load_record(P, base + 128 + lane, end);
     wave_lds_sync();
     // ---- four records per matrix-core step
     const int nst = (int)min((uint32_t)64, end - base);
@@ -637,10 +628,8 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
     wave_lds_sync();
     if (COMP && ((batch + 1) % kCompChunkBatches == 0) && base + 64 < end) fold(true);
     raw0 = raw1;
-    g0 = g1;
-    i0 = i1;
-    g1 = g2;
-    i1 = i2;
+    r0 = r1;
+    r1 = r2;
   }
   if (COMP) fold(false);
 
