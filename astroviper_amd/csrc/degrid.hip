@@ -19,7 +19,7 @@ namespace av {
 constexpr int kDegridThreads = 256;
 constexpr int kDegridXcdGroup = 16;  // matrix-core kernel: workgroups per XCD group
 constexpr int kDegridWaves = kDegridThreads / 64;
-constexpr int kRegionBatch = 8;  // grid cells each thread has in flight per pass
+constexpr int kRegionBatch = 12;  // grid values each thread has in flight per pass
 
 // Copy the region [u0 - halo, u0 - halo + rows) x [v0 - halo, v0 - halo +
 // cols) of every visibility polarisation's grid plane (through pol_map) into
@@ -371,12 +371,14 @@ void k_degrid_mfma(
     return g == kInvalidGeo ? Tu + 2 : (k < 0 ? 0 : (k > Tu + 1 ? Tu + 1 : k));
   };
   // records of the first sort pass: in flight during the region load
+  // (unconditional loads: a thread past the end rereads the last record and
+  // its copy is never used)
   uint32_t pg[HK], pi[HK];
 #pragma unroll
   for (int k = 0; k < HK; ++k) {
-    const uint32_t r = first + tid + k * kDegridThreads;
-    pg[k] = r < end ? P.rec_d[r] : kInvalidGeo;
-    pi[k] = r < end ? P.rec[r].x : 0u;
+    const uint32_t r = min(first + tid + k * kDegridThreads, end - 1);
+    pg[k] = P.rec_d[r];
+    pi[k] = P.rec[r].x;
   }
 
   const WTab W(ov, S);
@@ -402,9 +404,9 @@ void k_degrid_mfma(
     if (c0 != first) {  // later passes of a large unit
 #pragma unroll
       for (int k = 0; k < HK; ++k) {
-        const int r = tid + k * kDegridThreads;
-        pg[k] = r < n ? P.rec_d[c0 + r] : kInvalidGeo;
-        pi[k] = r < n ? P.rec[c0 + r].x : 0u;
+        const uint32_t r = c0 + min(tid + k * kDegridThreads, n - 1);
+        pg[k] = P.rec_d[r];
+        pi[k] = P.rec[r].x;
       }
     }
     // ---- counting sort of the pass's records by key, in LDS

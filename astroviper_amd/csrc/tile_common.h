@@ -134,11 +134,26 @@ struct WTab {
 };
 
 // Copy the call's zero-padded weight table (built once per call by
-// prepare_tables) into LDS with every thread of the workgroup.
+// prepare_tables) into LDS with every thread of the workgroup: four loads per
+// thread in flight at a time (one round trip for the common tables), each
+// from an in-bounds index so that no branch separates a load from its use.
 template <typename R>
 __device__ inline void copy_wtab(R *w, const WTab &W, const PlanDev &P) {
-  for (int i = threadIdx.x; i < W.size(); i += blockDim.x)
-    w[i] = sizeof(R) == 4 ? (R)P.wtab32[i] : (R)P.wtab64[i];
+  constexpr int K = 4;
+  const R *src;
+  if constexpr (sizeof(R) == 4)
+    src = P.wtab32;
+  else
+    src = P.wtab64;
+  const int n = W.size();
+  for (int i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
+    R v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = src[min(i0 + k * (int)blockDim.x, n - 1)];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + k * (int)blockDim.x < n) w[i0 + k * blockDim.x] = v[k];
+  }
 }
 
 }  // namespace av
