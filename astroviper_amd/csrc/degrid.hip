@@ -282,6 +282,19 @@ constexpr int kSortKeys = 136;  // rows -1 .. tile_u (<= 128) and one key for sk
 // l % 16 of every block) are contiguous and load as 16-byte reads; rows are
 // padded by 16 bytes so that the four rows of one step fall in different
 // banks.
+// part / norm for the matrix-core degridder, where norm > 0 is the sum of
+// the support's kernel weights.  FP32: hardware reciprocal, quotient, and one
+// fused correction step (Markstein's refinement: q + r * (a - b * q) with the
+// residual from a fused multiply-add), which is within one unit in the last
+// place and exact whenever the quotient is a power-of-two multiple (a = g * b,
+// a constant grid of g).  FP64 keeps IEEE division.
+__device__ inline float div_by_norm(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
+__device__ inline double div_by_norm(double a, double b) { return a / b; }
+
 template <typename R, int NPOL, int NB>
 __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support, int tu, int tv) {
   DegridMfmaLds L;
@@ -472,15 +485,16 @@ void k_degrid_mfma(
         rlo = min(rlo, __shfl_xor(rlo, off));
         rhi = max(rhi, __shfl_xor(rhi, off));
       }
+      // every quarter-wave holds the same 16 records, so this is uniform
+      if (rlo > rhi) continue;  // no record of the group has a degrid centre
       // v-weights of record li at every region column (lanes of one record
       // split the columns)
       for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
 
-      MV T[NB], T1 = MV{0, 0, 0, 0};
-#pragma unroll
-      for (int k = 0; k < NB; ++k) T[k] = MV{0, 0, 0, 0};
-      // row steps outside the band would only add zeros
-      for (int s4 = rlo >> 2; s4 <= (rhi >> 2); ++s4) {
+      // row steps outside the band would only add zeros; the first step
+      // starts the sums from zero
+      MV T[NB], T1;
+      auto row_step = [&](int s4, bool first_step) {
         const int row = 4 * s4 + lk;
         const R a = tab[W.index(row - urow0, ou)];
         const R *rp = reg + row * RS + li * NB;
@@ -491,10 +505,13 @@ void k_degrid_mfma(
 #pragma unroll
           for (int e = 0; e < VL; ++e) bv[k + e] = t.v[e];
         }
+        const MV zero{0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], T[k]);
-        T1 = MO::mma(a, (R)1, T1);
-      }
+        for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], first_step ? zero : T[k]);
+        T1 = MO::mma(a, (R)1, first_step ? zero : T1);
+      };
+      row_step(rlo >> 2, true);
+      for (int s4 = (rlo >> 2) + 1; s4 <= (rhi >> 2); ++s4) row_step(s4, false);
       wave_lds_sync();
       // ---- weighted sums over v: lane holds records MO::row(l/16, q),
       // cell k*CPB + li/C, component li%C of every block k
@@ -519,10 +536,11 @@ void k_degrid_mfma(
         if (li >= 16 - C && gq != kInvalidGeo) {
           const int c = li - (16 - C);
           R *dst = vis_out + (size_t)itq * C + c;
+          const R val = div_by_norm(part, npart);
           if (incremental) {
-            if (!((fq >> (8 * (c >> 1))) & 0xFFu)) *dst += part / npart;
+            if (!((fq >> (8 * (c >> 1))) & 0xFFu)) *dst += val;
           } else {
-            *dst = part / npart;
+            *dst = val;
           }
         }
       }
