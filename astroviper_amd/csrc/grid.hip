@@ -402,24 +402,58 @@ struct MfmaOp<double> {
 
 struct MfmaLds {
   int tab, nrm, st_d, st_geo, wave_bytes, total, ntab, nnrm;
+  int pad, nw, rows;  // weight table: rows of taps -pad .. S + pad - 1, nw offsets each
 };
 
+// The matrix-core kernel's weight table is the call's table (taps -1 .. S)
+// with zero rows added on both sides, so that every tap a 16 x 16 block can
+// ask for -- tile-relative positions run from -(S - 1 - S/2) to T - 1 + S/2,
+// taps from 1 - T to T + S - 2 -- is a row of it and no index needs a clamp.
 template <typename R, int NPOL>
-__host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support) {
+__host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support, int tile) {
   MfmaLds L;
   const int sc = support / 2;
   L.ntab = ov * (sc + 1);
   L.nnrm = 2 * (ov / 2 + 2) + 1;
+  L.pad = tile;
+  L.nw = WTab(ov, support).nw;
+  L.rows = support + 2 * L.pad;
   int o = 0;
-  L.tab = o;  // the zero-padded weight table (WTab)
-  o += (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
+  L.tab = o;
+  o += (L.rows * L.nw * (int)sizeof(R) + 15) & ~15;
   L.nrm = o;
   o += (L.nnrm * (int)sizeof(double) + 15) & ~15;
   L.st_d = o;                           // per wave: [64][2*NPOL] R
-  L.st_geo = 64 * 2 * NPOL * (int)sizeof(R);  // offset inside the wave region
-  L.wave_bytes = L.st_geo + 64 * 4;
+  L.st_geo = 64 * 2 * NPOL * (int)sizeof(R);  // offset inside the wave region: [64] uint2
+  L.wave_bytes = L.st_geo + 64 * 8;
   L.total = o + kGridMaxWaves * L.wave_bytes;
   return L;
+}
+
+// Fill the padded table: rows -1 .. S from the call's table, zeros elsewhere.
+template <typename R>
+__device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P) {
+  const R *src;
+  if constexpr (sizeof(R) == 4)
+    src = P.wtab32;
+  else
+    src = P.wtab64;
+  const int S = P.support, n = L.rows * L.nw, nsrc = (S + 2) * L.nw;
+  constexpr int K = 4;  // loads in flight per thread
+  for (int i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
+    R v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      // row r of the padded table is tap r - pad, row r - pad + 1 of the source
+      const int i = i0 + k * (int)blockDim.x;
+      const int j = i - (L.pad - 1) * L.nw;
+      v[k] = src[min(max(j, 0), nsrc - 1)];
+      if (j < 0 || j >= nsrc) v[k] = (R)0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + k * (int)blockDim.x < n) w[i0 + k * blockDim.x] = v[k];
+  }
 }
 
 // Workgroups per CU the FP32 4-polarisation 16 x 16 kernels are compiled for
@@ -442,13 +476,13 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   constexpr int C = 2 * NPOL;
   extern __shared__ __align__(16) char smem[];
   const int S = P.support, sc = S / 2, ov = P.ov;
-  const MfmaLds L = mfma_lds_layout<R, NPOL>(ov, S);
+  const MfmaLds L = mfma_lds_layout<R, NPOL>(ov, S, 16 * (MB > VB ? MB : VB));
   R *tab = (R *)(smem + L.tab);
   double *nrm = (double *)(smem + L.nrm);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   char *wbase = smem + L.st_d + wave * L.wave_bytes;
   R *st_d = (R *)wbase;                           // [record][C]
-  uint32_t *st_geo = (uint32_t *)(wbase + L.st_geo);  // [record]
+  uint2 *st_geo = (uint2 *)(wbase + L.st_geo);  // [record] table bases of the u and v taps
 
   constexpr bool psf = PSF;  // PSF data are real: the imaginary blocks are skipped
   const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
@@ -459,8 +493,7 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   // 16 workgroups of 4 units: the 64 channels of a tile under one L2
   const uint32_t blk = xcd_grouped_block(blockIdx.x, kGridXcdGroup);
   if (blk * kGridMaxWaves >= n_units) return;  // whole workgroup idle
-  const WTab W(ov, S);
-  copy_wtab(tab, W, P);
+  copy_wtab_padded(tab, L, P);
   for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
   __syncthreads();  // the only workgroup barrier: tables ready
   const uint32_t u = blk * kGridMaxWaves + wave;
@@ -585,7 +618,19 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
       for (int c = 0; c < CellLayout<R, NPOL>::kCC; ++c) t.v[c] = d[k + c];
       *(Vec<R, CellLayout<R, NPOL>::kCC> *)(st_d + lane * C + k) = t;
     }
-    st_geo[lane] = geo;
+    {
+      // table index of tap (li - position + sc) at this record's offsets is
+      // base + li * nw (taps -pad .. are rows 0 ..); a skipped record has
+      // zero data, so any in-range base (0) serves it
+      uint2 tb = make_uint2(0u, 0u);
+      if (geo != kInvalidGeo) {
+        int au, ou, avv, ovv;
+        unpack_geo(geo, au, ou, avv, ovv);
+        tb.x = (uint32_t)((sc - au + L.pad) * L.nw + ou + L.nw / 2);
+        tb.y = (uint32_t)((sc - avv + L.pad) * L.nw + ovv + L.nw / 2);
+      }
+      st_geo[lane] = tb;
+    }
     // ---- issue the next loads
     RawSample<R, NPOL> raw1;
     fetch_raw<R, NPOL>(raw1, r1.x, vis, weight, flag, psf, adjoint);
@@ -593,17 +638,15 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
     wave_lds_sync();
     // ---- four records per matrix-core step
     const int nst = (int)min((uint32_t)64, end - base);
+    const int li_nw = li * L.nw;
     for (int s4 = 0; s4 < nst; s4 += 4) {
       const int rec = s4 + lk;
-      // skipped records become a far-away geometry: all their weights are 0
-      const uint32_t g = geo_or_far(st_geo[rec]);
-      int au, ou, avv, ovv;
-      unpack_geo(g, au, ou, avv, ovv);
+      const uint2 tb = st_geo[rec];
       R a[MB], wv[VB];
 #pragma unroll
-      for (int m = 0; m < MB; ++m) a[m] = tab[W.index(16 * m + li - au + sc, ou)];
+      for (int m = 0; m < MB; ++m) a[m] = tab[tb.x + li_nw + 16 * m * L.nw];
 #pragma unroll
-      for (int vb = 0; vb < VB; ++vb) wv[vb] = tab[W.index(16 * vb + li - avv + sc, ovv)];
+      for (int vb = 0; vb < VB; ++vb) wv[vb] = tab[tb.y + li_nw + 16 * vb * L.nw];
       R dd[C];
 #pragma unroll
       for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {
@@ -768,7 +811,7 @@ static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
   auto kfn = k_grid_mfma<R, NPOL, MB, VB, COMP, PSF>;
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
-  const MfmaLds L = mfma_lds_layout<R, NPOL>(d.ov, d.support);
+  const MfmaLds L = mfma_lds_layout<R, NPOL>(d.ov, d.support, 16 * (MB > VB ? MB : VB));
   {
     const int rc_lds = ensure_lds_limit<k_grid_mfma<R, NPOL, MB, VB, COMP, PSF>>(L.total, "av_grid: hipFuncSetAttribute");
     if (rc_lds) return rc_lds;
