@@ -554,9 +554,9 @@ void k_degrid_mfma(
 // channel outside the grid) and, unless incremental, the flagged ones.  A
 // workgroup covers 64 rows x 64 channels: one validity word per channel and
 // contiguous flag reads along channels.
-template <typename R>
+template <typename R, int NPOL>
 __global__ __launch_bounds__(256) void k_degrid_finish(PlanDev P, const uint8_t *flag,
-                                                      R *vis_out, int npol, int incremental) {
+                                                      R *vis_out) {
   const int nch = P.row_chan ? 1 : P.n_chan;
   const int64_t ncb = (nch + 63) / 64;
   const int64_t blk = blockIdx.x;
@@ -564,24 +564,35 @@ __global__ __launch_bounds__(256) void k_degrid_finish(PlanDev P, const uint8_t 
   const int c = (int)((blk - rw * ncb) * 64) + (threadIdx.x & 63);
   if (c >= nch) return;
   const unsigned long long m = P.dmask[(int64_t)c * P.mask_words_per_chan + rw];
-  for (int k = threadIdx.x >> 6; k < 64; k += 4) {
+  // this thread's 16 samples: every flag word is loaded (from an in-bounds
+  // sample) before any store, then the unvisited and flagged ones are zeroed
+  constexpr int K = 64 / 4;
+  uint32_t fb[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int64_t row = min(rw * 64 + (threadIdx.x >> 6) + 4 * j, P.n_rows - 1);
+    const int64_t item = P.row_chan ? row : row * P.n_chan + c;
+    fb[j] = 0;
+    if (flag) {
+      if (NPOL == 4)
+        fb[j] = *(const uint32_t *)(flag + (size_t)item * 4);
+      else if (NPOL == 2)
+        fb[j] = *(const uint16_t *)(flag + (size_t)item * 2);
+      else
+        fb[j] = flag[item];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int k = (threadIdx.x >> 6) + 4 * j;
     const int64_t row = rw * 64 + k;
     if (row >= P.n_rows) break;
     const int64_t item = P.row_chan ? row : row * P.n_chan + c;
-    bool zero = !((m >> k) & 1ull);
-    if (!zero && !incremental && flag) {
-      // a sample's polarisations are flagged independently
-      R *o = vis_out + (size_t)item * npol * 2;
-      for (int p = 0; p < npol; ++p)
-        if (flag[(size_t)item * npol + p]) {
-          o[2 * p] = (R)0;
-          o[2 * p + 1] = (R)0;
-        }
-    }
-    if (zero && !incremental) {
-      R *o = vis_out + (size_t)item * npol * 2;
-      for (int q = 0; q < 2 * npol; ++q) o[q] = (R)0;
-    }
+    const bool visited = (m >> k) & 1ull;
+    Cplx2<R> *o = (Cplx2<R> *)vis_out + (size_t)item * NPOL;
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p)
+      if (!visited || ((fb[j] >> (8 * p)) & 0xFFu)) o[p] = Cplx2<R>{(R)0, (R)0};
   }
 }
 
@@ -688,8 +699,8 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
     const int64_t nch = d.row_chan ? 1 : d.n_chan;
     if (nch >= 32) {
       const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
-      hipLaunchKernelGGL(k_degrid_finish<R>, dim3((unsigned)blocks), dim3(256), 0, s, d, flag,
-                         (R *)vis_out, NPOL, 0);
+      hipLaunchKernelGGL((k_degrid_finish<R, NPOL>), dim3((unsigned)blocks), dim3(256), 0, s, d,
+                         flag, (R *)vis_out);
       AV_LAUNCH_CHECK("k_degrid_finish");
     } else {
       const int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 16384);
