@@ -331,8 +331,10 @@ struct DegridGroup {
 // are in flight together; the records are then sorted by tile row in LDS,
 // so that the 16 records of a group span a narrow band of region rows and
 // only the row steps of that band are run (the others would add exact
-// zeros).  Without AV_DEGRID_INCREMENTAL no flags are read here: the
-// finishing pass zeroes flagged samples afterwards.
+// zeros).  A record's flags are read with its group and applied as its
+// model is written: a flagged polarisation is set to zero (left alone when
+// AV_DEGRID_INCREMENTAL); the finishing pass zeroes the samples no unit
+// visits.
 // Workgroups per CU the FP32 kernels with up to 12 column blocks are compiled
 // for (at most 128 registers without spills, and four regions fit in LDS);
 // other variants keep the compiler's choice.
@@ -466,13 +468,19 @@ void k_degrid_mfma(
       const int r = g * 16 + li;
       const uint32_t geo = r < n ? s_geo[r] : kInvalidGeo;
       const uint32_t item = r < n ? s_item[r] : 0u;
+      // the record's flags, used when its model is written (a flagged
+      // polarisation is left alone when incremental, set to zero otherwise).
+      // The load is unconditional -- a lane without a record reads its slot's
+      // sample (0 or a valid one) -- and its result is needed only at the end
+      // of the group; neighbouring channels share the flag line in L2
       uint32_t fbits = 0;
-      if (incremental && flag && geo != kInvalidGeo) {
+      if (flag) {
         if (NPOL == 4)
           fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+        else if (NPOL == 2)
+          fbits = *(const uint16_t *)(flag + (size_t)item * 2);
         else
-          for (int p = 0; p < NPOL; ++p)
-            fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
+          fbits = flag[item];
       }
       int au, ou, avv, ovv;
       unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
@@ -537,10 +545,11 @@ void k_degrid_mfma(
           const int c = li - (16 - C);
           R *dst = vis_out + (size_t)itq * C + c;
           const R val = div_by_norm(part, npart);
+          const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
           if (incremental) {
-            if (!((fq >> (8 * (c >> 1))) & 0xFFu)) *dst += val;
+            if (!flagged) *dst += val;
           } else {
-            *dst = val;
+            *dst = flagged ? (R)0 : val;
           }
         }
       }
@@ -551,9 +560,9 @@ void k_degrid_mfma(
 }
 
 // Finishing pass: zero the samples no degrid unit visits (off-grid, NaN uvw,
-// channel outside the grid) and, unless incremental, the flagged ones.  A
-// workgroup covers 64 rows x 64 channels: one validity word per channel and
-// contiguous flag reads along channels.
+// channel outside the grid) and, when `flag` is given, the flagged
+// polarisations of every sample.  A workgroup covers 64 rows x 64 channels:
+// one validity word per channel and contiguous flag reads along channels.
 template <typename R, int NPOL>
 __global__ __launch_bounds__(256) void k_degrid_finish(PlanDev P, const uint8_t *flag,
                                                       R *vis_out) {
@@ -699,13 +708,15 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
     const int64_t nch = d.row_chan ? 1 : d.n_chan;
     if (nch >= 32) {
       const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
+      // the matrix-core kernel has zeroed the flagged polarisations of every
+      // sample it visited: only the unvisited samples are left
       hipLaunchKernelGGL((k_degrid_finish<R, NPOL>), dim3((unsigned)blocks), dim3(256), 0, s, d,
-                         flag, (R *)vis_out);
+                         (const uint8_t *)nullptr, (R *)vis_out);
       AV_LAUNCH_CHECK("k_degrid_finish");
     } else {
       const int64_t blocks = std::min<int64_t>((d.n_items + 255) / 256, 16384);
       hipLaunchKernelGGL(k_degrid_finish_items<R>, dim3((unsigned)blocks), dim3(256), 0, s, d,
-                         flag, (R *)vis_out, NPOL);
+                         (const uint8_t *)nullptr, (R *)vis_out, NPOL);
       AV_LAUNCH_CHECK("k_degrid_finish_items");
     }
   }
