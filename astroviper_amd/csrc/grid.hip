@@ -844,6 +844,13 @@ static int try_mfma(av_plan *plan, const void *vis, const void *weight, const ui
                     bool &done) {
   if constexpr (MB * 2 * NPOL * VB <= mfma_max_blocks<R, COMP>()) {
     if (plan->d.tile_u == 16 * MB && plan->d.tile_v == 16 * VB) {
+      // the padded weight table grows with the tile, the support and the
+      // oversampling: when it does not fit in LDS the LDS tile kernel runs
+      const DeviceProps &prop = device_props();
+      if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
+      const MfmaLds L = mfma_lds_layout<R, NPOL>(plan->d.ov, plan->d.support,
+                                                 16 * (MB > VB ? MB : VB));
+      if ((size_t)L.total > prop.lds_per_block) return AV_OK;
       done = true;
       if (mode & AV_GRID_PSF)
         return launch_grid_mfma<R, NPOL, MB, VB, COMP, true>(plan, vis, weight, flag, cgk, grid,

@@ -262,15 +262,26 @@ class GridPlan:
 
 
 # ------------------------------------------------------ kernel choice --
-def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = False) -> str:
+def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = False,
+                     support: int = 7, oversampling: int = 100,
+                     lds_bytes: int = 160 * 1024) -> str:
     """Name of the tile kernel av_grid runs for this plan shape: the
     matrix-core kernel for tiles of 16 x 16 multiples whose result blocks
     ((tile_u/16) * 2*n_pol * (tile_v/16)) fit its register budget (16 blocks
-    for FP32, 8 for FP64 and for compensated sums), else the LDS kernel."""
+    for FP32, 8 for FP64 and for compensated sums) and whose LDS (a weight
+    table padded by 16 * max(tile_u, tile_v)/16 zero rows per side, the FP64
+    kernel-sum table and four wavefronts' staging) fits in a workgroup's
+    `lds_bytes`, else the LDS kernel."""
     tu, tv = int(tile[0]), int(tile[1])
     limit = 8 if (fp64 or compensated) else 16
     if tu in (16, 32) and tv in (16, 32, 64) and (tu // 16) * 2 * n_pol * (tv // 16) <= limit:
-        return "k_grid_mfma"
+        r = 8 if fp64 else 4
+        nw = 2 * (oversampling // 2 + 2) + 1
+        pad = max(tu, tv)
+        up16 = lambda b: (b + 15) & ~15
+        total = up16((support + 2 * pad) * nw * r) + up16(nw * 8) + 4 * (64 * 2 * n_pol * r + 64 * 8)
+        if total <= lds_bytes:
+            return "k_grid_mfma"
     return "k_grid_tiles"
 
 

@@ -367,3 +367,34 @@ def test_plain_gridder_matches_oracle(gpu, oracle, prec):
                      torch.from_numpy(flag.reshape(rows, nchan, npol)).to(gpu), cell,
                      list(range(nchan)), npol).cpu().numpy()
     assert rel(m.reshape(mref.shape), mref) < tol
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_widest_kernel_table(gpu, oracle, prec):
+    """Support 16 at oversampling 250 on 32 x 16 tiles: in FP64 the matrix-core
+    kernel's padded weight table (80 rows x 255 offsets, 163 KB) does not fit
+    in LDS and the LDS tile kernel runs; in FP32 (82 KB) it fits where a
+    workgroup may have 160 KiB.  Either way the grid and sums of weights must
+    be the oracle's, and the degridder's model too."""
+    from astroviper_amd.device import grid_kernel_name
+    if prec == 64:  # beyond any workgroup's LDS, 64 KiB or 160 KiB
+        assert grid_kernel_name((32, 16), 1, fp64=True, support=16,
+                                oversampling=250) == "k_grid_tiles"
+    rng = np.random.default_rng(12)
+    nt, nb, nchan, npol, n = 4, 90, 2, 1, 192
+    uvw = rng.uniform(-1500, 1500, size=(nt, nb, 3))
+    freq = np.linspace(1.0e9, 1.1e9, nchan)
+    cell = (-0.5 * C / freq[-1] / 2600.0, 0.5 * C / freq[-1] / 2600.0)
+    vis = rng.standard_normal((nt, nb, nchan, npol)) + 1j * rng.standard_normal((nt, nb, nchan, npol))
+    w = rng.uniform(0.5, 1.5, size=(nt, nb, nchan, npol))
+    vis_p, w_p = as_prec(prec, vis, w)
+    gref, swref = oracle.standard_grid(vis_p, uvw, w_p, freq, (n, n), cell, support=16,
+                                       oversampling=250, chan_mode="cube")
+    g, sw, plan = device_grid(gpu, vis, uvw, w, None, freq, (n, n), cell, True, prec,
+                              support=16, ov=250, tile=(32, 16))
+    assert rel(g.cpu().numpy(), gref) < (TOL32 if prec == 32 else TOL64)
+    assert rel(sw.cpu().numpy(), swref) < 1e-12
+    flag = np.zeros(w.shape, bool)
+    mref = oracle.degrid(uvw, flag, freq, gref, cell, support=16, sampling=250)
+    m = plan.degrid(g).cpu().numpy().reshape(mref.shape)
+    assert rel(m, mref) < (10 * TOL32 if prec == 32 else 1e-10)
