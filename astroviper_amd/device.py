@@ -38,6 +38,15 @@ def _ptr(t: Optional[torch.Tensor]):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _aligned(t: Optional[torch.Tensor], nbytes: int) -> Optional[torch.Tensor]:
+    """`t`, or a copy of it when its data does not start on an `nbytes`
+    boundary: the kernels read a sample's polarisations with vector loads
+    (16 bytes of weights or visibilities, 4 bytes of flags)."""
+    if t is None or t.data_ptr() % nbytes == 0:
+        return t
+    return t.clone()
+
+
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -211,6 +220,9 @@ class GridPlan:
             flag = flag.reshape(ns, n_pol)
             if flag.dtype == torch.bool:
                 flag = flag.view(torch.uint8)
+        vis = _aligned(vis, min(16, 2 * n_pol * real_dtype(code).itemsize))
+        weight = _aligned(weight, min(16, n_pol * real_dtype(code).itemsize))
+        flag = _aligned(flag, n_pol)
         shape = (self.n_img_chan, n_pol, self.n_u, self.n_v)
         if grid is None:
             grid = torch.empty(shape, dtype=complex_dtype(code), device=self.device)
@@ -251,6 +263,7 @@ class GridPlan:
             flag = flag.reshape(ns, n_pol)
             if flag.dtype == torch.bool:
                 flag = flag.view(torch.uint8)
+        flag = _aligned(flag, n_pol)
         if pol_map is not None:
             pol_map = pol_map.to(self.device, torch.int32).contiguous()
         if ns == 0:

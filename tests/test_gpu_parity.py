@@ -398,3 +398,34 @@ def test_widest_kernel_table(gpu, oracle, prec):
     mref = oracle.degrid(uvw, flag, freq, gref, cell, support=16, sampling=250)
     m = plan.degrid(g).cpu().numpy().reshape(mref.shape)
     assert rel(m, mref) < (10 * TOL32 if prec == 32 else 1e-10)
+
+
+def test_misaligned_inputs(gpu):
+    """Visibilities, weights and flags whose data start off their vector
+    boundary (a contiguous view one element into a buffer) grid and degrid
+    to the same results as aligned copies."""
+    from astroviper_amd.device import GridPlan
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, flag_frac=0.2)
+    rows, nchan = uvw.shape[0] * uvw.shape[1], freq.size
+    ns, npol = rows * nchan, vis.shape[-1]
+    plan = GridPlan(rows, nchan, nchan, n_uv, cell, device=gpu)
+    plan.set_uvw(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu), torch.from_numpy(freq).to(gpu))
+
+    def shifted(a, dtype):
+        flat = torch.empty(a.size + 1, dtype=dtype, device=gpu)
+        flat[1:] = torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).to(gpu, dtype)
+        return flat[1:].view(ns, npol)
+
+    v0 = torch.from_numpy(vis.reshape(ns, npol)).to(gpu, torch.complex64)
+    w0 = torch.from_numpy(w.reshape(ns, npol)).to(gpu, torch.float32)
+    f0 = torch.from_numpy(flag.reshape(ns, npol).astype(np.uint8)).to(gpu)
+    v1, w1, f1 = shifted(vis, torch.complex64), shifted(w, torch.float32), shifted(flag.astype(np.uint8), torch.uint8)
+    assert v1.data_ptr() % 16 and w1.data_ptr() % 16 and f1.data_ptr() % 4
+    g0, s0 = plan.grid(v0, w0, f0)
+    g1, s1 = plan.grid(v1, w1, f1)
+    m0 = plan.degrid(g0, f0)
+    m1 = plan.degrid(g0, f1)
+    torch.cuda.synchronize()
+    assert rel(g1.cpu().numpy(), g0.cpu().numpy()) < 1e-6
+    assert rel(s1.cpu().numpy(), s0.cpu().numpy()) < 1e-12
+    assert torch.equal(m1, m0)
