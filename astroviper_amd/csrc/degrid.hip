@@ -515,41 +515,74 @@ void k_degrid_mfma(
         }
         const MV zero{0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < NB; ++k) T[k] = MO::mma(a, bv[k], first_step ? zero : T[k]);
-        T1 = MO::mma(a, (R)1, first_step ? zero : T1);
+        for (int k = 0; k < NB; ++k) T[k] = MO::mma(bv[k], a, first_step ? zero : T[k]);
+        T1 = MO::mma((R)1, a, first_step ? zero : T1);
       };
       row_step(rlo >> 2, true);
       for (int s4 = (rlo >> 2) + 1; s4 <= (rhi >> 2); ++s4) row_step(s4, false);
       wave_lds_sync();
-      // ---- weighted sums over v: lane holds records MO::row(l/16, q),
-      // cell k*CPB + li/C, component li%C of every block k
+      // ---- weighted sums over v.  The products are transposed (region
+      // columns x records), so lane l holds, for its own record li = l % 16,
+      // rows MO::row(l/16, r) of every block k: region component
+      // 16 k + row, i.e. cell k*CPB + row/C, component row%C.  Every row of
+      // the ones block is the record's sum of u-weights.
+      constexpr bool kF32 = sizeof(R) == 4;
+      // the four rows of a lane lie in one cell (FP32 layout, >= 4 components)
+      constexpr bool kOneCell = kF32 && C >= 4;
+      R part[4], npart[4];
+      // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
+      // and each step of part is then exactly g times the same step of npart;
+      // the two are combined below by the same adds in the same order
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rec = MO::row(lk, q);  // record slot in the group
-        const uint32_t gq = (uint32_t)__shfl((int)geo, rec);
-        const R *wrow = wvt + rec * WS + li / C;
-        R part = (R)0, npart = (R)0;
-        // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
-        // and each step of part is then exactly g times the same step of npart
+      for (int r = 0; r < 4; ++r) {
+        const R *wrow = wvt + li * WS + MO::row(lk, r) / C;
+        R p = (R)0, q = (R)0;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
           const R wv = wrow[k * CPB];
-          part = fma(T[k][q], wv, part);
-          npart = fma(T1[q], wv, npart);
+          p = fma(T[k][r], wv, p);
+          if (!kOneCell || r == 0) q = fma(T1[0], wv, q);
         }
-        part = sum_over_dv<C, CPB>(part);
-        npart = sum_over_dv<C, CPB>(npart);
-        const uint32_t itq = (uint32_t)__shfl((int)item, rec);
-        const uint32_t fq = (uint32_t)__shfl((int)fbits, rec);
-        if (li >= 16 - C && gq != kInvalidGeo) {
-          const int c = li - (16 - C);
-          R *dst = vis_out + (size_t)itq * C + c;
-          const R val = div_by_norm(part, npart);
-          const bool flagged = (fq >> (8 * (c >> 1))) & 0xFFu;
+        part[r] = p;
+        if (kOneCell && r > 0)
+          npart[r] = npart[0];
+        else
+          npart[r] = q;
+      }
+      // rows of one component: within the lane, then across quarter-waves
+      // (FP32 rows 4 lk + r, FP64 rows lk + 4 r)
+      auto combine = [&](R *x) {
+        if constexpr ((kF32 && C == 2) || (!kF32 && C == 8)) {
+          x[0] += x[2];
+          x[1] += x[3];
+        }
+        if constexpr (!kF32 && C <= 4) x[0] = (x[0] + x[1]) + (x[2] + x[3]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if constexpr (kF32 && C <= 4) x[r] += __shfl_xor(x[r], 16);
+          if constexpr (kF32 || C == 2) x[r] += __shfl_xor(x[r], 32);
+        }
+      };
+      combine(part);
+      combine(npart);
+      // the lanes holding a record's totals, and the component of total r
+      constexpr int kVals = kF32 ? (C == 2 ? 2 : 4) : (C == 8 ? 2 : 1);
+      const bool writer = kF32 ? (C == 8 ? lk < 2 : lk == 0) : (C == 2 ? lk < 2 : true);
+      auto comp_of = [&](int r) {
+        if constexpr (kF32) return C == 8 ? 4 * lk + r : r;
+        else return C == 8 ? lk + 4 * r : lk;
+      };
+      if (writer && geo != kInvalidGeo) {
+        R *base = vis_out + (size_t)item * C;
+#pragma unroll
+        for (int r = 0; r < kVals; ++r) {
+          const int c = comp_of(r);
+          const R val = div_by_norm(part[r], npart[r]);
+          const bool flagged = (fbits >> (8 * (c >> 1))) & 0xFFu;
           if (incremental) {
-            if (!flagged) *dst += val;
+            if (!flagged) base[c] += val;
           } else {
-            *dst = flagged ? (R)0 : val;
+            base[c] = flagged ? (R)0 : val;
           }
         }
       }
