@@ -133,9 +133,12 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, cnt = unit.z;
-  const int ic = (int)(b % P.n_img_chan);
-  const int64_t tile = b / P.n_img_chan;
-  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  // 32-bit divisions: bins are numbered below 2^32 (64-bit ones cost
+  // about a hundred scalar instructions each)
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
   const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
@@ -398,9 +401,12 @@ void k_degrid_mfma(
 
   const WTab W(ov, S);
   copy_wtab(tab, W, P);
-  const int ic = (int)(b % P.n_img_chan);
-  const int64_t tile = b / P.n_img_chan;
-  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  // 32-bit divisions: bins are numbered below 2^32 (64-bit ones cost
+  // about a hundred scalar instructions each)
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
   const R *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 

@@ -159,9 +159,12 @@ __global__ __launch_bounds__(kGridThreads) void k_grid_tiles(PlanDev P, const R 
   const uint4 unit = P.gunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
-  const int ic = (int)(b % P.n_img_chan);
-  const int64_t tile = b / P.n_img_chan;
-  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  // 32-bit divisions: bins are numbered below 2^32 (64-bit ones cost
+  // about a hundred scalar instructions each)
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
   const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
   R *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
@@ -503,9 +506,12 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   const uint4 unit = P.gunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
-  const int ic = (int)(b % P.n_img_chan);
-  const int64_t tile = b / P.n_img_chan;
-  const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+  // 32-bit divisions: bins are numbered below 2^32 (64-bit ones cost
+  // about a hundred scalar instructions each)
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
   const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
   R *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
@@ -749,9 +755,10 @@ __global__ void k_zero_multi(PlanDev P, R *grid, int npol) {
     const uint4 unit = P.gunits[u];
     if (!((unit.w & kUnitMulti) && (unit.w & kUnitFirst))) continue;
     const int64_t b = unit.x;
-    const int ic = (int)(b % P.n_img_chan);
-    const int64_t tile = b / P.n_img_chan;
-    const int u0 = (int)(tile / P.ntv) * Tu, v0 = (int)(tile % P.ntv) * Tv;
+    const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+    const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+    const uint32_t tu_i = tile / (uint32_t)P.ntv;
+    const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
     for (int idx = threadIdx.x; idx < npol * cells; idx += blockDim.x) {
       int p = idx / cells, c = idx - p * cells;
       int gu = u0 + c / Tv, gv = v0 + c % Tv;

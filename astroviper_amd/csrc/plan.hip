@@ -94,6 +94,13 @@ struct RecPlan {
   int tu0, tu1, tv0, tv1;  // tiles overlapped by the grid footprint
 };
 
+__device__ inline int tile_u_of(const PlanDev &P, int x) {
+  return (int)__umulhi((uint32_t)x, P.tile_u_mul);
+}
+__device__ inline int tile_v_of(const PlanDev &P, int x) {
+  return (int)__umulhi((uint32_t)x, P.tile_v_mul);
+}
+
 __device__ inline RecPlan record_plan(const PlanDev &P, const ItemGeo &g) {
   RecPlan r;
   const int sc = P.support / 2;
@@ -103,16 +110,18 @@ __device__ inline RecPlan record_plan(const PlanDev &P, const ItemGeo &g) {
   r.tu = r.tv = r.du_t = r.dv_t = 0;
   r.tu0 = r.tu1 = r.tv0 = r.tv1 = 0;
   if (g.gvalid) {
-    r.tu = g.iu / P.tile_u;
-    r.tv = g.iv / P.tile_v;
-    r.tu0 = (g.iu - sc) / P.tile_u;
-    r.tu1 = (g.iu + P.support - sc - 1) / P.tile_u;
-    r.tv0 = (g.iv - sc) / P.tile_v;
-    r.tv1 = (g.iv + P.support - sc - 1) / P.tile_v;
+    // positions of a valid footprint are in [0, n) (n < 2^24): tile
+    // numbers by multiply-high
+    r.tu = tile_u_of(P, g.iu);
+    r.tv = tile_v_of(P, g.iv);
+    r.tu0 = tile_u_of(P, g.iu - sc);
+    r.tu1 = tile_u_of(P, g.iu + P.support - sc - 1);
+    r.tv0 = tile_v_of(P, g.iv - sc);
+    r.tv1 = tile_v_of(P, g.iv + P.support - sc - 1);
   }
   if (r.dsep) {
-    r.du_t = g.lu / P.tile_u;
-    r.dv_t = g.lv / P.tile_v;
+    r.du_t = tile_u_of(P, g.lu);
+    r.dv_t = tile_v_of(P, g.lv);
   }
   return r;
 }
@@ -224,9 +233,10 @@ __global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
        item += stride) {
     int64_t row = item;
     int chan = 0;
-    if (!P.row_chan) {
-      row = item / P.n_chan;
-      chan = (int)(item - row * P.n_chan);
+    if (!P.row_chan) {  // samples number below 2^32: a 32-bit division
+      const uint32_t r32 = (uint32_t)item / (uint32_t)P.n_chan;
+      row = r32;
+      chan = (int)((uint32_t)item - r32 * (uint32_t)P.n_chan);
     }
     ItemGeo g;
     sample_geometry(P, row, chan, g);
@@ -518,6 +528,8 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   if (tile_v == 0) tile_v = 32;
   if (tile_u < support || tile_v < support || tile_u > 128 || tile_v > 128)
     return fail(AV_ERR_INVALID, "av_plan_create: tiles must be in [support, 128]");
+  if (n_u >= (1 << 24) || n_v >= (1 << 24))
+    return fail(AV_ERR_UNSUPPORTED, "av_plan_create: grid sides must be below 2^24");
   const bool row_channels = (geometry & AV_PLAN_ROW_CHANNELS) != 0;
   geometry &= ~AV_PLAN_ROW_CHANNELS;
   if (geometry != AV_GEOM_REFERENCE && geometry != AV_GEOM_DEGRID)
@@ -546,6 +558,8 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.ov = oversampling;
   d.tile_u = tile_u;
   d.tile_v = tile_v;
+  d.tile_u_mul = 0xFFFFFFFFu / (uint32_t)tile_u + 1u;
+  d.tile_v_mul = 0xFFFFFFFFu / (uint32_t)tile_v + 1u;
   d.geometry = geometry;
   d.row_channels = row_channels;
   d.unit_cap = unit_cap;
