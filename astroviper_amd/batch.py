@@ -127,7 +127,8 @@ class BatchedGridder:
     the model visibilities are degridded from the set's own grid."""
 
     def __init__(self, sets: List[PlaneSet], n_uv, cell, oversampling: int = 100,
-                 tiles: Optional[Dict[int, Tuple[int, int]]] = None, device=None):
+                 tiles: Optional[Dict[int, Tuple[int, int]]] = None, device=None,
+                 concurrent: bool = True):
         self.device = require_gpu(device if device is not None else sets[0].uvw.device)
         self.n_uv = (int(n_uv[0]), int(n_uv[1]))
         self.cell = cell
@@ -147,17 +148,36 @@ class BatchedGridder:
                                            device=self.device))
             self.models.append(torch.empty_like(ps.vis))
         self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self.concurrent = concurrent
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in self.sets]
 
     @property
     def n_samples(self) -> int:
         return sum(ps.n_samples for ps in self.sets)
 
+    def _run_set(self, i: int) -> None:
+        ps, plan = self.sets[i], self.plans[i]
+        plan.set_uvw(ps.uvw, ps.freq, None, ps.plane)
+        plan.grid(ps.vis, ps.weight, ps.flag, grid=self.grids[i], sumwt=self.sumwts[i])
+        plan.degrid(self.grids[i], ps.flag, self.models[i])
+
     def step(self) -> None:
-        for ps, plan, g, sw, m in zip(self.sets, self.plans, self.grids, self.sumwts,
-                                      self.models):
-            plan.set_uvw(ps.uvw, ps.freq, None, ps.plane)
-            plan.grid(ps.vis, ps.weight, ps.flag, grid=g, sumwt=sw)
-            plan.degrid(g, ps.flag, m)
+        """One step.  The plane sets share nothing, so each runs on a stream
+        of its own (default priority), forked from and joined back to the
+        calling stream: under capture the graph holds them as parallel
+        branches, and the short phases of one set (scans, small kernels, the
+        tail of a launch) overlap the others' work."""
+        if not self.concurrent or len(self.sets) == 1:
+            for i in range(len(self.sets)):
+                self._run_set(i)
+            return
+        main = torch.cuda.current_stream(self.device)
+        for i, st in enumerate(self.streams):
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                self._run_set(i)
+        for st in self.streams:
+            main.wait_stream(st)
 
     def capture(self) -> torch.cuda.CUDAGraph:
         """Record one step in a HIP graph.  The step is first run eagerly on

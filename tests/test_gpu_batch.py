@@ -146,3 +146,26 @@ def test_row_order_does_not_change_results(gpu, n_uv):
     assert rel(g1, g0) < 1e-6
     assert rel(w1, w0) < 1e-12
     assert rel(m1, m0[perm.cpu().numpy()]) < 1e-6
+
+
+def test_concurrent_sets_match_sequential(gpu):
+    """The plane sets run on streams of their own; the results are those of
+    running them one after another (to FP32 rounding of the grids: record
+    order within a bin is not fixed)."""
+    from astroviper_amd.batch import BatchedGridder
+    sets, cell, _ = small_sets(gpu)
+    out = []
+    for concurrent in (False, True):
+        bg = BatchedGridder(sets, (128, 128), cell, concurrent=concurrent)
+        bg.step()
+        torch.cuda.synchronize()
+        out.append(([g.cpu().numpy() for g in bg.grids], [w.cpu().numpy() for w in bg.sumwts],
+                    [m.cpu().numpy() for m in bg.models]))
+        bg.close()
+    (g0, w0, m0), (g1, w1, m1) = out
+    for a, b in zip(g0, g1):
+        assert rel(b, a) < 1e-6
+    for a, b in zip(w0, w1):
+        assert rel(b, a) < 1e-12
+    for a, b in zip(m0, m1):
+        assert rel(b, a) < 1e-6
