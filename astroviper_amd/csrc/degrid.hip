@@ -271,12 +271,12 @@ struct DegridMfma<double> {
 };
 
 struct DegridMfmaLds {
-  int tab, reg, wvt, sgeo, sitem, scnt, total, ntab, halo, ru, rup, rv, rvp, rs;
+  int tab, reg, sgeo, sitem, scnt, total, ntab, halo, ru, rup, rv, rvp, rs;
 };
 
 // The records of a degrid unit are sorted by tile row in LDS, this many at a
 // time, so that each 16-record group covers a narrow band of region rows.
-constexpr int kSortCap = 1024;
+constexpr int kSortCap = 512;
 constexpr int kSortKeys = 136;  // rows -1 .. tile_u (<= 128) and one key for skipped records
 
 // Region rows padded to a multiple of 4 (one matrix step), columns to NB
@@ -311,8 +311,7 @@ __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support,
   L.rs = NB * 16 + 16 / (int)sizeof(R);  // elements per region row
   L.tab = 0;  // the zero-padded weight table (WTab)
   L.reg = (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
-  L.wvt = L.reg + L.rup * L.rs * (int)sizeof(R);  // per wave: [16 records][rvp + 1]
-  L.sgeo = (L.wvt + kDegridWaves * 16 * (L.rvp + 1) * (int)sizeof(R) + 15) & ~15;
+  L.sgeo = (L.reg + L.rup * L.rs * (int)sizeof(R) + 15) & ~15;
   L.sitem = L.sgeo + kSortCap * 4;
   L.scnt = L.sitem + kSortCap * 4;
   L.total = L.scnt + kSortKeys * 4;
@@ -343,7 +342,7 @@ struct DegridGroup {
 // other variants keep the compiler's choice.
 template <typename R, int NB>
 constexpr int degrid_mfma_min_blocks() {
-  return (sizeof(R) == 4 && NB <= 12) ? 4 : 1;
+  return (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
 }
 
 template <typename R, int NPOL, int NB>
@@ -369,8 +368,6 @@ void k_degrid_mfma(
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int WS = L.rvp + 1;  // odd row stride: the 4 records of a step use different banks
-  R *wvt = (R *)(smem + L.wvt) + wave * 16 * WS;
   uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
   uint32_t *s_item = (uint32_t *)(smem + L.sitem);
   uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
@@ -501,9 +498,6 @@ void k_degrid_mfma(
       }
       // every quarter-wave holds the same 16 records, so this is uniform
       if (rlo > rhi) continue;  // no record of the group has a degrid centre
-      // v-weights of record li at every region column (lanes of one record
-      // split the columns)
-      for (int j = lk; j < L.rvp; j += 4) wvt[li * WS + j] = tab[W.index(j - vcol0, ovv)];
 
       // row steps outside the band would only add zeros; the first step
       // starts the sums from zero
@@ -526,7 +520,6 @@ void k_degrid_mfma(
       };
       row_step(rlo >> 2, true);
       for (int s4 = (rlo >> 2) + 1; s4 <= (rhi >> 2); ++s4) row_step(s4, false);
-      wave_lds_sync();
       // ---- weighted sums over v.  The products are transposed (region
       // columns x records), so lane l holds, for its own record li = l % 16,
       // rows MO::row(l/16, r) of every block k: region component
@@ -539,21 +532,30 @@ void k_degrid_mfma(
       // fused multiply-adds: a constant grid g gives T[k] = g * T1 exactly,
       // and each step of part is then exactly g times the same step of npart;
       // the two are combined below by the same adds in the same order
+      // The v-weight of this lane's record at region column `cell` comes
+      // straight from the weight table; each chain runs over k in order.
+      auto wv_at = [&](int cell) { return tab[W.index(cell - vcol0, ovv)]; };
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const R *wrow = wvt + li * WS + MO::row(lk, r) / C;
-        R p = (R)0, q = (R)0;
+      for (int r = 0; r < 4; ++r) part[r] = npart[r] = (R)0;
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-          const R wv = wrow[k * CPB];
-          p = fma(T[k][r], wv, p);
-          if (!kOneCell || r == 0) q = fma(T1[0], wv, q);
+      for (int k = 0; k < NB; ++k) {
+        if constexpr (kOneCell) {
+          const R wv = wv_at(k * CPB + MO::row(lk, 0) / C);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part[r] = fma(T[k][r], wv, part[r]);
+          npart[0] = fma(T1[0], wv, npart[0]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const R wv = wv_at(k * CPB + MO::row(lk, r) / C);
+            part[r] = fma(T[k][r], wv, part[r]);
+            npart[r] = fma(T1[0], wv, npart[r]);
+          }
         }
-        part[r] = p;
-        if (kOneCell && r > 0)
-          npart[r] = npart[0];
-        else
-          npart[r] = q;
+      }
+      if constexpr (kOneCell) {
+#pragma unroll
+        for (int r = 1; r < 4; ++r) npart[r] = npart[0];
       }
       // rows of one component: within the lane, then across quarter-waves
       // (FP32 rows 4 lk + r, FP64 rows lk + 4 r)
@@ -592,7 +594,6 @@ void k_degrid_mfma(
           }
         }
       }
-      wave_lds_sync();  // the next group rewrites this wavefront's weights
     }
     __syncthreads();  // the next pass rewrites the sorted records
   }
