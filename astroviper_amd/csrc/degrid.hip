@@ -338,8 +338,8 @@ struct DegridGroup {
 // AV_DEGRID_INCREMENTAL); the finishing pass zeroes the samples no unit
 // visits.
 // Workgroups per CU the FP32 kernels with up to 12 column blocks are compiled
-// for (at most 128 registers without spills, and four regions fit in LDS);
-// other variants keep the compiler's choice.
+// for: five (96 registers without spills; a 16 x 16 tile's workgroup needs
+// 27 KB of LDS); six would spill.  Other variants keep the compiler's choice.
 template <typename R, int NB>
 constexpr int degrid_mfma_min_blocks() {
   return (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
