@@ -427,8 +427,8 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support, int tile
   L.nrm = o;
   o += (L.nnrm * (int)sizeof(double) + 15) & ~15;
   L.st_d = o;                           // per wave: [64][2*NPOL] R
-  L.st_geo = 64 * 2 * NPOL * (int)sizeof(R);  // offset inside the wave region: [64] uint2
-  L.wave_bytes = L.st_geo + 64 * 8;
+  L.st_geo = 64 * 2 * NPOL * (int)sizeof(R);  // offset inside the wave region: [64] u32
+  L.wave_bytes = L.st_geo + 64 * 4;
   L.total = o + kGridMaxWaves * L.wave_bytes;
   return L;
 }
@@ -459,12 +459,12 @@ __device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P
   }
 }
 
-// Workgroups per CU the FP32 4-polarisation 16 x 16 kernels are compiled for
-// (94 registers, no spills: five wavefronts per SIMD); other variants keep
-// the compiler's choice.
+// Workgroups per CU the FP32 kernels with up to 8 result blocks are compiled
+// for: six (80 registers, no spills; a 16 x 16 tile's workgroup needs 26 KB
+// of LDS).  Other variants keep the compiler's choice.
 template <typename R, int NPOL, int MB, int VB, bool COMP>
 constexpr int grid_mfma_min_blocks() {
-  return (sizeof(R) == 4 && !COMP && MB * 2 * NPOL * VB <= 8) ? 5 : 1;
+  return (sizeof(R) == 4 && !COMP && MB * 2 * NPOL * VB <= 8) ? 6 : 1;
 }
 
 template <typename R, int NPOL, int MB, int VB, bool COMP, bool PSF>
@@ -485,7 +485,9 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   char *wbase = smem + L.st_d + wave * L.wave_bytes;
   R *st_d = (R *)wbase;                           // [record][C]
-  uint2 *st_geo = (uint2 *)(wbase + L.st_geo);  // [record] table bases of the u and v taps
+  // [record] table bases of the u and v taps, 16 bits each (a padded table
+  // has at most (16 + 2 * 64) rows x 255 offsets < 2^16 entries)
+  uint32_t *st_geo = (uint32_t *)(wbase + L.st_geo);
 
   constexpr bool psf = PSF;  // PSF data are real: the imaginary blocks are skipped
   const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
@@ -499,7 +501,8 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
   copy_wtab_padded(tab, L, P);
   for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
   __syncthreads();  // the only workgroup barrier: tables ready
-  const uint32_t u = blk * kGridMaxWaves + wave;
+  // the same for every lane: scalar registers for the unit and its bounds
+  const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blk * kGridMaxWaves + wave));
   if (u >= n_units) return;
 
   const int Tu = 16 * MB, Tv = 16 * VB;
@@ -628,12 +631,12 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
       // table index of tap (li - position + sc) at this record's offsets is
       // base + li * nw (taps -pad .. are rows 0 ..); a skipped record has
       // zero data, so any in-range base (0) serves it
-      uint2 tb = make_uint2(0u, 0u);
+      uint32_t tb = 0u;
       if (geo != kInvalidGeo) {
         int au, ou, avv, ovv;
         unpack_geo(geo, au, ou, avv, ovv);
-        tb.x = (uint32_t)((sc - au + L.pad) * L.nw + ou + L.nw / 2);
-        tb.y = (uint32_t)((sc - avv + L.pad) * L.nw + ovv + L.nw / 2);
+        tb = (uint32_t)((sc - au + L.pad) * L.nw + ou + L.nw / 2) |
+             ((uint32_t)((sc - avv + L.pad) * L.nw + ovv + L.nw / 2) << 16);
       }
       st_geo[lane] = tb;
     }
@@ -647,12 +650,13 @@ void k_grid_mfma(PlanDev P, const R *__restrict__ vis,
     const int li_nw = li * L.nw;
     for (int s4 = 0; s4 < nst; s4 += 4) {
       const int rec = s4 + lk;
-      const uint2 tb = st_geo[rec];
+      const uint32_t tb = st_geo[rec];
+      const uint32_t tbu = tb & 0xFFFFu, tbv = tb >> 16;
       R a[MB], wv[VB];
 #pragma unroll
-      for (int m = 0; m < MB; ++m) a[m] = tab[tb.x + li_nw + 16 * m * L.nw];
+      for (int m = 0; m < MB; ++m) a[m] = tab[tbu + li_nw + 16 * m * L.nw];
 #pragma unroll
-      for (int vb = 0; vb < VB; ++vb) wv[vb] = tab[tb.y + li_nw + 16 * vb * L.nw];
+      for (int vb = 0; vb < VB; ++vb) wv[vb] = tab[tbv + li_nw + 16 * vb * L.nw];
       R dd[C];
 #pragma unroll
       for (int k = 0; k < C; k += CellLayout<R, NPOL>::kCC) {
