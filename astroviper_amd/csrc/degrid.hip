@@ -367,7 +367,8 @@ void k_degrid_mfma(
   const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(ov, S, Tu, Tv);
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar group loop
   uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
   uint32_t *s_item = (uint32_t *)(smem + L.sitem);
   uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
