@@ -750,28 +750,28 @@ __global__ __launch_bounds__(256) void k_sumwt_reduce(PlanDev P, int npol, doubl
   }
 }
 
-// zero the tiles of bins that several units add into (overwrite mode)
+// zero the tiles of bins that several units add into (overwrite mode): one
+// thread per unit reads the unit list in one coalesced pass; the thread of a
+// split bin's first unit clears that tile (such bins are few: those with more
+// records than a unit holds)
 template <typename R>
-__global__ void k_zero_multi(PlanDev P, R *grid, int npol) {
+__global__ __launch_bounds__(256) void k_zero_multi(PlanDev P, R *grid, int npol) {
   const uint32_t n_units = P.gunit_start[P.n_bins];
-  const int Tu = P.tile_u, Tv = P.tile_v, cells = Tu * Tv;
-  for (uint32_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+  const int Tu = P.tile_u, Tv = P.tile_v;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_units;
+       u += gridDim.x * blockDim.x) {
     const uint4 unit = P.gunits[u];
     if (!((unit.w & kUnitMulti) && (unit.w & kUnitFirst))) continue;
-    const int64_t b = unit.x;
-    const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
-    const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+    const uint32_t tile = unit.x / (uint32_t)P.n_img_chan;
+    const int ic = (int)(unit.x - tile * (uint32_t)P.n_img_chan);
     const uint32_t tu_i = tile / (uint32_t)P.ntv;
     const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
-    for (int idx = threadIdx.x; idx < npol * cells; idx += blockDim.x) {
-      int p = idx / cells, c = idx - p * cells;
-      int gu = u0 + c / Tv, gv = v0 + c % Tv;
-      if (gu < P.n_u && gv < P.n_v) {
-        R *dst = grid + (((size_t)ic * npol + p) * P.n_u + gu) * P.n_v * 2 + (size_t)gv * 2;
-        dst[0] = (R)0;
-        dst[1] = (R)0;
+    const int u1 = min(u0 + Tu, P.n_u), v1 = min(v0 + Tv, P.n_v);
+    for (int p = 0; p < npol; ++p)
+      for (int gu = u0; gu < u1; ++gu) {
+        Cplx<R> *row = (Cplx<R> *)(grid + (((size_t)ic * npol + p) * P.n_u + gu) * P.n_v * 2);
+        for (int gv = v0; gv < v1; ++gv) row[gv] = Cplx<R>{(R)0, (R)0};
       }
-    }
   }
 }
 
@@ -931,7 +931,8 @@ static int grid_by_pol(av_plan *plan, const void *vis, const void *weight, const
     int64_t n = (int64_t)d.n_img_chan * npol;
     hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sumwt, n);
     AV_LAUNCH_CHECK("k_zero_f64");
-    hipLaunchKernelGGL(k_zero_multi<R>, dim3(1024), dim3(256), 0, s, d, (R *)grid, npol);
+    const unsigned zb = (unsigned)std::min<int64_t>((d.max_gunits + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero_multi<R>, dim3(zb), dim3(256), 0, s, d, (R *)grid, npol);
     AV_LAUNCH_CHECK("k_zero_multi");
   }
   switch (npol) {
