@@ -165,6 +165,10 @@ struct PlanDev {
   int32_t chunked;                  // chunked binning possible for this plan
   int32_t rows_per_chunk, chunks_per_chan;
   uint32_t *chunk_hist;             // [n_chan * chunks_per_chan][ntu * ntv]
+  // per chunk and tile, the first slot of the chunk's duplicates (the first
+  // slot of its primaries replaces the counts in chunk_hist)
+  uint32_t *chunk_dup;              // [n_chan * chunks_per_chan][ntu * ntv]
+  int32_t chunk_scatter;            // records written through per-chunk LDS cursors
   // row-block binning (rows carry their own channel): kRowBlockRows rows per
   // workgroup, per-tile counters and cursors in LDS
   int32_t row_blocks;

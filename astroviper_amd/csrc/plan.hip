@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "avgrid_internal.h"
+#include "tile_common.h"
 
 namespace av {
 
@@ -300,6 +301,77 @@ __global__ void k_chunk_totals(PlanDev P) {
   const int64_t b = (int64_t)t * P.n_img_chan + ic;
   P.cnt_prim[b] = sp;
   P.cnt_all[b] = sp + sd;
+}
+
+// Chunked pass 3 (after the bin offsets): the first slot of every chunk's
+// run in every bin.  The chunks of a bin take their runs in chunk order, so
+// the records of a bin are in sample order from chunk to chunk.  The first
+// primary slot overwrites the counts; the first duplicate slot goes to
+// chunk_dup.
+__global__ void k_chunk_bases(PlanDev P) {
+  const int ntiles = P.ntu * P.ntv;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)P.n_img_chan * ntiles) return;
+  const int ic = (int)(id / ntiles);
+  const int t = (int)(id - (int64_t)ic * ntiles);
+  const int64_t b = (int64_t)t * P.n_img_chan + ic;
+  uint32_t sp = P.bin_start[b];
+  uint32_t sd = sp + P.cnt_prim[b];
+  for (int c = 0; c < P.n_chan; ++c) {
+    if (P.chan_map[c] != ic) continue;
+    for (int k = 0; k < P.chunks_per_chan; ++k) {
+      const int64_t i = ((int64_t)c * P.chunks_per_chan + k) * ntiles + t;
+      const uint32_t hv = P.chunk_hist[i];
+      P.chunk_hist[i] = sp;
+      P.chunk_dup[i] = sd;
+      sp += hv & 0xFFFFu;
+      sd += hv >> 16;
+    }
+  }
+}
+
+// Chunked pass 4: a workgroup per chunk writes the chunk's records.  Its
+// runs' first slots are loaded into LDS cursors, and every record takes its
+// slot from an LDS atomic: no global atomics.  XCD-grouped numbering puts
+// kScatterXcdGroup consecutive chunks of a channel on one XCD, so the runs
+// they write side by side in a bin meet in one L2.
+constexpr int kScatterXcdGroup = 16;
+__global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P, int64_t n_chunks) {
+  extern __shared__ uint32_t lds[];
+  const int ntiles = P.ntu * P.ntv;
+  uint32_t *cp = lds, *cd = lds + ntiles;  // [n_tiles] each
+  const int64_t w = xcd_grouped_block(blockIdx.x, kScatterXcdGroup);
+  if (w >= n_chunks) return;  // padding of the grouped grid: the whole workgroup
+  const int chan = (int)(w / P.chunks_per_chan);
+  const int64_t r0 = (w - (int64_t)chan * P.chunks_per_chan) * P.rows_per_chunk;
+  const int64_t r1 = min(r0 + (int64_t)P.rows_per_chunk, P.n_rows);
+  const uint32_t *bp = P.chunk_hist + w * ntiles, *bd = P.chunk_dup + w * ntiles;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+    cp[i] = bp[i];
+    cd[i] = bd[i];
+  }
+  __syncthreads();
+  for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+    ItemGeo g;
+    sample_geometry(P, row, chan, g);
+    if (!(g.gvalid || g.dvalid)) continue;
+    const RecPlan rp = record_plan(P, g);
+    const int64_t item = row * P.n_chan + chan;
+    if (rp.gprim) {
+      const uint32_t slot = atomicAdd(&cp[rp.tu * P.ntv + rp.tv], 1u);
+      write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
+      for (int a = rp.tu0; a <= rp.tu1; ++a)
+        for (int b = rp.tv0; b <= rp.tv1; ++b)
+          if (a != rp.tu || b != rp.tv) {
+            const uint32_t s2 = atomicAdd(&cd[a * P.ntv + b], 1u);
+            write_records(P, g, rp, item, s2, a, b, false);
+          }
+    }
+    if (rp.dsep) {
+      const uint32_t slot = atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
+      write_degrid_record(P, g, rp, item, slot);
+    }
+  }
 }
 
 // ---- row-block binning (rows carry their own channel) ---------------------
@@ -601,7 +673,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_rec = take(8 * d.max_records), o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
          o_dun = take(sizeof(uint4) * d.max_dunits),
          o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
-         o_chh = take(4 * chunk_cells),
+         o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells),
          o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
          o_nrm = take(sizeof(double) * (2 * (oversampling / 2 + 2) + 1)),
          o_wt32 = take(sizeof(float) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
@@ -637,6 +709,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.dmask = (unsigned long long *)(c + o_dm);
   d.n_words = mask_words - 1;
   d.chunk_hist = chunk_cells ? (uint32_t *)(c + o_chh) : nullptr;
+  d.chunk_dup = chunk_cells ? (uint32_t *)(c + o_chd) : nullptr;
   d.tab32 = (float *)(c + o_tab);
   d.nrm = (double *)(c + o_nrm);
   d.wtab32 = (float *)(c + o_wt32);
@@ -678,11 +751,15 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   // row-block binning needs a counter and a cursor per tile in LDS
   const int64_t ntiles = (int64_t)d.ntu * d.ntv;
   d.row_blocks = 0;
-  if (row_chan && d.n_rows > 0) {
+  d.chunk_scatter = 0;
+  if (d.n_rows > 0 && (row_chan || d.chunked)) {
     const DeviceProps &prop = device_props();
     if (prop.error) return hip_fail(prop.error, "av_plan_set_uvw: device properties");
-    d.row_blocks = (size_t)(2 * sizeof(uint32_t) * ntiles) <= prop.lds_per_block;
+    const bool fits = (size_t)(2 * sizeof(uint32_t) * ntiles) <= prop.lds_per_block;
+    if (row_chan) d.row_blocks = fits;
+    else d.chunk_scatter = fits;
   }
+  const int64_t nchunks = (int64_t)d.n_chan * d.chunks_per_chan;
   const unsigned row_grid = (unsigned)((d.n_rows + kRowBlockRows - 1) / kRowBlockRows);
   // pass 1: counts per bin
   if (d.row_blocks) {
@@ -694,7 +771,6 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
     hipLaunchKernelGGL(k_count_rows, dim3(row_grid), dim3(kChunkThreads), lds, s, d);
     AV_LAUNCH_CHECK("k_count_rows");
   } else if (d.chunked && !row_chan) {
-    const int64_t nchunks = (int64_t)d.n_chan * d.chunks_per_chan;
     const int lds = (int)(sizeof(uint32_t) * d.ntu * d.ntv);
     if ((rc = ensure_lds_limit<k_count_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
       return rc;
@@ -715,15 +791,28 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   }
   // pass 2: bin offsets, cursors, records
   if ((rc = scan_exclusive_u32(d.cnt_all, d.bin_start, d.n_bins, d.scan_tmp, s))) return rc;
-  hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
-  AV_LAUNCH_CHECK("k_cursors");
-  if (d.row_blocks) {
+  if (d.chunk_scatter) {
+    const int64_t pairs = (int64_t)d.n_img_chan * d.ntu * d.ntv;
+    hipLaunchKernelGGL(k_chunk_bases, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_chunk_bases");
+    const int lds = (int)(2 * sizeof(uint32_t) * ntiles);
+    if ((rc = ensure_lds_limit<k_scatter_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
+      return rc;
+    const int64_t grid = xcd_grouped_grid(nchunks, kScatterXcdGroup);
+    hipLaunchKernelGGL(k_scatter_chunks, dim3((unsigned)grid), dim3(kChunkThreads), lds, s, d,
+                       nchunks);
+    AV_LAUNCH_CHECK("k_scatter_chunks");
+  } else if (d.row_blocks) {
+    hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_cursors");
     const int lds = (int)(2 * sizeof(uint32_t) * ntiles);
     if ((rc = ensure_lds_limit<k_scatter_rows>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
       return rc;
     hipLaunchKernelGGL(k_scatter_rows, dim3(row_grid), dim3(kChunkThreads), lds, s, d);
     AV_LAUNCH_CHECK("k_scatter_rows");
   } else {
+    hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_cursors");
     int64_t blocks = (d.n_items + 255) / 256;
     blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 16384));
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, s, d);
