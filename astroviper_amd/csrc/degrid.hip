@@ -277,7 +277,15 @@ struct DegridMfmaLds {
 // The records of a degrid unit are sorted by tile row in LDS, this many at a
 // time, so that each 16-record group covers a narrow band of region rows.
 constexpr int kSortCap = 512;
-constexpr int kSortKeys = 136;  // rows -1 .. tile_u (<= 128) and one key for skipped records
+// Sort keys: tile rows -1 .. tile_u, once per v-half when the records are
+// split by v, and one key for records without degrid geometry.  The scan
+// takes three keys per lane of one wavefront, and two words after the keys
+// hold the start of the second half and the sorted count (with padding).
+template <int NB, int NBW>
+__host__ __device__ constexpr int degrid_sort_keys(int tu) {
+  return (NBW < NB ? 2 : 1) * (tu + 2) + 1;
+}
+constexpr int kMaxSortKeys = 3 * 64;
 
 // Region rows padded to a multiple of 4 (one matrix step), columns to NB
 // blocks of 16 components.  Within a row, component n of the region is
@@ -298,7 +306,7 @@ __device__ inline float div_by_norm(float a, float b) {
 }
 __device__ inline double div_by_norm(double a, double b) { return a / b; }
 
-template <typename R, int NPOL, int NB>
+template <typename R, int NPOL, int NB, int NBW = NB>
 __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support, int tu, int tv) {
   DegridMfmaLds L;
   const int sc = support / 2;
@@ -312,15 +320,34 @@ __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support,
   L.tab = 0;  // the zero-padded weight table (WTab)
   L.reg = (WTab(ov, support).size() * (int)sizeof(R) + 15) & ~15;
   L.sgeo = (L.reg + L.rup * L.rs * (int)sizeof(R) + 15) & ~15;
-  L.sitem = L.sgeo + kSortCap * 4;
-  L.scnt = L.sitem + kSortCap * 4;
-  L.total = L.scnt + kSortKeys * 4;
+  // sorted records: up to 15 padding slots before the second v-half
+  L.sitem = L.sgeo + (kSortCap + 16) * 4;
+  L.scnt = L.sitem + (kSortCap + 16) * 4;
+  L.total = L.scnt + (degrid_sort_keys<NB, NBW>(tu) + 2) * 4;
   return L;
 }
 
 template <int NPOL>
 __host__ __device__ constexpr int degrid_mfma_blocks_needed(int tv, int support) {
   return ((tv + 2 * (support / 2 + 1)) * 2 * NPOL + 15) / 16;
+}
+
+// Records split by v into two halves: a record whose degrid centre is in
+// tile columns av < tv/2 goes to the first, the rest to the second, and each
+// half's groups compute NBW of the NB column blocks: the first half blocks
+// [0, NBW), the second [NB - NBW, NB).  A centre lies in columns -1 .. tv
+// (a grid-primary record carries a degrid centre within one pixel of its
+// grid centre), and its footprint covers region columns av + 1 .. av + S.
+// The split is used when both windows hold every footprint of their half,
+// the second window starts on a 16-byte boundary of the region rows, and
+// the keys fit the scan.
+template <typename R, int NPOL, int NB, int NBW>
+__host__ __device__ constexpr bool degrid_halves_fit(int tu, int tv, int support) {
+  constexpr int CPB = 16 / (2 * NPOL);
+  constexpr int VL = 16 / (int)sizeof(R);
+  return NBW < NB && (NB - NBW) % VL == 0 && tv / 2 + support <= NBW * CPB &&
+         tv / 2 + 1 >= (NB - NBW) * CPB && tv + support <= NB * CPB - 1 &&
+         degrid_sort_keys<NB, NBW>(tu) + 2 <= kMaxSortKeys;
 }
 
 // The 16 records of one group (lane l holds record l % 16).
@@ -333,20 +360,25 @@ struct DegridGroup {
 // are in flight together; the records are then sorted by tile row in LDS,
 // so that the 16 records of a group span a narrow band of region rows and
 // only the row steps of that band are run (the others would add exact
-// zeros).  A record's flags are read with its group and applied as its
+// zeros).  With v-halves (NBW < NB) the sort key puts the records of each
+// half together, each half starting on a group boundary, and a group
+// computes only its half's NBW column blocks: the blocks outside the window
+// hold no tap of the group's records.  A record's flags are read with its group and applied as its
 // model is written: a flagged polarisation is set to zero (left alone when
 // AV_DEGRID_INCREMENTAL); the finishing pass zeroes the samples no unit
 // visits.
-// Workgroups per CU the FP32 kernels with up to 12 column blocks are compiled
-// for: five (96 registers without spills; a 16 x 16 tile's workgroup needs
-// 27 KB of LDS); six would spill.  Other variants keep the compiler's choice.
+// Workgroups per CU the FP32 kernels are compiled for, by the column blocks a
+// group computes: six for up to 8 (80 registers without spills; a 16 x 16
+// tile's workgroup with v-halves needs 26,988 bytes of LDS, a sixth of the
+// CU's 160 KB); five for up to 12 (96 registers; six would spill).  Other
+// variants keep the compiler's choice.
 template <typename R, int NB>
 constexpr int degrid_mfma_min_blocks() {
-  return (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
+  return (sizeof(R) == 4 && NB <= 8) ? 6 : (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
 }
 
-template <typename R, int NPOL, int NB>
-__global__ __launch_bounds__(kDegridThreads, (degrid_mfma_min_blocks<R, NB>()))
+template <typename R, int NPOL, int NB, int NBW>
+__global__ __launch_bounds__(kDegridThreads, (degrid_mfma_min_blocks<R, NBW>()))
 void k_degrid_mfma(
     PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
     const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
@@ -357,6 +389,10 @@ void k_degrid_mfma(
   constexpr int CPB = 16 / C;               // cells per 16-column block
   constexpr int VL = 16 / (int)sizeof(R);   // elements per 16-byte read
   constexpr int HK = kSortCap / kDegridThreads;  // records per thread in a sort pass
+  // v-halves: each group computes the NBW column blocks of its half's window
+  constexpr bool kHalves = NBW < NB;
+  constexpr int KB1 = NB - NBW;  // first block of the second half's window
+  static_assert(KB1 % VL == 0, "the second window must start on a 16-byte boundary");
   extern __shared__ __align__(16) char smem[];
   // groups of kDegridXcdGroup consecutive units (neighbouring channels of a
   // tile) run under one L2
@@ -364,7 +400,7 @@ void k_degrid_mfma(
   if (u >= P.dunit_start[P.n_bins]) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
-  const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(ov, S, Tu, Tv);
+  const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB, NBW>(ov, S, Tu, Tv);
   R *tab = (R *)(smem + L.tab);
   R *reg = (R *)(smem + L.reg);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -375,16 +411,23 @@ void k_degrid_mfma(
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
   const int Hh = L.halo, RS = L.rs;
   const int li = lane & 15, lk = lane >> 4;
-  const int nkeys = Tu + 3;
+  const int rkeys = Tu + 2;  // row keys of one half
+  const int nkeys = degrid_sort_keys<NB, NBW>(Tu);
+  const int half_v = Tv / 2;
 
   const uint4 unit = P.dunits[u];
   const int64_t b = unit.x;
   const uint32_t first = unit.y, end = unit.y + unit.z;
-  // sort key: tile row + 1 of the degrid centre (rows -1 .. Tu), Tu + 2 for
-  // a record without degrid geometry
+  // sort key: tile row + 1 of the degrid centre (rows -1 .. Tu), plus rkeys
+  // in the second v-half; the last key for a record without degrid geometry
   auto key_of = [&](uint32_t g) {
     const int k = (int)(g & 0xFFu) - kPosBias + 1;
-    return g == kInvalidGeo ? Tu + 2 : (k < 0 ? 0 : (k > Tu + 1 ? Tu + 1 : k));
+    int key = k < 0 ? 0 : (k > Tu + 1 ? Tu + 1 : k);
+    if constexpr (kHalves) {
+      const int av = (int)((g >> 16) & 0xFFu) - kPosBias;
+      key += av >= half_v ? rkeys : 0;
+    }
+    return g == kInvalidGeo ? nkeys - 1 : key;
   };
   // records of the first sort pass: in flight during the region load
   // (unconditional loads: a thread past the end rereads the last record and
@@ -429,7 +472,7 @@ void k_degrid_mfma(
       }
     }
     // ---- counting sort of the pass's records by key, in LDS
-    for (int i = tid; i < kSortKeys; i += kDegridThreads) s_cnt[i] = 0u;
+    for (int i = tid; i < nkeys; i += kDegridThreads) s_cnt[i] = 0u;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < HK; ++k)
@@ -449,14 +492,35 @@ void k_degrid_mfma(
         if (lane >= off) incl += o;
       }
       uint32_t run = incl - t;
+      // the second half starts on a group boundary: its keys move up by the
+      // padding, and the padding slots hold no record
+      uint32_t pad = 0, b1 = 0;
+      if constexpr (kHalves) {
+        uint32_t h0 = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) h0 += 3 * lane + j < rkeys ? c3[j] : 0u;
+        for (int off = 32; off > 0; off >>= 1) h0 += __shfl_xor(h0, off);
+        b1 = (h0 + 15u) & ~15u;
+        pad = b1 - h0;
+        if ((uint32_t)lane < pad) {
+          s_geo[h0 + lane] = kInvalidGeo;
+          s_item[h0 + lane] = 0u;
+        }
+      }
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int kk = 3 * lane + j;
-        if (kk < nkeys) s_cnt[kk] = run;
+        if (kk < nkeys) s_cnt[kk] = run + (kk >= rkeys ? pad : 0u);
         run += c3[j];
+      }
+      if (lane == 0) {
+        s_cnt[nkeys] = b1;
+        s_cnt[nkeys + 1] = (uint32_t)n + pad;
       }
     }
     __syncthreads();
+    const int b1 = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys]);
+    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys + 1]);
 #pragma unroll
     for (int k = 0; k < HK; ++k)
       if (tid + k * kDegridThreads < n) {
@@ -467,11 +531,13 @@ void k_degrid_mfma(
     __syncthreads();
 
     // ---- 16-record groups of the sorted pass
-    const int ngroups = (n + 15) / 16;
+    const int ngroups = (np + 15) / 16;
     for (int g = wave; g < ngroups; g += kDegridWaves) {
       const int r = g * 16 + li;
-      const uint32_t geo = r < n ? s_geo[r] : kInvalidGeo;
-      const uint32_t item = r < n ? s_item[r] : 0u;
+      const uint32_t geo = r < np ? s_geo[r] : kInvalidGeo;
+      const uint32_t item = r < np ? s_item[r] : 0u;
+      // first column block of the group's window
+      const int kb0 = (kHalves && g * 16 >= b1) ? KB1 : 0;
       // the record's flags, used when its model is written (a flagged
       // polarisation is left alone when incremental, set to zero otherwise).
       // The load is unconditional -- a lane without a record reads its slot's
@@ -502,21 +568,21 @@ void k_degrid_mfma(
 
       // row steps outside the band would only add zeros; the first step
       // starts the sums from zero
-      MV T[NB], T1;
+      MV T[NBW], T1;
       auto row_step = [&](int s4, bool first_step) {
         const int row = 4 * s4 + lk;
         const R a = tab[W.index(row - urow0, ou)];
-        const R *rp = reg + row * RS + li * NB;
-        R bv[NB];
+        const R *rp = reg + row * RS + li * NB + kb0;
+        R bv[NBW];
 #pragma unroll
-        for (int k = 0; k < NB; k += VL) {
+        for (int k = 0; k < NBW; k += VL) {
           const Vec<R, VL> t = *(const Vec<R, VL> *)(rp + k);
 #pragma unroll
           for (int e = 0; e < VL; ++e) bv[k + e] = t.v[e];
         }
         const MV zero{0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < NB; ++k) T[k] = MO::mma(bv[k], a, first_step ? zero : T[k]);
+        for (int k = 0; k < NBW; ++k) T[k] = MO::mma(bv[k], a, first_step ? zero : T[k]);
         T1 = MO::mma((R)1, a, first_step ? zero : T1);
       };
       row_step(rlo >> 2, true);
@@ -535,11 +601,13 @@ void k_degrid_mfma(
       // the two are combined below by the same adds in the same order
       // The v-weight of this lane's record at region column `cell` comes
       // straight from the weight table; each chain runs over k in order.
-      auto wv_at = [&](int cell) { return tab[W.index(cell - vcol0, ovv)]; };
+      // (window block k is region block kb0 + k)
+      const int vcolw = vcol0 - kb0 * CPB;
+      auto wv_at = [&](int cell) { return tab[W.index(cell - vcolw, ovv)]; };
 #pragma unroll
       for (int r = 0; r < 4; ++r) part[r] = npart[r] = (R)0;
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
+      for (int k = 0; k < NBW; ++k) {
         if constexpr (kOneCell) {
           const R wv = wv_at(k * CPB + MO::row(lk, 0) / C);
 #pragma unroll
@@ -722,12 +790,13 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
   return AV_OK;
 }
 
-template <typename R, int NPOL, int NB>
+template <typename R, int NPOL, int NB, int NBW = NB>
 static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
                               const uint8_t *flag, const double *cgk, void *vis_out, int mode,
                               hipStream_t s) {
   PlanDev &d = plan->d;
-  const DegridMfmaLds L = degrid_mfma_layout<R, NPOL, NB>(d.ov, d.support, d.tile_u, d.tile_v);
+  const DegridMfmaLds L =
+      degrid_mfma_layout<R, NPOL, NB, NBW>(d.ov, d.support, d.tile_u, d.tile_v);
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
   if ((size_t)L.total > prop.lds_per_block)
@@ -735,12 +804,12 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
                                         std::to_string(L.total) + " bytes)");
   {
     const int rc_lds =
-        ensure_lds_limit<k_degrid_mfma<R, NPOL, NB>>(L.total, "av_degrid: hipFuncSetAttribute");
+        ensure_lds_limit<k_degrid_mfma<R, NPOL, NB, NBW>>(L.total, "av_degrid: hipFuncSetAttribute");
     if (rc_lds) return rc_lds;
   }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_degrid_mfma<R, NPOL, NB>),
+  hipLaunchKernelGGL((k_degrid_mfma<R, NPOL, NB, NBW>),
                      dim3((unsigned)xcd_grouped_grid(d.max_dunits, kDegridXcdGroup)),
                      dim3(kDegridThreads), L.total, s, d, (const R *)grid, npol_g, pol_map, flag,
                      cgk, (R *)vis_out, mode);
@@ -774,8 +843,15 @@ static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const in
     return launch_degrid_mfma<R, NPOL, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   if (nb <= 8)
     return launch_degrid_mfma<R, NPOL, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
-  if (nb <= 12)
+  if (nb <= 12) {
+    // FP32, four polarisations: records split by v, eight of the twelve blocks
+    if constexpr (sizeof(R) == 4 && NPOL == 4)
+      if (nb == 12 && degrid_halves_fit<R, NPOL, 12, 8>(plan->d.tile_u, plan->d.tile_v,
+                                                         plan->d.support))
+        return launch_degrid_mfma<R, NPOL, 12, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,
+                                                  mode, s);
     return launch_degrid_mfma<R, NPOL, 12>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
+  }
   if (nb <= 16)
     return launch_degrid_mfma<R, NPOL, 16>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   if (plan->d.support <= 8)
