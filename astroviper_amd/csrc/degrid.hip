@@ -367,18 +367,18 @@ struct DegridGroup {
 // model is written: a flagged polarisation is set to zero (left alone when
 // AV_DEGRID_INCREMENTAL); the finishing pass zeroes the samples no unit
 // visits.
-// Workgroups per CU the FP32 kernels are compiled for, by the column blocks a
-// group computes: six for up to 8 (80 registers without spills; a 16 x 16
-// tile's workgroup with v-halves needs 26,988 bytes of LDS, a sixth of the
-// CU's 160 KB); five for up to 12 (96 registers; six would spill).  Other
-// variants keep the compiler's choice.
-template <typename R, int NB>
+// Workgroups per CU the FP32 kernels are compiled for: six with v-halves of
+// at most 8 column blocks (80 registers without spills; a 16 x 16 tile's
+// workgroup then needs 26,988 bytes of LDS, a sixth of the CU's 160 KB);
+// five for up to 12 blocks without halves (96 registers; at six the
+// one-polarisation kernels spill).  Other variants keep the compiler's choice.
+template <typename R, int NB, int NBW>
 constexpr int degrid_mfma_min_blocks() {
-  return (sizeof(R) == 4 && NB <= 8) ? 6 : (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
+  return (sizeof(R) == 4 && NBW < NB && NBW <= 8) ? 6 : (sizeof(R) == 4 && NB <= 12) ? 5 : 1;
 }
 
 template <typename R, int NPOL, int NB, int NBW>
-__global__ __launch_bounds__(kDegridThreads, (degrid_mfma_min_blocks<R, NBW>()))
+__global__ __launch_bounds__(kDegridThreads, (degrid_mfma_min_blocks<R, NB, NBW>()))
 void k_degrid_mfma(
     PlanDev P, const R *__restrict__ grid, int npol_g, const int32_t *__restrict__ pol_map,
     const uint8_t *__restrict__ flag, const double *__restrict__ cgk, R *__restrict__ vis_out,
@@ -566,12 +566,24 @@ void k_degrid_mfma(
       // every quarter-wave holds the same 16 records, so this is uniform
       if (rlo > rhi) continue;  // no record of the group has a degrid centre
 
+      // Weight-table reads without a multiply or a clamp pair per read:
+      // WTab::index(t, off) = (clamp(t, -1, S) + 1) * nw + off + half, and
+      // with nw > 0, (clamp(t, -1, S) + 1) * nw = clamp((t + 1) * nw, 0,
+      // (S + 1) * nw) -- one median instruction -- where (t + 1) * nw moves
+      // by a fixed step from one row step or column block to the next.
+      // (The empty asm statements keep a per-group base opaque, so that the
+      // compiler does not fold base + step back into one multiply per read.)
+      const int nw = W.nw, tcap = (S + 1) * nw;
+      auto tap_read = [&](int x, int offh) { return tab[min(max(x, 0), tcap) + offh]; };
+      const int uofs = ou + W.half;
+      const int ub0 = (lk - urow0 + 1) * nw;
+
       // row steps outside the band would only add zeros; the first step
       // starts the sums from zero
       MV T[NBW], T1;
       auto row_step = [&](int s4, bool first_step) {
         const int row = 4 * s4 + lk;
-        const R a = tab[W.index(row - urow0, ou)];
+        const R a = tap_read(ub0 + s4 * (4 * nw), uofs);  // u-tap row - urow0
         const R *rp = reg + row * RS + li * NB + kb0;
         R bv[NBW];
 #pragma unroll
@@ -603,20 +615,29 @@ void k_degrid_mfma(
       // straight from the weight table; each chain runs over k in order.
       // (window block k is region block kb0 + k)
       const int vcolw = vcol0 - kb0 * CPB;
-      auto wv_at = [&](int cell) { return tab[W.index(cell - vcolw, ovv)]; };
+      const int vofs = ovv + W.half;
+      // v-tap of window cell k * CPB + j, scaled by nw: (j - vcolw + 1) * nw + k * CPB * nw
+      constexpr int NVB = kOneCell ? 1 : 4;  // one base per cell a lane's rows lie in
+      int vb0[NVB];
+#pragma unroll
+      for (int r = 0; r < NVB; ++r) {
+        vb0[r] = (MO::row(lk, r) / C - vcolw + 1) * nw;
+        asm("" : "+v"(vb0[r]));
+      }
+      auto wv_at = [&](int k, int r) { return tap_read(vb0[r] + k * (CPB * nw), vofs); };
 #pragma unroll
       for (int r = 0; r < 4; ++r) part[r] = npart[r] = (R)0;
 #pragma unroll
       for (int k = 0; k < NBW; ++k) {
         if constexpr (kOneCell) {
-          const R wv = wv_at(k * CPB + MO::row(lk, 0) / C);
+          const R wv = wv_at(k, 0);
 #pragma unroll
           for (int r = 0; r < 4; ++r) part[r] = fma(T[k][r], wv, part[r]);
           npart[0] = fma(T1[0], wv, npart[0]);
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const R wv = wv_at(k * CPB + MO::row(lk, r) / C);
+            const R wv = wv_at(k, r);
             part[r] = fma(T[k][r], wv, part[r]);
             npart[r] = fma(T1[0], wv, npart[r]);
           }
