@@ -306,6 +306,25 @@ __device__ inline float div_by_norm(float a, float b) {
 }
 __device__ inline double div_by_norm(double a, double b) { return a / b; }
 
+// x + (x of lane l ^ 16) and x + (x of lane l ^ 32).  FP32 uses the gfx950
+// lane swaps: with x in both operands, v_permlane16_swap leaves rows
+// (x0, x0, x2, x2) in one result and (x1, x1, x3, x3) in the other (rows of
+// 16 lanes), v_permlane32_swap the lower half of x in both halves of one and
+// the upper half in both halves of the other.  Their sum has the same two
+// addends in every lane as x plus the shuffled x, with no LDS round trip.
+__device__ inline float sum_xor16(float x) {
+  const auto p =
+      __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ inline float sum_xor32(float x) {
+  const auto p =
+      __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ inline double sum_xor16(double x) { return x + __shfl_xor(x, 16); }
+__device__ inline double sum_xor32(double x) { return x + __shfl_xor(x, 32); }
+
 template <typename R, int NPOL, int NB, int NBW = NB>
 __host__ __device__ inline DegridMfmaLds degrid_mfma_layout(int ov, int support, int tu, int tv) {
   DegridMfmaLds L;
@@ -556,7 +575,9 @@ void k_degrid_mfma(
       unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
       const int urow0 = au + Hh - sc;   // region row of the first u-tap
       const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
-      // the group's band of region rows (records are sorted by row)
+      // the group's band of region rows (records are sorted by row).  Taking
+      // the band from lane 0 and the last valid lane with scalar reads was
+      // slower: the branch and loop bounds then wait for the record reads.
       const bool valid = geo != kInvalidGeo;
       int rlo = valid ? urow0 : 1 << 20, rhi = valid ? urow0 + S - 1 : -1;
       for (int off = 1; off < 16; off <<= 1) {
@@ -569,8 +590,8 @@ void k_degrid_mfma(
       // Weight-table reads without a multiply or a clamp pair per read:
       // WTab::index(t, off) = (clamp(t, -1, S) + 1) * nw + off + half, and
       // with nw > 0, (clamp(t, -1, S) + 1) * nw = clamp((t + 1) * nw, 0,
-      // (S + 1) * nw) -- one median instruction -- where (t + 1) * nw moves
-      // by a fixed step from one row step or column block to the next.
+      // (S + 1) * nw), where (t + 1) * nw moves by a fixed step from one row
+      // step or column block to the next.
       // (The empty asm statements keep a per-group base opaque, so that the
       // compiler does not fold base + step back into one multiply per read.)
       const int nw = W.nw, tcap = (S + 1) * nw;
@@ -657,8 +678,8 @@ void k_degrid_mfma(
         if constexpr (!kF32 && C <= 4) x[0] = (x[0] + x[1]) + (x[2] + x[3]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if constexpr (kF32 && C <= 4) x[r] += __shfl_xor(x[r], 16);
-          if constexpr (kF32 || C == 2) x[r] += __shfl_xor(x[r], 32);
+          if constexpr (kF32 && C <= 4) x[r] = sum_xor16(x[r]);
+          if constexpr (kF32 || C == 2) x[r] = sum_xor32(x[r]);
         }
       };
       combine(part);
