@@ -55,6 +55,12 @@ __host__ __device__ inline uint32_t pack_geo(int au, int ou, int av_, int ov) {
 constexpr uint32_t kFarGeo = 0x80FF80FFu;
 __device__ inline uint32_t geo_or_far(uint32_t g) { return g == kInvalidGeo ? kFarGeo : g; }
 
+// A record's words as vectors of 4-byte alignment: one 12-byte store per
+// primary record (one 8-byte store per duplicate), and the grid kernel reads
+// the first two words only.
+typedef uint32_t RecWords __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t RecGridWords __attribute__((ext_vector_type(2), aligned(4)));
+
 __device__ inline void unpack_geo(uint32_t g, int &au, int &ou, int &av_, int &ov) {
   au = (int)(g & 0xFFu) - kPosBias;
   ou = (int)((g >> 8) & 0xFFu) - kOffBias;
@@ -151,9 +157,9 @@ struct PlanDev {
   uint32_t *cur_prim, *cur_dup;     // [n_bins]
   uint32_t *gunit_start, *dunit_start;  // [n_bins + 1]
   uint32_t *scan_tmp;               // block sums
-  // records (SoA)
-  uint2 *rec;                       // [max_records] {sample, packed grid geometry}
-  uint32_t *rec_d;                  // [max_records] packed degrid geometry (primaries only)
+  // records, three words each: {sample, packed grid geometry, packed
+  // degrid geometry}; the third word is written for primary records only
+  uint32_t *rec;                    // [max_records][3]
   // work units: x = bin, y = first record, z = count, w = flags
   uint4 *gunits, *dunits;
   // validity bitmasks, one bit per sample: grid-valid, degrid-valid; bit

@@ -159,12 +159,13 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
     const uint32_t r = r0 + wave * 64 + lane;
     int pos = -1;
     if (r < end) {
-      const uint32_t g = P.rec_d[r];
+      const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
+      const uint32_t g = rw.z;
       if (g != kInvalidGeo) {
         int au, ou, avv, ovv;
         unpack_geo(g, au, ou, avv, ovv);
         pos = (au + kPosBias) | ((avv + kPosBias) << 16);
-        st_item[lane] = P.rec[r].x;
+        st_item[lane] = rw.x;
 #pragma unroll
         for (int t = 0; t < DVW; ++t) {
           st_cu[lane * DVW + t] = t < S ? tab[tab_index(ov, t, sc, ou)] : (R)0;
@@ -455,8 +456,9 @@ void k_degrid_mfma(
 #pragma unroll
   for (int k = 0; k < HK; ++k) {
     const uint32_t r = min(first + tid + k * kDegridThreads, end - 1);
-    pg[k] = P.rec_d[r];
-    pi[k] = P.rec[r].x;
+    const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
+    pg[k] = rw.z;
+    pi[k] = rw.x;
   }
 
   const WTab W(ov, S);
@@ -486,8 +488,9 @@ void k_degrid_mfma(
 #pragma unroll
       for (int k = 0; k < HK; ++k) {
         const uint32_t r = c0 + min(tid + k * kDegridThreads, n - 1);
-        pg[k] = P.rec_d[r];
-        pi[k] = P.rec[r].x;
+        const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
+        pg[k] = rw.z;
+        pi[k] = rw.x;
       }
     }
     // ---- counting sort of the pass's records by key, in LDS

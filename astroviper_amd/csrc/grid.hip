@@ -108,7 +108,8 @@ __device__ inline void fetch_raw(RawSample<R, NPOL> &s, uint32_t item,
 // the end reads the range's last record, which keeps the load unconditional;
 // the caller knows from base + lane < end whether the record is its own.
 __device__ inline uint2 load_record(const PlanDev &P, uint32_t idx, uint32_t end) {
-  return P.rec[min(idx, end - 1)];
+  const RecGridWords v = *(const RecGridWords *)(P.rec + 3 * (size_t)min(idx, end - 1));
+  return make_uint2(v.x, v.y);
 }
 
 template <typename R, int NPOL, int DVW, bool COMP>

@@ -188,21 +188,27 @@ __global__ void k_cursors(PlanDev P) {
   P.cur_dup[b] = s + P.cnt_prim[b];
 }
 
-// One 8-byte store per record; the degrid geometry is stored for primary
-// records only (degrid units cover the primary records of a bin).
+// One store per record: 12 bytes for a primary record, 8 for a duplicate
+// (degrid units cover the primary records of a bin only, so a duplicate's
+// degrid word is never read).
 __device__ inline void write_records(const PlanDev &P, const ItemGeo &g, const RecPlan &rp,
                                      int64_t item, uint32_t slot, int a, int b, bool primary) {
   const int u0 = a * P.tile_u, v0 = b * P.tile_v;
-  P.rec[slot] = make_uint2((uint32_t)item, pack_geo(g.iu - u0, g.ou, g.iv - v0, g.ov));
-  if (primary)
-    P.rec_d[slot] = rp.share ? pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv) : kInvalidGeo;
+  uint32_t *dst = P.rec + 3 * (size_t)slot;
+  const uint32_t gg = pack_geo(g.iu - u0, g.ou, g.iv - v0, g.ov);
+  if (primary) {
+    const uint32_t dg = rp.share ? pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv) : kInvalidGeo;
+    *(RecWords *)dst = RecWords{(uint32_t)item, gg, dg};
+  } else {
+    *(RecGridWords *)dst = RecGridWords{(uint32_t)item, gg};
+  }
 }
 
 __device__ inline void write_degrid_record(const PlanDev &P, const ItemGeo &g, const RecPlan &rp,
                                            int64_t item, uint32_t slot) {
   const int u0 = rp.du_t * P.tile_u, v0 = rp.dv_t * P.tile_v;
-  P.rec[slot] = make_uint2((uint32_t)item, kInvalidGeo);
-  P.rec_d[slot] = pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv);
+  *(RecWords *)(P.rec + 3 * (size_t)slot) =
+      RecWords{(uint32_t)item, kInvalidGeo, pack_geo(g.lu - u0, g.fu, g.lv - v0, g.fv)};
 }
 
 // The records of one sample, their slots taken from the global bin cursors.
@@ -670,7 +676,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_call = take(4 * nb), o_cprim = take(4 * nb), o_bstart = take(4 * (nb + 1)),
          o_curp = take(4 * nb), o_curd = take(4 * nb), o_gus = take(4 * (nb + 1)),
          o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
-         o_rec = take(8 * d.max_records), o_rd = take(4 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
+         o_rec = take(12 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
          o_dun = take(sizeof(uint4) * d.max_dunits),
          o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
          o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells),
@@ -687,7 +693,12 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   }
   p->alloc_base = base;
   p->alloc_bytes = off;
-  hipGetDevice(&p->device);
+  e = hipGetDevice(&p->device);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    delete p;
+    return hip_fail(e, "av_plan_create: hipGetDevice");
+  }
   char *c = (char *)base;
   d.gscale_u = (double *)(c + o_gsu);
   d.gscale_v = (double *)(c + o_gsv);
@@ -701,8 +712,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.gunit_start = (uint32_t *)(c + o_gus);
   d.dunit_start = (uint32_t *)(c + o_dus);
   d.scan_tmp = (uint32_t *)(c + o_scan);
-  d.rec = (uint2 *)(c + o_rec);
-  d.rec_d = (uint32_t *)(c + o_rd);
+  d.rec = (uint32_t *)(c + o_rec);
   d.gunits = (uint4 *)(c + o_gun);
   d.dunits = (uint4 *)(c + o_dun);
   d.gmask = (unsigned long long *)(c + o_gm);
@@ -722,9 +732,9 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
 
 int av_plan_destroy(av_plan *plan) {
   if (!plan) return AV_OK;
-  if (plan->alloc_base) hipFree(plan->alloc_base);
+  const hipError_t e = plan->alloc_base ? hipFree(plan->alloc_base) : hipSuccess;
   delete plan;
-  return AV_OK;
+  return e == hipSuccess ? AV_OK : hip_fail(e, "av_plan_destroy: hipFree");
 }
 
 int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
