@@ -889,8 +889,8 @@ static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const in
   if (nb <= 8)
     return launch_degrid_mfma<R, NPOL, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   if (nb <= 12) {
-    // FP32, four polarisations: records split by v, eight of the twelve blocks
-    if constexpr (sizeof(R) == 4 && NPOL == 4)
+    // four polarisations: records split by v, eight of the twelve blocks
+    if constexpr (NPOL == 4)
       if (nb == 12 && degrid_halves_fit<R, NPOL, 12, 8>(plan->d.tile_u, plan->d.tile_v,
                                                          plan->d.support))
         return launch_degrid_mfma<R, NPOL, 12, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,

@@ -321,12 +321,13 @@ def test_degrid_pol_map_and_constant_grid(gpu, oracle):
     assert np.all(m32[np.abs(m32) > 0] == np.float32(2.0))
 
 
-@pytest.mark.parametrize("support,cap", [(7, 0), (7, 64), (6, 0), (5, 0)])
-def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap):
-    """FP32, four polarisations, 16 x 16 tiles: supports 6 and 7 split each
-    unit's records into two v-halves computing 8 of the 12 column blocks;
-    support 5 (11 blocks) runs all blocks.  Dense units take several sort
-    passes of 512 records (cap 0) or are cut into 64-record units."""
+@pytest.mark.parametrize("support,cap,prec", [(7, 0, 32), (7, 64, 32), (6, 0, 32), (5, 0, 32),
+                                              (7, 0, 64)])
+def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap, prec):
+    """Four polarisations, 16 x 16 tiles: supports 6 and 7 split each unit's
+    records into two v-halves computing 8 of the 12 column blocks; support 5
+    (11 blocks) runs all blocks.  Dense units take several sort passes of
+    512 records (cap 0) or are cut into 64-record units."""
     from astroviper_amd import synth
     arr = synth.ArrayConfig(n_pol=4, seed=3)
     obs = synth.make_observation(arr, synth.SkyConfig())
@@ -338,29 +339,31 @@ def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap):
     rng = np.random.default_rng(support + cap)
     flag = rng.random((nt, nb, nchan, 4)) < 0.1
     grid = rng.standard_normal((nchan, 4) + n_uv) + 1j * rng.standard_normal((nchan, 4) + n_uv)
-    grid = as_prec(32, grid)
+    grid = as_prec(prec, grid)
     mref = oracle.degrid(uvw, flag, freq, grid, cell, support=support)
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    tol = TOL32 if prec == 32 else TOL64
     w = np.ones((nt, nb, nchan, 4))
     vis = np.zeros((nt, nb, nchan, 4), dtype=np.complex128)
     _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, True, 64,
                              support=support, tile=(16, 16), unit_cap=cap)
     st = plan.stats()
     assert st["degrid_valid"] > 0.9 * nt * nb * nchan
-    gd = torch.from_numpy(grid).to(gpu, torch.complex64)
+    gd = torch.from_numpy(grid).to(gpu, cdt)
     fl = torch.from_numpy(flag.reshape(-1, 4).astype(np.uint8)).to(gpu)
     m = plan.degrid(gd, fl).cpu().numpy().reshape(mref.shape)
-    assert rel(m, mref) < TOL32
+    assert rel(m, mref) < tol
     assert np.all(m[flag] == 0)
-    start = torch.ones((m.size // 4, 4), dtype=torch.complex64, device=gpu)
+    start = torch.ones((m.size // 4, 4), dtype=cdt, device=gpu)
     m2 = plan.degrid(gd, fl, start, incremental=True).cpu().numpy().reshape(mref.shape)
-    assert rel(m2, mref + 1.0) < TOL32
+    assert rel(m2, mref + 1.0) < tol
     assert np.all(m2[flag] == 1.0)
     # a constant grid of 2 degrids to exactly 2 wherever a sample lands
-    g2 = torch.full((nchan, 4) + n_uv, 2.0 + 0j, dtype=torch.complex64, device=gpu)
+    g2 = torch.full((nchan, 4) + n_uv, 2.0 + 0j, dtype=cdt, device=gpu)
     m3 = plan.degrid(g2).cpu().numpy()
     on = np.abs(m3) > 0
     assert on.mean() > 0.9
-    assert np.all(m3[on] == np.float32(2.0))
+    assert np.all(m3[on] == 2.0)
 
 
 # --------------------------------------------------------------- adjoint
