@@ -176,8 +176,10 @@ struct PlanDev {
   uint32_t *chunk_dup;              // [n_chan * chunks_per_chan][ntu * ntv]
   int32_t chunk_scatter;            // records written through per-chunk LDS cursors
   // row-block binning (rows carry their own channel): kRowBlockRows rows per
-  // workgroup, per-tile counters and cursors in LDS
+  // workgroup, per-tile counters and cursors in LDS; the counting pass keeps
+  // each block's own-channel counts (prim | dup << 16) for the scatter pass
   int32_t row_blocks;
+  uint32_t *row_hist;               // [row blocks][ntu * ntv], or NULL
   // per-call tables: FP32 kernel table, FP64 per-axis kernel sums by offset
   float *tab32;                     // [ov * (support/2 + 1)]
   double *nrm;                      // [2 * (ov/2 + 2) + 1]

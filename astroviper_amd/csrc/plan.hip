@@ -428,8 +428,10 @@ __global__ __launch_bounds__(kChunkThreads) void k_count_rows(PlanDev P) {
   }
   __syncthreads();
   if (own < 0) return;
+  uint32_t *keep = P.row_hist + (int64_t)blockIdx.x * ntiles;
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
     const uint32_t hv = h[i];
+    keep[i] = hv;
     if (!hv) continue;
     const int64_t b = (int64_t)i * P.n_img_chan + own;
     const uint32_t np = hv & 0xFFFFu;
@@ -438,9 +440,10 @@ __global__ __launch_bounds__(kChunkThreads) void k_count_rows(PlanDev P) {
   }
 }
 
-// Row-block pass 2: the block counts its own-channel records per tile again,
-// reserves one run of slots per tile from each bin cursor (primaries and
-// duplicates), and hands out the slots of each run from LDS cursors.
+// Row-block pass 2: the block loads its own-channel counts per tile (kept by
+// pass 1), reserves one run of slots per tile from each bin cursor
+// (primaries and duplicates), and hands out the slots of each run from LDS
+// cursors.
 __global__ __launch_bounds__(kChunkThreads) void k_scatter_rows(PlanDev P) {
   extern __shared__ uint32_t lds[];
   const int ntiles = P.ntu * P.ntv;
@@ -448,21 +451,12 @@ __global__ __launch_bounds__(kChunkThreads) void k_scatter_rows(PlanDev P) {
   const int64_t r0 = (int64_t)blockIdx.x * kRowBlockRows;
   const int64_t r1 = min(r0 + (int64_t)kRowBlockRows, P.n_rows);
   const int own = row_image_channel(P, r0);
-  for (int i = threadIdx.x; i < 2 * ntiles; i += blockDim.x) lds[i] = 0u;
-  __syncthreads();
   if (own >= 0) {
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
-      ItemGeo g;
-      sample_geometry(P, row, 0, g);
-      if (!(g.gvalid || g.dvalid) || g.ic != own) continue;
-      const RecPlan rp = record_plan(P, g);
-      if (rp.gprim) {
-        atomicAdd(&cp[rp.tu * P.ntv + rp.tv], 1u);
-        for (int a = rp.tu0; a <= rp.tu1; ++a)
-          for (int b = rp.tv0; b <= rp.tv1; ++b)
-            if (a != rp.tu || b != rp.tv) atomicAdd(&cd[a * P.ntv + b], 1u);
-      }
-      if (rp.dsep) atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
+    const uint32_t *kept = P.row_hist + (int64_t)blockIdx.x * ntiles;
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+      const uint32_t hv = kept[i];
+      cp[i] = hv & 0xFFFFu;
+      cd[i] = hv >> 16;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
@@ -665,6 +659,11 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
                                           (n_rows + kChunkThreads - 1) / kChunkThreads * kChunkThreads));
   d.chunks_per_chan = (int32_t)((n_rows + d.rows_per_chunk - 1) / d.rows_per_chunk);
   const int64_t chunk_cells = d.chunked ? (int64_t)n_chan * d.chunks_per_chan * ntiles : 0;
+  // row-block counts, kept for plans whose rows carry their channel when a
+  // block's counters and cursors fit in 160 KB of LDS
+  const int64_t row_cells = (row_channels && n_rows > 0 && 8 * ntiles <= 160 * 1024)
+                                ? (n_rows + kRowBlockRows - 1) / kRowBlockRows * ntiles
+                                : 0;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -679,7 +678,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_rec = take(12 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
          o_dun = take(sizeof(uint4) * d.max_dunits),
          o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
-         o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells),
+         o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells), o_rh = take(4 * row_cells),
          o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
          o_nrm = take(sizeof(double) * (2 * (oversampling / 2 + 2) + 1)),
          o_wt32 = take(sizeof(float) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
@@ -720,6 +719,7 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.n_words = mask_words - 1;
   d.chunk_hist = chunk_cells ? (uint32_t *)(c + o_chh) : nullptr;
   d.chunk_dup = chunk_cells ? (uint32_t *)(c + o_chd) : nullptr;
+  d.row_hist = row_cells ? (uint32_t *)(c + o_rh) : nullptr;
   d.tab32 = (float *)(c + o_tab);
   d.nrm = (double *)(c + o_nrm);
   d.wtab32 = (float *)(c + o_wt32);
@@ -766,7 +766,7 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
     const DeviceProps &prop = device_props();
     if (prop.error) return hip_fail(prop.error, "av_plan_set_uvw: device properties");
     const bool fits = (size_t)(2 * sizeof(uint32_t) * ntiles) <= prop.lds_per_block;
-    if (row_chan) d.row_blocks = fits;
+    if (row_chan) d.row_blocks = fits && d.row_hist != nullptr;
     else d.chunk_scatter = fits;
   }
   const int64_t nchunks = (int64_t)d.n_chan * d.chunks_per_chan;
