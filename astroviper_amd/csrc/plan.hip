@@ -22,7 +22,6 @@
 #include <algorithm>
 
 #include "avgrid_internal.h"
-#include "tile_common.h"
 
 namespace av {
 
@@ -338,16 +337,14 @@ __global__ void k_chunk_bases(PlanDev P) {
 
 // Chunked pass 4: a workgroup per chunk writes the chunk's records.  Its
 // runs' first slots are loaded into LDS cursors, and every record takes its
-// slot from an LDS atomic: no global atomics.  XCD-grouped numbering puts
-// kScatterXcdGroup consecutive chunks of a channel on one XCD, so the runs
-// they write side by side in a bin meet in one L2.
-constexpr int kScatterXcdGroup = 16;
-__global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P, int64_t n_chunks) {
+// slot from an LDS atomic: no global atomics.  (Numbering the chunks so that
+// 16, 32 or 64 consecutive chunks of a channel share an XCD's L2 measured
+// the same as plain numbering.)
+__global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P) {
   extern __shared__ uint32_t lds[];
   const int ntiles = P.ntu * P.ntv;
   uint32_t *cp = lds, *cd = lds + ntiles;  // [n_tiles] each
-  const int64_t w = xcd_grouped_block(blockIdx.x, kScatterXcdGroup);
-  if (w >= n_chunks) return;  // padding of the grouped grid: the whole workgroup
+  const int64_t w = blockIdx.x;
   const int chan = (int)(w / P.chunks_per_chan);
   const int64_t r0 = (w - (int64_t)chan * P.chunks_per_chan) * P.rows_per_chunk;
   const int64_t r1 = min(r0 + (int64_t)P.rows_per_chunk, P.n_rows);
@@ -808,9 +805,7 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
     const int lds = (int)(2 * sizeof(uint32_t) * ntiles);
     if ((rc = ensure_lds_limit<k_scatter_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
       return rc;
-    const int64_t grid = xcd_grouped_grid(nchunks, kScatterXcdGroup);
-    hipLaunchKernelGGL(k_scatter_chunks, dim3((unsigned)grid), dim3(kChunkThreads), lds, s, d,
-                       nchunks);
+    hipLaunchKernelGGL(k_scatter_chunks, dim3((unsigned)nchunks), dim3(kChunkThreads), lds, s, d);
     AV_LAUNCH_CHECK("k_scatter_chunks");
   } else if (d.row_blocks) {
     hipLaunchKernelGGL(k_cursors, dim3(bb), dim3(256), 0, s, d);
