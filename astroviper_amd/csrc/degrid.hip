@@ -17,7 +17,7 @@
 namespace av {
 
 constexpr int kDegridThreads = 256;
-constexpr int kDegridXcdGroup = 16;  // matrix-core kernel: workgroups per XCD group
+constexpr int kDegridXcdGroup = 8;  // matrix-core kernel: workgroups per XCD group (8 measured best of 2-64)
 constexpr int kDegridWaves = kDegridThreads / 64;
 constexpr int kRegionBatch = 12;  // grid values each thread has in flight per pass
 
