@@ -276,8 +276,9 @@ struct DegridMfmaLds {
 };
 
 // The records of a degrid unit are sorted by tile row in LDS, this many at a
-// time, so that each 16-record group covers a narrow band of region rows.
-constexpr int kSortCap = 512;
+// time (one per thread: passes of 512 were slower), so that each 16-record
+// group covers a narrow band of region rows.
+constexpr int kSortCap = 256;
 // Sort keys: tile rows -1 .. tile_u, once per v-half when the records are
 // split by v, and one key for records without degrid geometry.  The scan
 // takes three keys per lane of one wavefront, and two words after the keys

@@ -327,7 +327,7 @@ def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap, prec):
     """Four polarisations, 16 x 16 tiles: supports 6 and 7 split each unit's
     records into two v-halves computing 8 of the 12 column blocks; support 5
     (11 blocks) runs all blocks.  Dense units take several sort passes of
-    512 records (cap 0) or are cut into 64-record units."""
+    256 records (cap 0) or are cut into 64-record units."""
     from astroviper_amd import synth
     arr = synth.ArrayConfig(n_pol=4, seed=3)
     obs = synth.make_observation(arr, synth.SkyConfig())
