@@ -435,6 +435,8 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support, int tile
 }
 
 // Fill the padded table: rows -1 .. S from the call's table, zeros elsewhere.
+// The zero rows are plain stores; only the call's rows are loaded, K per
+// thread in flight (one round trip for the common tables).
 template <typename R>
 __device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P) {
   const R *src;
@@ -443,20 +445,17 @@ __device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P
   else
     src = P.wtab64;
   const int S = P.support, n = L.rows * L.nw, nsrc = (S + 2) * L.nw;
+  const int lo = (L.pad - 1) * L.nw;  // row pad - 1 of the padded table is tap -1
+  for (int i = threadIdx.x; i < lo; i += blockDim.x) w[i] = (R)0;
+  for (int i = lo + nsrc + threadIdx.x; i < n; i += blockDim.x) w[i] = (R)0;
   constexpr int K = 4;  // loads in flight per thread
-  for (int i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
+  for (int i0 = threadIdx.x; i0 < nsrc; i0 += K * blockDim.x) {
     R v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      // row r of the padded table is tap r - pad, row r - pad + 1 of the source
-      const int i = i0 + k * (int)blockDim.x;
-      const int j = i - (L.pad - 1) * L.nw;
-      v[k] = src[min(max(j, 0), nsrc - 1)];
-      if (j < 0 || j >= nsrc) v[k] = (R)0;
-    }
+    for (int k = 0; k < K; ++k) v[k] = src[min(i0 + k * (int)blockDim.x, nsrc - 1)];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (i0 + k * (int)blockDim.x < n) w[i0 + k * blockDim.x] = v[k];
+      if (i0 + k * (int)blockDim.x < nsrc) w[lo + i0 + k * blockDim.x] = v[k];
   }
 }
 
