@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 4  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 5  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -26,6 +26,7 @@ AV_GRID_PSF = 1
 AV_GRID_ACCUMULATE = 2
 AV_GRID_COMPENSATED = 4
 AV_GRID_ADJOINT = 8
+AV_GRID_SUBTILE = 16
 AV_DEGRID_INCREMENTAL = 1
 
 AV_GEOM_REFERENCE = 0
@@ -65,6 +66,15 @@ SIGNATURES = {
     "av_shift_taper": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I32, _D,
                                   _I32, _I32, _I32, _I32, _P]),
     "av_normalise": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _I32, _P]),
+    "av_hogbom_work_bytes": (_c.c_int, [_P]),
+    "av_hogbom_maximg": (_c.c_int, [_P, _P, _I32, _I32, _I32, _P, _P]),
+    "av_hogbom_clean": (_c.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
+                                   _I32, _I32, _I32, _I32, _D, _D, _D, _I32, _I32, _I32, _P,
+                                   _P]),
+    "av_grid_imaging_weights": (_c.c_int, [_P, _I64, _P, _I32, _P, _I32, _I32, _I32, _D, _D,
+                                           _P, _P, _P]),
+    "av_degrid_imaging_weights": (_c.c_int, [_P, _P, _P, _I64, _P, _I32, _P, _I32, _I32, _D,
+                                             _D, _P, _P, _P]),
 }
 
 _lib = None

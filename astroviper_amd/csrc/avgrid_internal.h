@@ -238,3 +238,11 @@ struct av_plan {
   size_t alloc_bytes;
   int device;
 };
+
+// Validate a plan's parameters, fill plan->d's sizes and return in *bytes
+// the device memory its arrays need; with base non-NULL also place the
+// arrays in [base, base + *bytes).  Defined in plan.hip.
+int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u, int32_t n_v,
+                double cell_u, double cell_v, int32_t support, int32_t oversampling,
+                int32_t tile_u, int32_t tile_v, int32_t geometry, int32_t unit_cap, av_plan *p,
+                char *base, size_t *bytes);

@@ -21,6 +21,23 @@ constexpr int kDegridXcdGroup = 8;  // matrix-core kernel: workgroups per XCD gr
 constexpr int kDegridWaves = kDegridThreads / 64;
 constexpr int kRegionBatch = 12;  // grid values each thread has in flight per pass
 
+// Flag bytes (0xFF in byte p) of the visibility polarisations whose pol_map
+// entry lies outside [0, npol_g): dgrid_numba skips such a polarisation
+// (standard_degrid.py:419-420), so it is treated like a flagged one -- left
+// alone when incremental, zero otherwise.
+template <int NPOL>
+__device__ inline uint32_t pol_map_skips(const int32_t *pol_map, int npol_g) {
+  uint32_t bad = 0;
+  if (pol_map) {
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p) {
+      const int q = pol_map[p];
+      if (q < 0 || q >= npol_g) bad |= 0xFFu << (8 * p);
+    }
+  }
+  return bad;
+}
+
 // Copy the region [u0 - halo, u0 - halo + rows) x [v0 - halo, v0 - halo +
 // cols) of every visibility polarisation's grid plane (through pol_map) into
 // LDS, zero outside the grid and beyond (ru_real, rv_real).  The cells are
@@ -125,6 +142,7 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
   int *st_pos = (int *)(smem + L.st_pos) + wave * 64;
   uint32_t *st_item = (uint32_t *)(smem + L.st_item) + wave * 64;
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const uint32_t polbad = pol_map_skips<NPOL>(pol_map, npol_g);
   const int Hh = L.halo, RV = L.rv;
 
   for (int i = tid; i < L.ntab; i += kDegridThreads)
@@ -208,10 +226,10 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
       nacc = sum_over_dv<H, DVW>(nacc);
       if (pj >= 0 && dv == DVW - 1) {
         const uint32_t item = st_item[j];
-        uint32_t fbits = 0;
+        uint32_t fbits = polbad;
         if (flag) {
           if (NPOL == 4)
-            fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+            fbits |= *(const uint32_t *)(flag + (size_t)item * 4);
           else
             for (int p = 0; p < NPOL; ++p) fbits |= (uint32_t)flag[(size_t)item * NPOL + p] << (8 * p);
         }
@@ -222,11 +240,12 @@ __global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(
 #pragma unroll
         for (int c = 0; c < CC; ++c) {
           const int p = (h * CC + c) >> 1;
-          const bool flagged = (fbits >> (8 * p)) & 0xFFu;
+          // a zero kernel sum adds nothing (standard_degrid.py:482-485)
+          const bool skip = ((fbits >> (8 * p)) & 0xFFu) || nacc == (R)0;
           if (incremental) {
-            if (!flagged) o.v[c] += a[c] / nacc;
+            if (!skip) o.v[c] += a[c] / nacc;
           } else {
-            o.v[c] = flagged ? (R)0 : a[c] / nacc;
+            o.v[c] = skip ? (R)0 : a[c] / nacc;
           }
         }
         *dst = o;
@@ -430,6 +449,7 @@ void k_degrid_mfma(
   uint32_t *s_item = (uint32_t *)(smem + L.sitem);
   uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
   const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const uint32_t polbad = pol_map_skips<NPOL>(pol_map, npol_g);
   const int Hh = L.halo, RS = L.rs;
   const int li = lane & 15, lk = lane >> 4;
   const int rkeys = Tu + 2;  // row keys of one half
@@ -566,14 +586,14 @@ void k_degrid_mfma(
       // The load is unconditional -- a lane without a record reads its slot's
       // sample (0 or a valid one) -- and its result is needed only at the end
       // of the group; neighbouring channels share the flag line in L2
-      uint32_t fbits = 0;
+      uint32_t fbits = polbad;
       if (flag) {
         if (NPOL == 4)
-          fbits = *(const uint32_t *)(flag + (size_t)item * 4);
+          fbits |= *(const uint32_t *)(flag + (size_t)item * 4);
         else if (NPOL == 2)
-          fbits = *(const uint16_t *)(flag + (size_t)item * 2);
+          fbits |= *(const uint16_t *)(flag + (size_t)item * 2);
         else
-          fbits = flag[item];
+          fbits |= flag[item];
       }
       int au, ou, avv, ovv;
       unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
@@ -701,11 +721,12 @@ void k_degrid_mfma(
         for (int r = 0; r < kVals; ++r) {
           const int c = comp_of(r);
           const R val = div_by_norm(part[r], npart[r]);
-          const bool flagged = (fbits >> (8 * (c >> 1))) & 0xFFu;
+          // a zero kernel sum adds nothing (standard_degrid.py:482-485)
+          const bool skip = ((fbits >> (8 * (c >> 1))) & 0xFFu) || npart[r] == (R)0;
           if (incremental) {
-            if (!flagged) base[c] += val;
+            if (!skip) base[c] += val;
           } else {
-            base[c] = flagged ? (R)0 : val;
+            base[c] = skip ? (R)0 : val;
           }
         }
       }

@@ -27,6 +27,7 @@ constexpr int kGridThreads = 256;
 constexpr int kGridMaxWaves = kGridThreads / 64;
 constexpr int kCompChunkBatches = 4;  // compensated mode: fold every 4*64 records
 constexpr int kGridXcdGroup = 16;     // matrix-core kernel: workgroups per XCD group
+constexpr int kGridSubtile = AV_GRID_SUBTILE;
 
 template <typename R>
 struct Cplx {
@@ -842,6 +843,353 @@ static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
   return AV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Sub-tile scheduled matrix-core gridder (FP32, 16 x 16 tiles).  A record's
+// footprint (at most S x S cells) covers only a few of the tile's sixteen
+// 4 x 4 sub-tiles, yet a 16 x 16 x 4 product spends its full 16 x 16 block
+// on every record.  v_mfma_f32_4x4x1_16b_f32 instead performs sixteen
+// independent rank-1 4 x 4 updates, one per block of four lanes, and each
+// block may take a different record: block b accumulates sub-tile
+// (b / 4, b % 4), and in every round it applies the next record of its own
+// list of the batch's records that touch that sub-tile.  A batch of 64
+// records then takes max_b |list_b| rounds (about a quarter of the records
+// for a dense tile, never more than the records) of 2 * npol instructions of
+// 8 cycles, where the 16 x 16 x 4 form takes 2 * npol instructions of 32
+// cycles per four records.
+//
+// Lane l of a 4 x 4 x 1 product supplies A[block l/4][row l%4] and
+// B[block l/4][column l%4], and holds rows 0..3 (registers) of column l%4 of
+// block l/4 of the result.  So lane l evaluates, for its block's record of
+// the round, the u-weight at tile row 4*(b/4) + l%4 and the v-weight at tile
+// column 4*(b%4) + l%4 times each data component.  A list entry beyond a
+// block's records is 64, a staging slot that holds zero data.
+//
+// Weight table: a block is only ever given records whose footprint reaches
+// it, so every tap it reads lies within 3 rows of the footprint (taps -3 ..
+// S + 3 cover odd and even supports).  The LDS table is the call's table
+// (taps -1 .. S) padded to taps -4 .. S + 3 with zero rows, and to at least
+// 16 rows, so that the null record -- table base row 0 -- reads in-bounds
+// finite weights for every row of every block (its data are zero).  A
+// record's stored table base is offset by 16 rows so that it is never
+// negative; the lane's row offset carries the -16.
+constexpr int kSubNull = 64;  // staging slot of the null record
+constexpr int kSubPad = 4;    // zero rows below tap 0 (and above tap S - 1, at least)
+constexpr int kSubShift = 16;
+
+struct SubLds {
+  int tab, nrm, wave, st_d, st_geo, list, wave_bytes, total, pad, nw, rows, nnrm;
+};
+
+template <int NPOL>
+__host__ __device__ inline SubLds sub_lds_layout(int ov, int support) {
+  SubLds L;
+  L.nnrm = 2 * (ov / 2 + 2) + 1;
+  L.pad = kSubPad;
+  L.nw = WTab(ov, support).nw;
+  L.rows = support + 2 * kSubPad > 16 ? support + 2 * kSubPad : 16;
+  int o = 0;
+  L.tab = o;
+  o += (L.rows * L.nw * 4 + 15) & ~15;
+  L.nrm = o;
+  o += (L.nnrm * 8 + 15) & ~15;
+  L.wave = o;
+  // per wave: 65 staged records' data and table bases, then the block lists
+  // (16 x 64 slot numbers, plus 16 bytes that a read one round ahead may touch)
+  L.st_d = 0;
+  L.st_geo = (kSubNull + 1) * 2 * NPOL * 4;
+  L.list = (L.st_geo + (kSubNull + 1) * 4 + 15) & ~15;
+  L.wave_bytes = L.list + 16 * 64 + 16;
+  L.total = o + kGridMaxWaves * L.wave_bytes;
+  return L;
+}
+
+// Sub-tiles (bit 4 * bu + bv) touched by a footprint centred at tile-local
+// (au, av) with half-width sc, clipped to the 16 x 16 tile.
+__device__ inline uint32_t subtile_mask(int au, int av, int sc) {
+  const int rlo = max(au - sc, 0) >> 2, rhi = min(au + sc, 15) >> 2;
+  const int clo = max(av - sc, 0) >> 2, chi = min(av + sc, 15) >> 2;
+  if (rlo > rhi || clo > chi) return 0u;
+  const uint32_t rows = ((2u << rhi) - 1u) & ~((1u << rlo) - 1u);
+  const uint32_t cols = ((2u << chi) - 1u) & ~((1u << clo) - 1u);
+  // rows bit bu -> bit 4 * bu, times the column bits (no carries)
+  const uint32_t spread = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
+  return spread * cols;
+}
+
+// MODE: 0 visibilities x weights, 1 weights only (PSF), 2 visibilities / norm
+// (the adjoint of the degridder).  A template parameter, so that no mode's
+// arithmetic is evaluated speculatively in another's.
+template <int NPOL, int MODE>
+__global__ __launch_bounds__(kGridThreads, 6)
+void k_grid_sub(PlanDev P, const float *__restrict__ vis, const float *__restrict__ weight,
+                const uint8_t *__restrict__ flag, float *__restrict__ grid, int mode) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  constexpr int C = 2 * NPOL;
+  constexpr bool psf = MODE == 1, adjoint = MODE == 2;
+  using CL = CellLayout<float, NPOL>;
+  extern __shared__ __align__(16) char smem[];
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const SubLds L = sub_lds_layout<NPOL>(ov, S);
+  float *tab = (float *)(smem + L.tab);
+  double *nrm = (double *)(smem + L.nrm);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char *wbase = smem + L.wave + wave * L.wave_bytes;
+  float *st_d = (float *)(wbase + L.st_d);          // [65][C]
+  uint32_t *st_geo = (uint32_t *)(wbase + L.st_geo);  // [65] table bases, 16 + 16 bits
+  uint8_t *list = (uint8_t *)(wbase + L.list);       // [16][64] slot numbers
+  const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
+  const int half = ov / 2 + 2;
+
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  const uint32_t blk = xcd_grouped_block(blockIdx.x, kGridXcdGroup);
+  if (blk * kGridMaxWaves >= n_units) return;  // whole workgroup idle
+  {
+    MfmaLds ML;
+    ML.pad = L.pad;
+    ML.nw = L.nw;
+    ML.rows = L.rows;
+    copy_wtab_padded(tab, ML, P);
+  }
+  for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
+  // the null record: zero data; its table bases point at row 0
+  if (lane < C) st_d[kSubNull * C + lane] = 0.0f;
+  if (lane == 0) st_geo[kSubNull] = (uint32_t)(kSubShift * L.nw) * 0x10001u;
+  __syncthreads();  // the only workgroup barrier: tables ready
+  const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blk * kGridMaxWaves + wave));
+  if (u >= n_units) return;
+
+  const uint4 unit = P.gunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * 16, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * 16;
+  const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
+  float *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+  if (cnt == 0) {
+    if (lane < NPOL) P.upart[(size_t)u * 4 + lane] = 0.0;
+    if (!accumulate) {
+      for (int idx = lane; idx < NPOL * 256; idx += 64) {
+        const int p = idx >> 8, c = idx & 255;
+        const int gu = u0 + (c >> 4), gv = v0 + (c & 15);
+        if (gu < P.n_u && gv < P.n_v)
+          *(Cplx<float> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) =
+              Cplx<float>{0.0f, 0.0f};
+      }
+    }
+    return;
+  }
+
+  // this lane's block and its row / column inside the block
+  const int blk_id = lane >> 2, li = lane & 3;
+  const int bu = blk_id >> 2, bv = blk_id & 3;
+  // table index of tile row r for a record with stored base tb: tb + (r - 16) nw
+  const int rowoff = (4 * bu + li - kSubShift) * L.nw;
+  const int coloff = (4 * bv + li - kSubShift) * L.nw;
+  const int lbase = blk_id * 64;
+
+  V4 acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = V4{0.0f, 0.0f, 0.0f, 0.0f};
+  double sw[NPOL];
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+
+  const uint32_t end = first + cnt;
+  uint2 r0 = load_record(P, first + lane, end), r1 = load_record(P, first + 64 + lane, end);
+  RawSample<float, NPOL> raw0;
+  fetch_raw<float, NPOL>(raw0, r0.x, vis, weight, flag, psf, adjoint);
+  for (uint32_t base = first; base < end; base += 64) {
+    // ---- stage this lane's record: data (zeros when skipped) and geometry
+    uint32_t geo = kInvalidGeo;
+    float d[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) d[k] = 0.0f;
+    const uint32_t g0 = base + lane < end ? r0.y : kInvalidGeo;
+    if (g0 != kInvalidGeo) {
+      int au, ou, avv, ovv;
+      unpack_geo(g0, au, ou, avv, ovv);
+      const double norm = nrm[ou + half] * nrm[ovv + half];
+      const bool prim = base + lane < prim_end;
+      float inv = 0.0f;
+      if constexpr (adjoint) inv = (float)(1.0 / norm);
+      bool any = false;
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        float dre, dim;
+        if constexpr (adjoint) {
+          dre = raw0.vv[p].re * inv;
+          dim = raw0.vv[p].im * inv;
+        } else {
+          float w = raw0.wv[p];
+          if ((raw0.fbits >> (8 * p)) & 0xFFu) w = 0.0f;
+          if constexpr (psf) {
+            dre = w;
+            dim = 0.0f;
+          } else {
+            dre = raw0.vv[p].re * w;
+            dim = raw0.vv[p].im * w;
+          }
+        }
+        // standard_grid.py:378: skip NaN or exactly-zero data
+        const bool ok = !(isnan(dre) || isnan(dim)) && !(dre == 0.0f && dim == 0.0f);
+        if (ok) {
+          any = true;
+          d[2 * p] = dre;
+          d[2 * p + 1] = dim;
+          if constexpr (!adjoint)
+            if (prim) sw[p] += (double)raw0.wv[p] * norm;
+        }
+      }
+      if (any) geo = g0;
+    }
+#pragma unroll
+    for (int k = 0; k < C; k += CL::kCC) {
+      Vec<float, CL::kCC> t;
+#pragma unroll
+      for (int c = 0; c < CL::kCC; ++c) t.v[c] = d[k + c];
+      *(Vec<float, CL::kCC> *)(st_d + lane * C + k) = t;
+    }
+    uint32_t touched = 0u;
+    {
+      uint32_t tb = 0u;
+      if (geo != kInvalidGeo) {
+        int au, ou, avv, ovv;
+        unpack_geo(geo, au, ou, avv, ovv);
+        // tile row r is tap r - au + sc, table row r - au + sc + pad
+        tb = (uint32_t)((sc - au + kSubPad + kSubShift) * L.nw + ou + L.nw / 2) |
+             ((uint32_t)((sc - avv + kSubPad + kSubShift) * L.nw + ovv + L.nw / 2) << 16);
+        touched = subtile_mask(au, avv, sc);
+      }
+      st_geo[lane] = tb;
+    }
+    // ---- the blocks' lists: every entry starts as the null record, then
+    // each record is appended to the list of every sub-tile it touches
+    *(uint4 *)(list + lane * 16) = make_uint4(0x40404040u, 0x40404040u, 0x40404040u, 0x40404040u);
+    wave_lds_sync();
+    uint32_t rounds = 0;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const bool t = (touched >> s) & 1u;
+      const uint64_t bal = __ballot(t);
+      const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+      if (t) list[s * 64 + pos] = (uint8_t)lane;
+      rounds = max(rounds, (uint32_t)__popcll(bal));
+    }
+    // ---- issue the next loads
+    RawSample<float, NPOL> raw1;
+    fetch_raw<float, NPOL>(raw1, r1.x, vis, weight, flag, psf, adjoint);
+    const uint2 r2 = load_record(P, base + 128 + lane, end);
+    wave_lds_sync();
+    // ---- rounds: every block applies the next record of its list
+    for (uint32_t i = 0; i < rounds; ++i) {
+      const int k = list[lbase + i];
+      const uint32_t tb = st_geo[k];
+      float dd[C];
+#pragma unroll
+      for (int q = 0; q < C; q += CL::kCC) {
+        const Vec<float, CL::kCC> t = *(const Vec<float, CL::kCC> *)(st_d + k * C + q);
+#pragma unroll
+        for (int c = 0; c < CL::kCC; ++c) dd[q + c] = t.v[c];
+      }
+      const float a = tab[(tb & 0xFFFFu) + rowoff];
+      const float wv = tab[(tb >> 16) + coloff];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if constexpr (psf) {
+          if (c & 1) continue;
+        }
+        acc[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, wv * dd[c], acc[c], 0, 0, 0);
+      }
+    }
+    wave_lds_sync();
+    raw0 = raw1;
+    r0 = r1;
+    r1 = r2;
+  }
+
+  // ---- flush: lane holds rows 0..3 of column li of sub-tile (bu, bv)
+  const bool multi = (fl & kUnitMulti) != 0;
+  const int gv = v0 + 4 * bv + li;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int gu = u0 + 4 * bu + r;
+    if (gu >= P.n_u || gv >= P.n_v) continue;
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p) {
+      const float re = acc[2 * p][r], im = acc[2 * p + 1][r];
+      float *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+      if (multi) {
+        if (re != 0.0f) atomicAdd(dst, re);
+        if (im != 0.0f) atomicAdd(dst + 1, im);
+      } else if (accumulate) {
+        Cplx<float> old = *(Cplx<float> *)dst;
+        old.re += re;
+        old.im += im;
+        *(Cplx<float> *)dst = old;
+      } else {
+        *(Cplx<float> *)dst = Cplx<float>{re, im};
+      }
+    }
+  }
+  // ---- sums of weights (FP64): this unit's partial per polarisation
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) {
+    double sv = adjoint ? 0.0 : sw[p];
+    for (int off = 32; off >= 1; off >>= 1) sv += __shfl_xor(sv, off);
+    if (lane == 0) P.upart[(size_t)u * 4 + p] = sv;
+  }
+}
+
+template <int NPOL, int MODE>
+static int launch_grid_sub(av_plan *plan, const void *vis, const void *weight,
+                           const uint8_t *flag, void *grid, double *sumwt, int mode,
+                           hipStream_t s) {
+  PlanDev &d = plan->d;
+  const SubLds L = sub_lds_layout<NPOL>(d.ov, d.support);
+  {
+    const int rc_lds =
+        ensure_lds_limit<k_grid_sub<NPOL, MODE>>(L.total, "av_grid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
+  const unsigned nwg = (unsigned)xcd_grouped_grid(
+      (d.max_gunits + kGridMaxWaves - 1) / kGridMaxWaves, kGridXcdGroup);
+  hipLaunchKernelGGL((k_grid_sub<NPOL, MODE>), dim3(nwg), dim3(kGridThreads), L.total, s, d,
+                     (const float *)vis, (const float *)weight, flag, (float *)grid, mode);
+  AV_LAUNCH_CHECK("k_grid_sub");
+  if (!(mode & AV_GRID_ADJOINT)) {
+    hipLaunchKernelGGL(k_sumwt_reduce, dim3((unsigned)d.n_img_chan), dim3(256), 0, s, d, NPOL,
+                       sumwt);
+    AV_LAUNCH_CHECK("k_sumwt_reduce");
+  }
+  return AV_OK;
+}
+
+// The sub-tile kernel serves FP32 16 x 16 tiles without compensated sums
+// whenever its padded weight table fits in a workgroup's LDS.
+template <int NPOL>
+static int try_grid_sub(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                        const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s,
+                        bool &done) {
+  const PlanDev &d = plan->d;
+  if (d.tile_u != 16 || d.tile_v != 16 || (mode & AV_GRID_COMPENSATED)) return AV_OK;
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
+  if ((size_t)sub_lds_layout<NPOL>(d.ov, d.support).total > prop.lds_per_block) return AV_OK;
+  int rc = prepare_tables(d, cgk, s);
+  if (rc) return rc;
+  done = true;
+  if (mode & AV_GRID_PSF)
+    return launch_grid_sub<NPOL, 1>(plan, vis, weight, flag, grid, sumwt, mode, s);
+  if (mode & AV_GRID_ADJOINT)
+    return launch_grid_sub<NPOL, 2>(plan, vis, weight, flag, grid, sumwt, mode, s);
+  return launch_grid_sub<NPOL, 0>(plan, vis, weight, flag, grid, sumwt, mode, s);
+}
+
 // Result blocks a matrix-core unit may hold in registers: 16 for FP32 (64
 // registers), 8 for FP64 and for the compensated mode.
 template <typename R, bool COMP>
@@ -898,6 +1246,10 @@ static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, con
   bool done = false;
   int rc;
   if constexpr (sizeof(R) == 4) {
+    if (!comp && (mode & kGridSubtile)) {
+      rc = try_grid_sub<NPOL>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s, done);
+      if (done || rc) return rc;
+    }
     if (comp)
       rc = grid_mfma_shapes<R, NPOL, true>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s,
                                            done);

@@ -579,14 +579,14 @@ using namespace av;
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-extern "C" {
-
-int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u, int32_t n_v,
-                   double cell_u, double cell_v, int32_t support, int32_t oversampling,
-                   int32_t tile_u, int32_t tile_v, int32_t geometry, int32_t unit_cap,
-                   av_plan **out) {
-  if (!out) return fail(AV_ERR_INVALID, "av_plan_create: out is NULL");
-  *out = nullptr;
+// Validate the plan parameters, fill plan->d's sizes and compute the bytes
+// of device memory the plan's arrays need; with base non-NULL, also point
+// the arrays into [base, base + bytes).  Shared by av_plan_create (one
+// hipMalloc) and av_grid_noalloc (the library's static scratch array).
+int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u, int32_t n_v,
+                double cell_u, double cell_v, int32_t support, int32_t oversampling,
+                int32_t tile_u, int32_t tile_v, int32_t geometry, int32_t unit_cap, av_plan *p,
+                char *base, size_t *bytes) {
   if (n_rows < 0 || n_chan < 1 || n_img_chan < 1 || n_u < 1 || n_v < 1)
     return fail(AV_ERR_INVALID, "av_plan_create: sizes must be positive");
   if (support < 2 || support > kMaxSupport)
@@ -609,7 +609,6 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   if (n_items > ((int64_t)1 << 32) / 5 - 1)
     return fail(AV_ERR_UNSUPPORTED, "av_plan_create: more than 858993458 samples per plan");
 
-  av_plan *p = new av_plan();
   PlanDev &d = p->d;
   d.n_rows = n_rows;
   d.n_chan = n_chan;
@@ -681,21 +680,9 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
          o_wt32 = take(sizeof(float) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
          o_wt64 = take(sizeof(double) * (support + 2) * (2 * (oversampling / 2 + 2) + 1)),
          o_upart = take(sizeof(double) * 4 * d.max_gunits), o_ctr = take(4 * 16);
-  void *base = nullptr;
-  hipError_t e = hipMalloc(&base, off);
-  if (e != hipSuccess) {
-    delete p;
-    return hip_fail(e, "av_plan_create: hipMalloc");
-  }
-  p->alloc_base = base;
-  p->alloc_bytes = off;
-  e = hipGetDevice(&p->device);
-  if (e != hipSuccess) {
-    (void)hipFree(base);
-    delete p;
-    return hip_fail(e, "av_plan_create: hipGetDevice");
-  }
-  char *c = (char *)base;
+  *bytes = off;
+  if (!base) return AV_OK;
+  char *c = base;
   d.gscale_u = (double *)(c + o_gsu);
   d.gscale_v = (double *)(c + o_gsv);
   d.freq = (double *)(c + o_freq);
@@ -723,6 +710,39 @@ int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n
   d.wtab64 = (double *)(c + o_wt64);
   d.upart = (double *)(c + o_upart);
   d.counters = (uint32_t *)(c + o_ctr);
+  p->alloc_base = base;
+  p->alloc_bytes = off;
+  return AV_OK;
+}
+
+extern "C" {
+
+int av_plan_create(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u, int32_t n_v,
+                   double cell_u, double cell_v, int32_t support, int32_t oversampling,
+                   int32_t tile_u, int32_t tile_v, int32_t geometry, int32_t unit_cap,
+                   av_plan **out) {
+  if (!out) return fail(AV_ERR_INVALID, "av_plan_create: out is NULL");
+  *out = nullptr;
+  av_plan probe{};
+  size_t bytes = 0;
+  int rc = plan_layout(n_rows, n_chan, n_img_chan, n_u, n_v, cell_u, cell_v, support,
+                       oversampling, tile_u, tile_v, geometry, unit_cap, &probe, nullptr, &bytes);
+  if (rc) return rc;
+  void *base = nullptr;
+  hipError_t e = hipMalloc(&base, bytes);
+  if (e != hipSuccess) return hip_fail(e, "av_plan_create: hipMalloc");
+  av_plan *p = new av_plan();
+  rc = plan_layout(n_rows, n_chan, n_img_chan, n_u, n_v, cell_u, cell_v, support, oversampling,
+                   tile_u, tile_v, geometry, unit_cap, p, (char *)base, &bytes);
+  if (rc == AV_OK) {
+    e = hipGetDevice(&p->device);
+    if (e != hipSuccess) rc = hip_fail(e, "av_plan_create: hipGetDevice");
+  }
+  if (rc) {
+    (void)hipFree(base);
+    delete p;
+    return rc;
+  }
   *out = p;
   return AV_OK;
 }

@@ -15,7 +15,7 @@ import torch
 from . import _lib
 from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_DEGRID, AV_PLAN_ROW_CHANNELS,
                    AV_GEOM_REFERENCE, AV_GRID_ACCUMULATE, AV_GRID_ADJOINT,
-                   AV_GRID_COMPENSATED, AV_GRID_PSF, call)
+                   AV_GRID_COMPENSATED, AV_GRID_PSF, AV_GRID_SUBTILE, call)
 
 
 def require_gpu(device=None) -> torch.device:
@@ -195,11 +195,14 @@ class GridPlan:
              flag: Optional[torch.Tensor] = None, grid: Optional[torch.Tensor] = None,
              sumwt: Optional[torch.Tensor] = None, *, n_pol: Optional[int] = None,
              psf: bool = False, accumulate: bool = False, compensated: bool = False,
-             adjoint: bool = False, dtype: Optional[torch.dtype] = None):
+             adjoint: bool = False, dtype: Optional[torch.dtype] = None,
+             subtile: bool = False):
         """Grid samples onto [n_img_chan, n_pol, n_u, n_v] complex planes.
 
         vis [samples, n_pol] complex, weight [samples, n_pol] real, flag
         [samples, n_pol] uint8/bool or None.  Returns (grid, sumwt).
+        ``subtile`` runs FP32 16 x 16 tiles through the sub-tile scheduled
+        4x4x1 matrix-core kernel instead of the dense 16x16x4 one.
         """
         ref = vis if vis is not None else weight
         if ref is None:
@@ -240,7 +243,7 @@ class GridPlan:
             return grid, sumwt
         mode = ((AV_GRID_PSF if psf else 0) | (AV_GRID_ACCUMULATE if accumulate else 0)
                 | (AV_GRID_COMPENSATED if compensated else 0)
-                | (AV_GRID_ADJOINT if adjoint else 0))
+                | (AV_GRID_ADJOINT if adjoint else 0) | (AV_GRID_SUBTILE if subtile else 0))
         call("av_grid", self._h, _ptr(None if psf else vis),
              _ptr(None if adjoint else weight), _ptr(flag), int(n_pol), _ptr(self.cgk),
              _ptr(grid), _ptr(sumwt), code, mode, _stream())
@@ -277,8 +280,10 @@ class GridPlan:
 # ------------------------------------------------------ kernel choice --
 def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = False,
                      support: int = 7, oversampling: int = 100,
-                     lds_bytes: int = 160 * 1024) -> str:
-    """Name of the tile kernel av_grid runs for this plan shape: the
+                     lds_bytes: int = 160 * 1024, subtile: bool = False) -> str:
+    """Name of the tile kernel av_grid runs for this plan shape: for FP32
+    16 x 16 tiles without compensated sums, the sub-tile scheduled kernel
+    (4x4x1 products) when ``subtile`` is asked for, else the dense
     matrix-core kernel for tiles of 16 x 16 multiples whose result blocks
     ((tile_u/16) * 2*n_pol * (tile_v/16)) fit its register budget (16 blocks
     for FP32, 8 for FP64 and for compensated sums) and whose LDS (a weight
@@ -286,6 +291,13 @@ def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = F
     kernel-sum table and four wavefronts' staging) fits in a workgroup's
     `lds_bytes`, else the LDS kernel."""
     tu, tv = int(tile[0]), int(tile[1])
+    if (tu, tv) == (16, 16) and not fp64 and not compensated and subtile:
+        nw = 2 * (oversampling // 2 + 2) + 1
+        up16 = lambda b: (b + 15) & ~15
+        wave = up16(up16(65 * 8 * n_pol) + 65 * 4) + 16 * 64 + 16
+        total = up16(max(support + 8, 16) * nw * 4) + up16(nw * 8) + 4 * wave
+        if total <= lds_bytes:
+            return "k_grid_sub"
     limit = 8 if (fp64 or compensated) else 16
     if tu in (16, 32) and tv in (16, 32, 64) and (tu // 16) * 2 * n_pol * (tv // 16) <= limit:
         r = 8 if fp64 else 4

@@ -37,9 +37,9 @@ def _degrid_to_device(uvw, flag, freq, grid, scale, offset, support, sampling, c
     # channels mapped outside the grid are skipped (standard_degrid.py:423)
     cmap = np.where((cmap >= 0) & (cmap < nchan_g), cmap, -1).astype(np.int32)
     cmap_t = torch.from_numpy(cmap).to(dev)
-    pmap = np.asarray(polmap).astype(np.int32)
-    bad_pol = (pmap < 0) | (pmap >= npol_g)
-    pmap_t = torch.from_numpy(np.where(bad_pol, 0, pmap).astype(np.int32)).to(dev)
+    # polarisations mapped outside the grid are skipped by av_degrid itself
+    # (standard_degrid.py:419-420)
+    pmap_t = torch.from_numpy(np.asarray(polmap).astype(np.int32)).to(dev)
     step = max(1, _MAX_PLAN_SAMPLES // max(1, nf))
     for r0 in range(0, rows, step):
         r1 = min(rows, r0 + step)
@@ -51,9 +51,7 @@ def _degrid_to_device(uvw, flag, freq, grid, scale, offset, support, sampling, c
         _dev.call("av_plan_set_degrid_transform", plan._h, float(scale[0]), float(scale[1]),
                   float(offset[0]), float(offset[1]))
         plan.set_uvw(torch.from_numpy(uvw_h[r0:r1]).to(dev), freq_t, cmap_t)
-        fl = f_h[r0:r1].copy()
-        if bad_pol.any():  # polarisations outside the grid are skipped (:419-420)
-            fl[:, :, bad_pol] = 1
+        fl = f_h[r0:r1]
         out = values_dev[r0:r1].reshape((r1 - r0) * nf, npol)
         plan.degrid(g_dev, torch.from_numpy(fl).to(dev), out, n_pol=npol, pol_map=pmap_t,
                     incremental=incremental)

@@ -27,6 +27,54 @@ def _vars(data_vars, default):
     return data_vars
 
 
+def make_empty_padded_uv_image(image, image_size, uv_data_array: str = "VISIBILITY_RESIDUAL",
+                               insert_res_or_mod: bool = True):
+    """standard_image_grid_preparation.py:123-233: an image dataset of
+    `image_size` carrying (time, frequency, polarization, u, v) complex64
+    zeros in `uv_data_array` and float64 zeros in its _NORMALIZATION, on the
+    l/m coordinates of `image` padded to `image_size`.  With
+    insert_res_or_mod, the MODEL or RESIDUAL image matching
+    VISIBILITY_MODEL / VISIBILITY_RESIDUAL is copied into the centre of a
+    float32 image of the padded size.  u and v are in wavelengths,
+    (i - n//2) / (n * cell), so index n//2 is the origin."""
+    from .msv4 import DataArray, ImageDataset
+    image_size = np.array(image_size)
+    cur = np.array([image.sizes["l"], image.sizes["m"]])
+    if np.any(cur > image_size):
+        raise Exception(f"current image size {cur} is larger than requested {image_size}")
+    cl = float(image.l[1].data - image.l[0].data)
+    cm = float(image.m[1].data - image.m[0].data)
+    nx, ny = int(image_size[0]), int(image_size[1])
+    coords = {
+        "time": image.coords["time"].values,
+        "frequency": image.coords["frequency"].values,
+        "polarization": image.coords["polarization"].values,
+        "l": (np.arange(nx) - nx // 2) * cl,
+        "m": (np.arange(ny) - ny // 2) * cm,
+        "u": (np.arange(nx) - nx // 2) / (nx * abs(cl)),
+        "v": (np.arange(ny) - ny // 2) / (ny * abs(cm)),
+    }
+    lead = (len(coords["time"]), len(coords["frequency"]), len(coords["polarization"]))
+    out = ImageDataset({}, coords, attrs=dict(getattr(image, "attrs", {})))
+    uv_dims = ("time", "frequency", "polarization", "u", "v")
+    out[uv_data_array] = DataArray(np.zeros(lead + (nx, ny), dtype=np.complex64), dims=uv_dims,
+                                   coords={k: coords[k] for k in uv_dims})
+    out[uv_data_array + "_NORMALIZATION"] = DataArray(
+        np.zeros(lead, dtype=np.float64), dims=uv_dims[:3],
+        coords={k: coords[k] for k in uv_dims[:3]})
+    if insert_res_or_mod:
+        name = {"VISIBILITY_MODEL": "MODEL", "VISIBILITY_RESIDUAL": "RESIDUAL"}.get(uv_data_array)
+        if name and name in image.data_vars:
+            dat = np.asarray(image[name].data)
+            blc = (image_size - np.array(dat.shape[-2:])) // 2
+            trc = blc + np.array(dat.shape[-2:])
+            sky_dims = ("time", "frequency", "polarization", "l", "m")
+            padded = np.zeros(lead + (nx, ny), dtype=np.float32)
+            padded[:, :, :, blc[0]:trc[0], blc[1]:trc[1]] = dat
+            out[name] = DataArray(padded, dims=sky_dims, coords={k: coords[k] for k in sky_dims})
+    return out
+
+
 def fft_to_uv(image, data_vars: Union[str, List[str]] = "") -> None:
     """standard_image_grid_preparation.py:312-341."""
     for sky_var in _vars(data_vars, ["SKY", "RESIDUAL", "MODEL"]):
@@ -34,7 +82,7 @@ def fft_to_uv(image, data_vars: Union[str, List[str]] = "") -> None:
             dat_name = "VISIBILITY_" + sky_var
             if dat_name not in image.data_vars:
                 raise Exception(f"{dat_name} is not in {image}")
-            image[dat_name].data[...] = fft_lm_to_uv(image[sky_var].data, axes=[3, 4])
+            image[dat_name].data = fft_lm_to_uv(image[sky_var].data, axes=[3, 4])
 
 
 def ifft_to_lm(image, data_vars: Union[str, List[str]] = "") -> None:
@@ -42,7 +90,7 @@ def ifft_to_lm(image, data_vars: Union[str, List[str]] = "") -> None:
     for vis_var in _vars(data_vars, ["VISIBILITY", "VISIBILITY_RESIDUAL", "VISIBILITY_MODEL"]):
         if vis_var in image.data_vars:
             dat_name = "SKY" if vis_var == "VISIBILITY" else vis_var.split("VISIBILITY_", 1)[1]
-            image[dat_name].data[...] = ifft_uv_to_lm(image[vis_var].data, axes=[3, 4])
+            image[dat_name].data = ifft_uv_to_lm(image[vis_var].data, axes=[3, 4])
 
 
 def correct_fft_to_uv(image, data_vars: Union[str, List[str]] = "") -> None:

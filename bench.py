@@ -67,47 +67,191 @@ def measured_traffic(kernel, visibilities, n_uv, mode, tile):
     return None
 
 
-def cpu_baseline(obs, uvw_d, vis_d, w_d, flag_d, n_uv, cell, cube, budget_s=12.0):
-    """Single-threaded FP64 oracle grid+degrid on a bounded sample: the first
-    rows (time-major) in the first 4 channels, grown until the pair takes a
-    few seconds.  Per-visibility cost does not depend on the channel, and 4
-    channels keep the FP64 host grid at 1 GB."""
+def host_threads():
+    """Host threads this process may use: OMP_NUM_THREADS when set (the GPU
+    hosts set it to their CPU share), else the CPUs it may run on, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def _oracle_pair(O, uvw, vis, w, fl, freq, n_uv, cell, mode):
+    """One FP64 oracle grid + degrid of the given sample; (t_grid, t_degrid)."""
+    t0 = time.perf_counter()
+    g, _ = O.standard_grid(vis, uvw, w, freq, n_uv, cell, chan_mode=mode)
+    t1 = time.perf_counter()
+    O.degrid(uvw, fl, freq, g, cell)
+    return t1 - t0, time.perf_counter() - t1
+
+
+def cpu_baseline(obs, uvw_d, vis_d, w_d, flag_d, n_uv, cell, cube, leg_s=3.0, reps=3):
+    """The FP64 C oracle (a loop-by-loop restatement of the reference's numba
+    kernels) timed on the host: grid + degrid of the first rows (time-major)
+    of the benchmark workload, the median of `reps` repetitions.
+
+    * one core: the first 4 channels, rows grown until one pair takes about
+      `leg_s` seconds (the reference's numba kernels are single-threaded);
+    * all host threads: one channel per thread (each its own cube plane, as
+      an OpenMP loop over planes would run), the same number of rows, the
+      channels' pairs run concurrently (the oracle's C calls release the
+      interpreter lock).
+    Per-visibility cost does not depend on the channel."""
+    import concurrent.futures as cf
+
     from oracle import oracle as O
 
     nt, nb, _ = obs.uvw.shape
     npol = vis_d.shape[2]
-    nch = min(4, vis_d.shape[1])
-    freq = obs.freq[:nch]
+    nchan = vis_d.shape[1]
     total_rows = nt * nb
+    mode = "cube" if cube else "continuum"
+
+    def sample(rows, c0, c1):
+        sel = slice(0, rows)
+        nch = c1 - c0
+        uvw = uvw_d[sel].cpu().numpy().reshape(1, rows, 3)
+        vis = vis_d[sel, c0:c1].cpu().numpy().astype(np.complex128).reshape(1, rows, nch, npol)
+        w = w_d[sel, c0:c1].cpu().numpy().astype(np.float64).reshape(1, rows, nch, npol)
+        fl = flag_d[sel, c0:c1].cpu().numpy().astype(bool).reshape(1, rows, nch, npol)
+        w[fl] = 0
+        return uvw, vis, w, fl, obs.freq[c0:c1]
+
+    # one core: size the sample, then time it
+    nch1 = min(4, nchan)
     rows = 20000
     while True:
         rows = min(rows, total_rows)
-        sel = slice(0, rows)
-        uvw = uvw_d[sel].cpu().numpy().reshape(1, rows, 3)
-        vis = vis_d[sel, :nch].cpu().numpy().astype(np.complex128).reshape(1, rows, nch, npol)
-        w = w_d[sel, :nch].cpu().numpy().astype(np.float64).reshape(1, rows, nch, npol)
-        fl = flag_d[sel, :nch].cpu().numpy().astype(bool).reshape(1, rows, nch, npol)
-        w[fl] = 0
-        t0 = time.perf_counter()
-        g, sw = O.standard_grid(vis, uvw, w, freq, n_uv, cell,
-                                chan_mode="cube" if cube else "continuum")
-        t1 = time.perf_counter()
-        O.degrid(uvw, fl, freq, g, cell)
-        t2 = time.perf_counter()
-        if (t2 - t0) > budget_s / 4 or rows * 4 > total_rows:
+        smp = sample(rows, 0, nch1)
+        tg, td = _oracle_pair(O, *smp[:4], smp[4], n_uv, cell, mode)
+        if tg + td > leg_s / 4 or rows * 4 > total_rows:
             break
         rows *= 4
-    tg, td = t1 - t0, t2 - t1
-    nvis = rows * nch * npol
+    rows = min(total_rows, int(rows * max(1.0, leg_s / max(tg + td, 1e-3))))
+    smp = sample(rows, 0, nch1)
+    times1 = [sum(_oracle_pair(O, *smp[:4], smp[4], n_uv, cell, mode)) for _ in range(reps)]
+    t1 = float(np.median(times1))
+    nvis1 = rows * nch1 * npol
+    del smp
+
+    # all threads: one channel each, the same rows
+    nthr = min(host_threads(), nchan)
+    chans = [sample(rows, c, c + 1) for c in range(nthr)]
+    with cf.ThreadPoolExecutor(max_workers=nthr) as ex:
+        def run_all():
+            t0 = time.perf_counter()
+            list(ex.map(lambda smp: _oracle_pair(O, *smp[:4], smp[4], n_uv, cell, mode), chans))
+            return time.perf_counter() - t0
+        timesn = [run_all() for _ in range(reps)]
+    tn = float(np.median(timesn))
+    nvisn = rows * nthr * npol
+    del chans
+    model = cpu_model()
     return {
-        "value": round(nvis / (tg + td) / 1e6, 3),
+        "value": round(nvisn / tn / 1e6, 3),
         "unit": "M visibilities/sec",
-        "cores": 1,
+        "cores": nthr,
         "kind": "port",
-        "sample": f"first {rows} of {total_rows} rows x first {nch} of {vis_d.shape[1]} "
-                  f"channels x {npol} pols = {nvis} visibilities, FP64 grid {tg:.2f} s + "
-                  f"degrid {td:.2f} s, one thread of {cpu_model()}",
+        "sample": f"first {rows} of {total_rows} rows x {nthr} of {nchan} channels x {npol} pols "
+                  f"= {nvisn} visibilities, FP64 grid + degrid, one channel (cube plane) per "
+                  f"thread on {nthr} threads, median of {reps}: {tn:.2f} s",
+        "single_core": {"value": round(nvis1 / t1 / 1e6, 3), "unit": "M visibilities/sec",
+                        "sample": f"first {rows} rows x first {nch1} channels x {npol} pols = "
+                                  f"{nvis1} visibilities, median of {reps}: {t1:.2f} s"},
+        "nproc": os.cpu_count(),
+        "threads_allowed": host_threads(),
+        "cpu_model": model,
     }
+
+
+def small_workload_leg(dev, reps=10):
+    """The 27-antenna workload (100 times x 4 channels x 2 pols = 280,800
+    visibilities, 256 x 256, 5" cells, cube) in full: the FP64 oracle on one
+    host core against the batched FP32 GPU path (plan + grid + degrid, HIP
+    events, median of `reps`), and the GPU results' errors against the
+    oracle: grid, dirty image, PSF, sum of weights, model visibilities."""
+    import torch
+
+    from astroviper_amd import synth
+    from astroviper_amd.device import GridPlan, dirty_image
+    from oracle import oracle as O
+
+    cfg = synth.CONFIG1
+    obs = synth.make_observation(cfg["array"], cfg["sky"])
+    vis, w, flag = synth.host_visibilities(obs, cfg["sky"], flag_frac=0.1, weight_range=(0.5, 1.5))
+    vis = vis.astype(np.complex64).astype(np.complex128)
+    w = w.astype(np.float32).astype(np.float64)
+    c = cfg["cell_arcsec"] * synth.ARCSEC
+    cell, n_uv = (-c, c), cfg["n_uv"]
+    nt, nb, nchan, npol = w.shape
+    rows, V = nt * nb, w.size
+    w0 = np.where(flag, 0.0, w)
+    tcpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        gref, swref = O.standard_grid(vis, obs.uvw, w0, obs.freq, n_uv, cell, chan_mode="cube")
+        mref = O.degrid(obs.uvw, flag, obs.freq, gref, cell)
+        tcpu.append(time.perf_counter() - t0)
+    pref, pswref = O.standard_grid(None, obs.uvw, w0, obs.freq, n_uv, cell, chan_mode="cube",
+                                   do_psf=True)
+    iref, psfref = O.normalise(gref, swref), O.normalise(pref, pswref)
+
+    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev)
+    uvw_t = torch.from_numpy(obs.uvw.reshape(rows, 3)).to(dev)
+    f_t = torch.from_numpy(obs.freq).to(dev)
+    v = torch.from_numpy(vis.reshape(rows * nchan, npol)).to(dev, torch.complex64)
+    wt = torch.from_numpy(w.reshape(rows * nchan, npol)).to(dev, torch.float32)
+    fl = torch.from_numpy(flag.reshape(rows * nchan, npol).astype(np.uint8)).to(dev)
+    g = torch.empty((nchan, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
+    sw = torch.empty((nchan, npol), dtype=torch.float64, device=dev)
+    m = torch.empty((rows * nchan, npol), dtype=torch.complex64, device=dev)
+
+    def step():
+        plan.set_uvw(uvw_t, f_t)
+        plan.grid(v, wt, fl, grid=g, sumwt=sw)
+        plan.degrid(g, fl, m)
+
+    step()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        step()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    tgpu = float(np.median(ts))
+    img = dirty_image(g, sw).cpu().numpy()
+    pg, psw = plan.grid(None, wt, fl, psf=True)
+    psf = dirty_image(pg, psw).cpu().numpy()
+    _, corr = O.create_prolate_spheroidal_kernel(100, 7, np.array(n_uv))
+    inner = corr >= 1e-2
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    def img_err(a, b):
+        d = np.abs(a - b)
+        return {"full": float(d.max() / np.abs(b).max()),
+                "taper_ge_1e-2": float(d[..., inner].max() / np.abs(b).max())}
+
+    # the GPU degrids its own FP32 grid; the oracle model is from the FP64 grid
+    mm = m.cpu().numpy().reshape(mref.shape)
+    plan.close()
+    return {"workload": f"27 antennas x {nt} times x {nchan} chan x {npol} pol = {V} vis, "
+                        f"{n_uv[0]}x{n_uv[1]} cube, 5\" cells, 10% flagged, weights U(0.5,1.5)",
+            "cpu_fp64_one_core": {"value": round(V / float(np.median(tcpu)) / 1e6, 3),
+                                  "unit": "M visibilities/sec", "median_s": float(np.median(tcpu))},
+            "gpu_fp32": {"value": round(V / (tgpu / 1e3) / 1e6, 1), "unit": "M visibilities/sec",
+                         "ms": round(tgpu, 4), "what": "plan + grid + degrid, median of events"},
+            "max_rel_err_vs_fp64_oracle": {
+                "grid": rel(g.cpu().numpy(), gref), "sumwt": rel(sw.cpu().numpy(), swref),
+                "dirty": img_err(img, iref), "psf": img_err(psf, psfref),
+                "degrid_from_own_grid": rel(mm, mref)}}
 
 
 def main():
@@ -119,6 +263,8 @@ def main():
     ap.add_argument("--tile", type=int, nargs=2, default=[16, 16])
     ap.add_argument("--unit-cap", type=int, default=0)
     ap.add_argument("--compensated", action="store_true")
+    ap.add_argument("--subtile", action="store_true",
+                    help="grid FP32 16x16 tiles with the sub-tile scheduled 4x4x1 kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -171,7 +317,8 @@ def main():
         plan.set_uvw(uvw, freq)
         if ev is not None:
             ev[1].record()
-        plan.grid(vis2, w2, fl2, grid=grid, sumwt=sumwt, compensated=args.compensated)
+        plan.grid(vis2, w2, fl2, grid=grid, sumwt=sumwt, compensated=args.compensated,
+                  subtile=args.subtile)
         if ev is not None:
             ev[2].record()
         plan.degrid(grid, fl2, model)
@@ -207,7 +354,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     tdom, bdom = kern[dom]
     achieved = bdom / (tdom / 1e3) / 1e9
-    names = {"grid": grid_kernel_name(args.tile, npol, compensated=args.compensated),
+    names = {"grid": grid_kernel_name(args.tile, npol, compensated=args.compensated,
+                                      subtile=args.subtile),
              "degrid": degrid_kernel_name(args.tile, npol, 7)}
     roofline = {"bound": "hbm", "kernel": names[dom],
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -255,13 +403,33 @@ def main():
         def relerr(a, b):
             return float((a - b).abs().max() / b.abs().max())
 
-        extra["max_rel_err_vs_fp64"] = {
+        errs = {
             "grid": relerr(grid.to(torch.complex128), g64),
             "sumwt": relerr(sumwt, sw64),
             "degrid_vs_fp64_grid": relerr(model.to(torch.complex128), m64),
         }
+        del m64
+        # dirty images and PSFs of the first image channel's planes (FFT in
+        # FP64 for both, as grid2image_spheroid_ms4 does)
+        from astroviper_amd.device import dirty_image, ps_correction_1d
+        tu = ps_correction_1d(n_uv[0], dev)
+        tv = ps_correction_1d(n_uv[1], dev)
+        inner = (tu[:, None] * tv[None, :]) >= 1e-2
+
+        def img_err(a, b):
+            d = (a - b).abs()
+            m = b.abs().max()
+            return {"full": float(d.max() / m), "taper_ge_1e-2": float(d[..., inner].max() / m)}
+
+        errs["dirty"] = img_err(dirty_image(grid[:1], sumwt[:1]), dirty_image(g64[:1], sw64[:1]))
+        del g64
+        p32, ps32 = plan.grid(None, w2, fl2, psf=True, subtile=args.subtile)
+        p64, ps64 = plan.grid(None, w2.to(torch.float64), fl2, psf=True)
+        errs["psf"] = img_err(dirty_image(p32[:1], ps32[:1]), dirty_image(p64[:1], ps64[:1]))
+        errs["images_of"] = f"image channel 0 ({npol} planes)"
+        extra["max_rel_err_vs_fp64"] = errs
         log(f"bench: errors vs FP64 {extra['max_rel_err_vs_fp64']}")
-        del g64, m64
+        del p32, p64
 
     if args.calibrate_copy > 0:
         src = torch.empty(args.calibrate_copy // 4, dtype=torch.float32, device=dev)
@@ -274,7 +442,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(obs, uvw, vis, w, flag, n_uv, cell, cube)
-        log(f"bench: cpu baseline {cpu['value']:.2f} M vis/s ({cpu['sample']})")
+        log(f"bench: cpu baseline {cpu['value']:.2f} M vis/s ({cpu['sample']}); one core "
+            f"{cpu['single_core']['value']:.2f} M vis/s")
+        extra["small_workload"] = small_workload_leg(dev)
+        log(f"bench: 27-antenna workload {extra['small_workload']}")
 
     out = {
         "metric": "M visibilities/sec (grid+degrid, support 7) batched FP32 vs plain GPU @<=1e-4 rel err",
@@ -294,7 +465,7 @@ def main():
                                f"{n_uv[0]}x{n_uv[1]} {args.mode}, support 7, oversampling 100, "
                                f"10% flagged",
                    "mode": args.mode, "tile": list(args.tile), "visibilities": V,
-                   "compensated": bool(args.compensated)},
+                   "compensated": bool(args.compensated), "subtile": bool(args.subtile)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -35,6 +35,9 @@ extern "C" {
 #define AV_GRID_COMPENSATED 4 /* FP32 only: compensated per-tile sums */
 #define AV_GRID_ADJOINT 8     /* weight each sample by 1/norm (adjoint of
                                  av_degrid); weights/flags are ignored */
+#define AV_GRID_SUBTILE 16    /* FP32 16 x 16 tiles: run the sub-tile
+                                 scheduled 4x4x1 matrix-core kernel instead of
+                                 the dense 16x16x4 one (same results) */
 /* av_degrid mode bits */
 #define AV_DEGRID_INCREMENTAL 1 /* add into vis_out instead of overwriting */
 
@@ -50,7 +53,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 4
+#define AV_ABI_VERSION 5
 
 const char *av_last_error(void);
 int av_version(void);
@@ -152,8 +155,12 @@ int av_grid(av_plan *plan, const void *vis, const void *weight,
  * zero dphase.  grid [n_img_chan][n_grid_pol][n_u][n_v] complex,
  * pol_map [n_pol] int32 grid polarisation of each visibility polarisation
  * (NULL = p % n_grid_pol), flag [rows][n_chan][n_pol] uint8 or NULL,
- * vis_out [rows][n_chan][n_pol] complex.  Flagged or off-grid samples are
- * left unchanged with AV_DEGRID_INCREMENTAL and set to 0 without it. */
+ * vis_out [rows][n_chan][n_pol] complex.  Flagged or off-grid samples, the
+ * polarisations whose pol_map entry lies outside [0, n_grid_pol) and
+ * samples whose kernel-weight sum is zero (a caller's table that vanishes
+ * over the footprint) are skipped as dgrid_numba skips them
+ * (standard_degrid.py:419-423,482): left unchanged with
+ * AV_DEGRID_INCREMENTAL and set to 0 without it. */
 int av_degrid(av_plan *plan, const void *grid, int32_t n_grid_pol,
               const int32_t *pol_map, const uint8_t *flag, int32_t n_pol,
               const double *cgk, void *vis_out, int32_t dtype, int32_t mode,
@@ -208,6 +215,74 @@ int av_normalise(const void *fft_planes, const double *sumwt,
                  const double *taper_u, const double *taper_v,
                  int64_t n_planes, int32_t n_u, int32_t n_v, int32_t dtype,
                  void *out, int32_t out_dtype, void *stream);
+
+/* ------------------------------------------------------- Hogbom CLEAN --
+ * core/imaging/deconvolvers/hogbom (the _hogbom_ext extension).  Images are
+ * [ny][nx] row-major of dtype AV_F32 or AV_F64; mask NULL = no mask (else a
+ * pixel is used where mask > 0.5). */
+
+/* Bytes of the device work buffer av_hogbom_clean needs (host output). */
+int av_hogbom_work_bytes(int64_t *bytes);
+
+/* maximg, src/hclean.cpp:51-81: out[0] = min, out[1] = max of the masked
+ * image, starting from +1e20 / -1e20; out is 2 device values of dtype. */
+int av_hogbom_maximg(const void *image, const void *mask, int32_t nx,
+                     int32_t ny, int32_t dtype, void *out, void *stream);
+
+/* clean, src/hclean.cpp:106-210, in place: model += components, residual -=
+ * gain * peak * psf (psf centred at (nx/2, ny/2)) over iterations start ..
+ * niter - 1 inside the clean box [xbeg, xend) x [ybeg, yend); siter is the
+ * reference's start iteration (the threshold is thres * 2^((iter -
+ * siter)/cspeedup) when cspeedup > 0).  (px0, py0) is the previous peak
+ * (0, 0 for a fresh run; a box whose pixels are all 0 keeps it).  With
+ * report_cycle > 0 the run also stops, before subtracting, at the first
+ * iteration other than skip_report that is siter or has iter % report_cycle
+ * == 1 -- where the reference calls its progress and stop callbacks -- so
+ * the caller can run them and resume from there.  One cooperative launch;
+ * work is a device buffer of av_hogbom_work_bytes() bytes whose header
+ * receives {uint32 barrier, uint32 error, int32 iter, px, py, reason,
+ * double peak}: reason 0 = threshold reached or niter done, 1 = report
+ * point, -1 = a workgroup barrier timed out (the results are then invalid).
+ * Calls sharing a work buffer must not overlap. */
+int av_hogbom_clean(void *model, void *residual, const void *psf,
+                    const void *mask, int32_t nx, int32_t ny, int32_t xbeg,
+                    int32_t xend, int32_t ybeg, int32_t yend, int32_t niter,
+                    int32_t siter, int32_t start, int32_t report_cycle,
+                    int32_t skip_report, double gain, double thres,
+                    double cspeedup, int32_t px0, int32_t py0, int32_t dtype,
+                    void *work, void *stream);
+
+/* ---------------------------------------------------- imaging weights --
+ * core/imaging/imaging_weighting/grid_imaging_weights.py.  uvw [n_rows][3]
+ * float64, freq [n_chan], cell in radians; a sample's pixel is int(x + 0.5)
+ * of its grid position x, truncated toward zero. */
+
+/* grid_imaging_weights_jit (:105-206): for every sample whose pixel lies
+ * inside the grid and whose weight (weight[(row * n_chan + chan) *
+ * weight_stride], NaN = skip) is not NaN, add the weight at the pixel and at
+ * the conjugate pixel (skipped where that lies outside the grid; the
+ * reference writes one past the end there) of grid [n_chan][n_u][n_v]
+ * float64, and 2 * weight to sum_weight [n_chan].  Accumulates. */
+int av_grid_imaging_weights(const double *uvw, int64_t n_rows,
+                            const double *freq, int32_t n_chan,
+                            const double *weight, int32_t weight_stride,
+                            int32_t n_u, int32_t n_v, double cell_u,
+                            double cell_v, double *grid, double *sum_weight,
+                            void *stream);
+
+/* degrid_imaging_weights_jit (:274-443), one polarisation: imaging_weight
+ * [n_rows][n_chan] = weight [n_rows][n_chan] where the pixel is inside the
+ * grid (0 elsewhere), divided by briggs[0][chan] * g + briggs[1][chan]
+ * (briggs [2][n_chan]) when the weight and the grid value g are finite and
+ * non-zero.  *nan_flag (device int32, set by the caller to 0) becomes
+ * non-zero when a Briggs factor used was NaN (the reference raises). */
+int av_degrid_imaging_weights(const double *grid, const double *briggs,
+                              const double *uvw, int64_t n_rows,
+                              const double *freq, int32_t n_chan,
+                              const double *weight, int32_t n_u, int32_t n_v,
+                              double cell_u, double cell_v,
+                              double *imaging_weight, int32_t *nan_flag,
+                              void *stream);
 
 #ifdef __cplusplus
 }

@@ -426,3 +426,157 @@ void ora_grid_adjoint(const double *uvw, const double *values, const uint8_t *fl
     }
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Hogbom CLEAN, deconvolvers/hogbom/src/hclean.cpp (the reference's own C++,
+ * a restatement of the AIPS++ Fortran).  Images are [ny][nx] row-major.
+ *
+ * ora_maximg_T: hclean.cpp:51-81 -- min and max over the (masked) image,
+ * starting from +-1e20.
+ * ora_hogbom_T: hclean.cpp:106-210 without the progress/stop callbacks:
+ * every iteration finds the first pixel of largest |residual| inside the
+ * box (row-major scan, strict >, so |r| = 0 or NaN never wins and the
+ * previous peak position is kept), stops when that is below the threshold
+ * (scaled by 2^((iter - siter)/cspeedup) when cspeedup > 0), adds
+ * gain * peak to the model and subtracts gain * peak * psf centred at
+ * (nx/2, ny/2) shifted to the peak.  Returns the last iteration number.
+ * ------------------------------------------------------------------------ */
+#define ORA_HOGBOM(T, SFX, ABS, POW)                                                    \
+  void ora_maximg_##SFX(const T *img, int domask, const T *mask, int nx, int ny,        \
+                        T *fmin, T *fmax) {                                             \
+    T lo = (T)1e20, hi = (T)-1e20;                                                      \
+    for (int iy = 0; iy < ny; ++iy)                                                     \
+      for (int ix = 0; ix < nx; ++ix)                                                   \
+        if (domask == 0 || mask[iy * nx + ix] > (T)0.5) {                               \
+          T w = img[iy * nx + ix];                                                      \
+          if (w > hi) hi = w;                                                           \
+          if (w < lo) lo = w;                                                           \
+        }                                                                               \
+    *fmin = lo;                                                                         \
+    *fmax = hi;                                                                         \
+  }                                                                                     \
+  int ora_hogbom_##SFX(T *model, T *resid, const T *psf, int domask, const T *mask,     \
+                       int nx, int ny, int xbeg, int xend, int ybeg, int yend,          \
+                       int niter, int siter, T gain, T thres, T cspeedup, int *peak) {  \
+    int px = 0, py = 0, iter;                                                           \
+    T maxval = (T)0;                                                                    \
+    for (iter = siter; iter < niter; ++iter) {                                          \
+      T absval = (T)0;                                                                  \
+      for (int iy = ybeg; iy < yend; ++iy)                                              \
+        for (int ix = xbeg; ix < xend; ++ix)                                            \
+          if (domask == 0 || mask[iy * nx + ix] > (T)0.5) {                             \
+            T v = ABS(resid[iy * nx + ix]);                                             \
+            if (v > absval) {                                                           \
+              px = ix;                                                                  \
+              py = iy;                                                                  \
+              absval = v;                                                               \
+            }                                                                           \
+          }                                                                             \
+      maxval = resid[py * nx + px];                                                     \
+      T cthres = cspeedup > (T)0 ? thres * POW((T)2, (T)(iter - siter) / cspeedup)     \
+                                 : thres;                                               \
+      if (absval < cthres) break;                                                       \
+      int x1 = px - nx / 2 > 0 ? px - nx / 2 : 0;                                       \
+      int y1 = py - ny / 2 > 0 ? py - ny / 2 : 0;                                       \
+      int x2 = px + nx / 2 - 1 < nx - 1 ? px + nx / 2 - 1 : nx - 1;                     \
+      int y2 = py + ny / 2 - 1 < ny - 1 ? py + ny / 2 - 1 : ny - 1;                     \
+      T pv = gain * maxval;                                                             \
+      model[py * nx + px] += pv;                                                        \
+      for (int iy = y1; iy <= y2; ++iy)                                                 \
+        for (int ix = x1; ix <= x2; ++ix) {                                             \
+          int qx = nx / 2 + ix - px, qy = ny / 2 + iy - py;                             \
+          if (qx < 0 || qx >= nx || qy < 0 || qy >= ny) continue;                       \
+          resid[iy * nx + ix] -= pv * psf[qy * nx + qx];                                \
+        }                                                                               \
+    }                                                                                   \
+    if (iter > niter) iter = niter;                                                     \
+    if (peak) {                                                                         \
+      peak[0] = px;                                                                     \
+      peak[1] = py;                                                                     \
+    }                                                                                   \
+    return iter;                                                                        \
+  }
+
+ORA_HOGBOM(double, f64, fabs, pow)
+ORA_HOGBOM(float, f32, fabsf, powf)
+
+/* ------------------------------------------------------------------------
+ * Imaging weights, core/imaging/imaging_weighting/grid_imaging_weights.py.
+ *
+ * ora_grid_imaging_weights: grid_imaging_weights_jit (:105-206).  For each
+ * (time, baseline, channel) in that order: u = uvw_u * -(f*dl*N)/c (likewise
+ * v); skip NaN; nearest pixel int(u + N//2 + 0.5) (truncation toward zero)
+ * and the conjugate int(-u + N//2 + 0.5); if the direct pixel is inside the
+ * grid and the weight (polarisation 0) is not NaN, add the weight at both
+ * pixels and 2 * weight to sum_weight.  The reference does not bound the
+ * conjugate pixel (for a sample on row or column 0 of an even grid it is N,
+ * one past the end); this restatement, like the GPU, skips such a
+ * conjugate update and keeps the rest.
+ * grid [n_chan][n_u][n_v], sum_weight [n_chan], weight [nt][nb][nchan][npol].
+ * ------------------------------------------------------------------------ */
+void ora_grid_imaging_weights(double *grid, double *sum_weight, const double *uvw, int nt,
+                              int nb, const double *freq, int nchan, const double *weight,
+                              int npol_w, int n_u, int n_v, double cell_u, double cell_v) {
+  const double c = 299792458.0;
+  const int cu = n_u / 2, cv = n_v / 2;
+  for (int t = 0; t < nt; ++t)
+    for (int b = 0; b < nb; ++b)
+      for (int ch = 0; ch < nchan; ++ch) {
+        const double su = -(freq[ch] * cell_u * (double)n_u) / c;
+        const double sv = -(freq[ch] * cell_v * (double)n_v) / c;
+        const int64_t row = (int64_t)t * nb + b;
+        const double u = uvw[row * 3] * su, v = uvw[row * 3 + 1] * sv;
+        if (isnan(u) || isnan(v)) continue;
+        const double up = u + (double)cu, vp = v + (double)cv;
+        const double upc = -u + (double)cu, vpc = -v + (double)cv;
+        if (!(fabs(up) < 1e9 && fabs(vp) < 1e9)) continue;
+        const int iu = (int)(up + 0.5), iv = (int)(vp + 0.5);
+        if (!(iu < n_u && iv < n_v && iu >= 0 && iv >= 0)) continue;
+        const double w = weight[(row * nchan + ch) * npol_w];
+        if (isnan(w)) continue;
+        double *g = grid + (int64_t)ch * n_u * n_v;
+        g[(int64_t)iu * n_v + iv] = g[(int64_t)iu * n_v + iv] + w;
+        const int iuc = (int)(upc + 0.5), ivc = (int)(vpc + 0.5);
+        if (iuc >= 0 && iuc < n_u && ivc >= 0 && ivc < n_v)
+          g[(int64_t)iuc * n_v + ivc] = g[(int64_t)iuc * n_v + ivc] + w;
+        sum_weight[ch] = sum_weight[ch] + 2.0 * w;
+      }
+}
+
+/* ora_degrid_imaging_weights: degrid_imaging_weights_jit (:274-443) with
+ * the single-polarisation map of degrid_imaging_weights (:260): imaging
+ * weight = data weight where the sample's pixel is inside the grid (0
+ * elsewhere, the output starts at zero), divided by briggs[0][ch] * g +
+ * briggs[1][ch] when the data weight is finite and non-zero and the grid
+ * value g at the pixel is finite and non-zero.  Returns 1 when a Briggs
+ * factor used is NaN (the reference raises ValueError), else 0. */
+int ora_degrid_imaging_weights(double *imaging_weight, const double *grid, const double *briggs,
+                               const double *uvw, int nt, int nb, const double *freq, int nchan,
+                               const double *weight, int n_u, int n_v, double cell_u,
+                               double cell_v) {
+  const double c = 299792458.0;
+  const int cu = n_u / 2, cv = n_v / 2;
+  for (int t = 0; t < nt; ++t)
+    for (int b = 0; b < nb; ++b)
+      for (int ch = 0; ch < nchan; ++ch) {
+        const double su = -(freq[ch] * cell_u * (double)n_u) / c;
+        const double sv = -(freq[ch] * cell_v * (double)n_v) / c;
+        const int64_t row = (int64_t)t * nb + b;
+        const double u = uvw[row * 3] * su, v = uvw[row * 3 + 1] * sv;
+        if (isnan(u) || isnan(v)) continue;
+        const double up = u + (double)cu, vp = v + (double)cv;
+        if (!(fabs(up) < 1e9 && fabs(vp) < 1e9)) continue;
+        const int iu = (int)(up + 0.5), iv = (int)(vp + 0.5);
+        if (!(iu < n_u && iv < n_v && iu >= 0 && iv >= 0)) continue;
+        const int64_t k = row * nchan + ch;
+        const double w = weight[k];
+        imaging_weight[k] = w;
+        if (isnan(w) || w == 0.0) continue;
+        const double g = grid[((int64_t)ch * n_u + iu) * n_v + iv];
+        if (isnan(g) || g == 0.0) continue;
+        if (isnan(briggs[ch]) || isnan(briggs[nchan + ch])) return 1;
+        const double denom = briggs[ch] * g + briggs[nchan + ch];
+        imaging_weight[k] = imaging_weight[k] / denom;
+      }
+  return 0;
+}

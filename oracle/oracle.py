@@ -333,3 +333,144 @@ def measure_adjoint(y, uvw, freq, n_uv, cell, nchan_g=1, npol_g=1, support=7, sa
     nn = n_uv[0] * n_uv[1]
     return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(g, axes=[2, 3]), axes=[2, 3]),
                            axes=[2, 3]) * nn / t
+
+
+# --------------------------------------------------------------------------
+# Hogbom CLEAN  (core/imaging/deconvolvers/hogbom: src/hclean.cpp and the
+# result dictionary of python/bindings.cpp:61-193)
+# --------------------------------------------------------------------------
+def _hogbom_lib(dtype):
+    L = lib()
+    sfx = "f32" if dtype == np.float32 else "f64"
+    f = getattr(L, f"ora_hogbom_{sfx}")
+    m = getattr(L, f"ora_maximg_{sfx}")
+    if f.argtypes is None:
+        T = ctypes.c_float if dtype == np.float32 else ctypes.c_double
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        f.argtypes = [P, P, P, I, P, I, I, I, I, I, I, I, I, T, T, T, P]
+        f.restype = I
+        m.argtypes = [P, I, P, I, I, P, P]
+        m.restype = None
+    return f, m
+
+
+def hogbom_maximg(image, mask=None):
+    """maximg (hclean.cpp:51-81): (min, max) over the masked image."""
+    img = np.ascontiguousarray(image)
+    dt = np.float32 if img.dtype == np.float32 else np.float64
+    img = img.astype(dt, copy=False)
+    ny, nx = img.shape
+    _, m = _hogbom_lib(dt)
+    mk = None if mask is None or np.asarray(mask).size == 0 else \
+        np.ascontiguousarray(mask, dtype=dt)
+    lo, hi = np.zeros(1, dt), np.zeros(1, dt)
+    m(img.ctypes.data, int(mk is not None), None if mk is None else mk.ctypes.data, nx, ny,
+      lo.ctypes.data, hi.ctypes.data)
+    return lo[0], hi[0]
+
+
+def hogbom_clean(dirty_image, psf, mask=None, clean_box=(-1, -1, -1, -1), max_iter=100,
+                 start_iter=0, gain=0.1, threshold=0.0, speedup=0.0):
+    """clean (hclean.cpp:106-210 through bindings.cpp:61-193, no callbacks):
+    the result dictionary of the reference's extension."""
+    d = np.ascontiguousarray(dirty_image)
+    dt = np.float32 if d.dtype == np.float32 else np.float64
+    resid = d.astype(dt, copy=True)
+    p = np.ascontiguousarray(psf, dtype=dt)
+    ny, nx = resid.shape
+    mk = None if mask is None or np.asarray(mask).size == 0 else \
+        np.ascontiguousarray(mask, dtype=dt)
+    xbeg, xend, ybeg, yend = 0, nx, 0, ny
+    if len(clean_box) == 4:
+        xbeg, xend, ybeg, yend = (int(v) for v in clean_box)
+        xbeg = 0 if xbeg == -1 else xbeg
+        xend = nx if xend == -1 else xend
+        ybeg = 0 if ybeg == -1 else ybeg
+        yend = ny if yend == -1 else yend
+        xbeg = max(0, min(xbeg, nx - 1))
+        xend = max(xbeg + 1, min(xend, nx))
+        ybeg = max(0, min(ybeg, ny - 1))
+        yend = max(ybeg + 1, min(yend, ny))
+    model = np.zeros((ny, nx), dt)
+    f, _ = _hogbom_lib(dt)
+    it = f(model.ctypes.data, resid.ctypes.data, p.ctypes.data, int(mk is not None),
+           None if mk is None else mk.ctypes.data, nx, ny, xbeg, xend, ybeg, yend,
+           int(max_iter), int(start_iter), dt(gain), dt(threshold), dt(speedup), None)
+    lo, hi = hogbom_maximg(resid, mk)
+    final_peak = max(abs(lo), abs(hi))
+    total = dt(0)
+    for v in np.abs(model).ravel():  # the binding's sequential sum (bindings.cpp:175-178)
+        total = dt(total + v)
+    return {"model_image": model, "residual_image": resid, "iterations_performed": it,
+            "final_peak": final_peak, "total_flux_cleaned": total,
+            "converged": bool(final_peak <= dt(threshold))}
+
+
+# --------------------------------------------------------------------------
+# Imaging weights (core/imaging/imaging_weighting)
+# --------------------------------------------------------------------------
+def grid_imaging_weights(grid, sum_weight, uvw, data_weight, freq_chan, n_uv, cell):
+    """grid_imaging_weights (grid_imaging_weights.py:7-100 ->
+    grid_imaging_weights_jit :105-206) in place: grid (n_chan, 1, n_u, n_v)
+    float64, sum_weight (n_chan, 1); data_weight (nt, nb, nchan, npol),
+    polarisation 0 is gridded."""
+    L = lib()
+    f = L.ora_grid_imaging_weights
+    if f.argtypes is None:
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        f.argtypes = [P, P, P, I, I, P, I, P, I, I, I, D, D]
+        f.restype = None
+    assert grid.dtype == np.float64 and grid.flags.c_contiguous
+    assert sum_weight.dtype == np.float64 and sum_weight.flags.c_contiguous
+    u = np.ascontiguousarray(uvw, dtype=np.float64)
+    w = np.ascontiguousarray(data_weight, dtype=np.float64)
+    fr = np.ascontiguousarray(freq_chan, dtype=np.float64)
+    nt, nb, nchan, npol = w.shape
+    f(grid.ctypes.data, sum_weight.ctypes.data, u.ctypes.data, nt, nb, fr.ctypes.data, nchan,
+      w.ctypes.data, npol, int(n_uv[0]), int(n_uv[1]), float(cell[0]), float(cell[1]))
+    return grid, sum_weight
+
+
+def degrid_imaging_weights(grid_imaging_weight, uvw, data_weight, briggs_factors, freq_chan,
+                           n_uv, cell):
+    """degrid_imaging_weights (grid_imaging_weights.py:209-271 ->
+    degrid_imaging_weights_jit :274-443): per-sample imaging weights of
+    shape data_weight (single polarisation, nt, nb, nchan, 1)."""
+    L = lib()
+    f = L.ora_degrid_imaging_weights
+    if f.argtypes is None:
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        f.argtypes = [P, P, P, P, I, I, P, I, P, I, I, D, D]
+        f.restype = I
+    w = np.ascontiguousarray(data_weight, dtype=np.float64)
+    nt, nb, nchan, npol = w.shape
+    assert npol == 1
+    out = np.zeros(w.shape, dtype=np.float64)
+    g = np.ascontiguousarray(grid_imaging_weight, dtype=np.float64)
+    bf = np.ascontiguousarray(np.asarray(briggs_factors, dtype=np.float64).reshape(2, -1)[:, :nchan])
+    u = np.ascontiguousarray(uvw, dtype=np.float64)
+    fr = np.ascontiguousarray(freq_chan, dtype=np.float64)
+    if f(out.ctypes.data, g.ctypes.data, bf.ctypes.data, u.ctypes.data, nt, nb, fr.ctypes.data,
+         nchan, w.ctypes.data, int(n_uv[0]), int(n_uv[1]), float(cell[0]), float(cell[1])):
+        raise ValueError("NaN in briggs_factors")
+    return out
+
+
+def calculate_briggs_params(grid_of_imaging_weights, sum_weight, imaging_weights_parms):
+    """briggs_weighting.py:1-31 (NumPy, as the reference)."""
+    if imaging_weights_parms["weighting"] == "briggs":
+        robust = imaging_weights_parms["robust"]
+        bf = np.ones((2,) + sum_weight.shape)
+        sq = np.sum(grid_of_imaging_weights ** 2, axis=(2, 3))
+        with np.errstate(invalid="ignore", divide="ignore"):
+            bf[0, :, :] = (np.square(5.0 * 10.0 ** (-robust)) / (sq / sum_weight))[None, None, :, :]
+    elif imaging_weights_parms["weighting"] == "briggs_abs":
+        robust = imaging_weights_parms["robust"]
+        bf = np.ones((2,) + sum_weight.shape)
+        bf[0, :, :] = bf[0, :, :] * np.square(robust)
+        bf[1, :, :] = bf[1, :, :] * 2.0 * np.square(imaging_weights_parms["briggs_abs_noise"])
+    else:
+        bf = np.zeros((2, 1, 1) + sum_weight.shape)
+        bf[0, :, :] = np.ones((1, 1, 1) + sum_weight.shape)
+    return bf

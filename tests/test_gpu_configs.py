@@ -1,0 +1,178 @@
+"""GPU parity on subsets of the two large workloads, through the exact kernel
+instantiations the benchmark and the precision sweep time.
+
+* 64 antennas (2016 baselines), 4 polarisations, 1" cells, 2048 x 2048 grid,
+  16 x 16 tiles, 10% flagged, weights U(0.5, 1.5): the benchmark's generator
+  (synth.CONFIG2) over 100 of its 1000 time samples and 2 of its 64 channels
+  (1.0 and 1.4 GHz), cube and continuum, FP32 and FP64.  These plans run
+  k_grid_mfma<float,4,1,1> / <double,4,1,1> and the v-half degridder
+  k_degrid_mfma (4 polarisations, 12 column blocks, 8 per half).
+* The same generator at 4096 x 4096 (the precision sweep's grid), 40 time
+  samples x 2 channels, cube: FP32, FP32 compensated and FP64.
+
+Every output the gridder produces is compared with the FP64 oracle (a
+loop-by-loop restatement of standard_grid_jit, standard_grid.py:246-432, and
+dgrid_numba, standard_degrid.py:388-485) on the same samples: the grid, the
+sum of weights, the dirty image and the PSF (standard_grid.py:693-708), and
+the degridded model.  The samples are generated as float32/complex64 values
+so the FP32 and FP64 runs grid identical inputs and share one oracle run.
+
+Bars (max |x - ref| / max |ref| per output): FP64 grids, sums of weights and
+models 1e-12, FP64 images 1e-9 (the taper correction magnifies FFT rounding
+up to 6e4 times at the image corners); FP32 grids, models and images 1e-4,
+the images over the full field (on these dense grids the FP32 rounding
+errors average out; the largest measured is 2e-5), and 1e-6 where the taper
+is at least 1e-2.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+IMG64 = 1e-9
+TOL32 = 1e-4
+INNER32 = 1e-6
+
+
+def rel(a, b):
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - b)) / (den if den > 0 else 1.0))
+
+
+def image_errors(img, ref):
+    """(full field, taper >= 1e-2) max errors relative to max |ref|."""
+    from oracle import oracle as O
+    _, corr = O.create_prolate_spheroidal_kernel(100, 7, np.array(img.shape[-2:]))
+    inner = corr >= 1e-2
+    den = np.max(np.abs(ref))
+    d = np.abs(img - ref)
+    return float(d.max() / den), float(d[..., inner].max() / den)
+
+
+def make_subset(n_time, chans, n):
+    """The benchmark generator restricted to `n_time` time samples (over the
+    full hour-angle range) and the channels `chans` of its 64."""
+    from astroviper_amd import synth
+    cfg = synth.CONFIG2
+    arr = synth.ArrayConfig(**{**cfg["array"].__dict__, "n_time": n_time})
+    obs = synth.make_observation(arr, cfg["sky"])
+    obs.freq = np.ascontiguousarray(obs.freq[list(chans)])
+    vis, w, flag = synth.host_visibilities(obs, cfg["sky"], cfg["flag_frac"],
+                                           cfg["weight_range"])
+    vis = vis.astype(np.complex64).astype(np.complex128)
+    w = w.astype(np.float32).astype(np.float64)
+    c = cfg["cell_arcsec"] * synth.ARCSEC
+    return dict(uvw=obs.uvw, freq=obs.freq, vis=vis, w=w, flag=flag, n_uv=(n, n),
+                cell=(-c, c))
+
+
+def oracle_outputs(oracle, d, mode):
+    w0 = d["w"].copy()
+    w0[d["flag"]] = 0
+    g, sw = oracle.standard_grid(d["vis"], d["uvw"], w0, d["freq"], d["n_uv"], d["cell"],
+                                 chan_mode=mode)
+    pg, psw = oracle.standard_grid(None, d["uvw"], w0, d["freq"], d["n_uv"], d["cell"],
+                                   chan_mode=mode, do_psf=True)
+    out = dict(grid=g, sumwt=sw, dirty=oracle.normalise(g, sw), psf=oracle.normalise(pg, psw),
+               psf_sumwt=psw)
+    return out
+
+
+def run_gpu(gpu, d, mode, prec, compensated=False, tile=(16, 16), subtile=False):
+    from astroviper_amd.device import (GridPlan, degrid_kernel_name, dirty_image,
+                                       grid_kernel_name)
+    nt, nb, nchan, npol = d["w"].shape
+    rows = nt * nb
+    cube = mode == "cube"
+    # FP32 without compensated sums runs the sub-tile kernel (4x4x1 products)
+    # when asked for, else the dense 16x16x4 kernel
+    sub = prec == 32 and not compensated and subtile
+    assert grid_kernel_name(tile, npol, fp64=prec == 64, compensated=compensated,
+                            subtile=subtile) == ("k_grid_sub" if sub else "k_grid_mfma")
+    assert degrid_kernel_name(tile, npol, 7) == "k_degrid_mfma"
+    plan = GridPlan(rows, nchan, nchan if cube else 1, d["n_uv"], d["cell"], 7, 100,
+                    tile=tile, device=gpu)
+    plan.set_uvw(torch.from_numpy(np.ascontiguousarray(d["uvw"].reshape(rows, 3))).to(gpu),
+                 torch.from_numpy(d["freq"]).to(gpu))
+    rdt = torch.float32 if prec == 32 else torch.float64
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    v = torch.from_numpy(d["vis"].reshape(rows * nchan, npol)).to(gpu, cdt)
+    w = torch.from_numpy(d["w"].reshape(rows * nchan, npol)).to(gpu, rdt)
+    fl = torch.from_numpy(d["flag"].reshape(rows * nchan, npol).astype(np.uint8)).to(gpu)
+    g, sw = plan.grid(v, w, fl, compensated=compensated, subtile=subtile)
+    dirty = dirty_image(g, sw)
+    pg, psw = plan.grid(None, w, fl, psf=True, compensated=compensated, subtile=subtile)
+    psf = dirty_image(pg, psw)
+    torch.cuda.synchronize()
+    st = plan.stats()
+    out = dict(grid=g.cpu().numpy(), sumwt=sw.cpu().numpy(), dirty=dirty.cpu().numpy(),
+               psf=psf.cpu().numpy(), psf_sumwt=psw.cpu().numpy(), plan=plan, flag=fl,
+               stats=st)
+    return out
+
+
+def check_outputs(got, ref, prec, label):
+    e_g = rel(got["grid"], ref["grid"])
+    e_s = rel(got["sumwt"], ref["sumwt"])
+    e_ps = rel(got["psf_sumwt"], ref["psf_sumwt"])
+    d_full, d_in = image_errors(got["dirty"], ref["dirty"])
+    p_full, p_in = image_errors(got["psf"], ref["psf"])
+    print(f"{label}: grid {e_g:.2e} sumwt {e_s:.2e} dirty {d_full:.2e} (taper>=1e-2 {d_in:.2e}) "
+          f"psf {p_full:.2e} (taper>=1e-2 {p_in:.2e})", flush=True)
+    assert e_s < 1e-12 and e_ps < 1e-12, label
+    if prec == 64:
+        assert e_g < TOL64, label
+        assert d_full < IMG64 and p_full < IMG64, label
+    else:
+        assert e_g < TOL32, label
+        assert d_full < TOL32 and p_full < TOL32, label
+        assert d_in < INNER32 and p_in < INNER32, label
+
+
+# ----------------------------------------------------- 2048^2, 4 pols
+@pytest.fixture(scope="module")
+def c2_subset(oracle):
+    d = make_subset(100, (0, 63), 2048)
+    refs = {m: oracle_outputs(oracle, d, m) for m in ("cube", "continuum")}
+    return d, refs
+
+
+@pytest.mark.parametrize("mode", ["cube", "continuum"])
+@pytest.mark.parametrize("prec,subtile", [(32, False), (32, True), (64, False)])
+def test_benchmark_workload_subset(gpu, oracle, c2_subset, mode, prec, subtile):
+    d, refs = c2_subset
+    ref = refs[mode]
+    got = run_gpu(gpu, d, mode, prec, subtile=subtile)
+    st = got["stats"]
+    assert st["grid_valid"] > 0.6 * st["samples"]
+    check_outputs(got, ref, prec, f"2048^2 {mode} FP{prec}{' subtile' if subtile else ''}")
+    # degrid the oracle's grid (rounded to the run's precision) with the
+    # benchmark's plan, against the oracle's degridder on that same grid
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    gin = ref["grid"].astype(np.complex64).astype(np.complex128) if prec == 32 else ref["grid"]
+    mref = oracle.degrid(d["uvw"], d["flag"], d["freq"], gin, d["cell"])
+    m = got["plan"].degrid(torch.from_numpy(gin).to(gpu, cdt), got["flag"]).cpu().numpy()
+    m = m.reshape(mref.shape)
+    e_m = rel(m, mref)
+    print(f"2048^2 {mode} FP{prec}: degrid {e_m:.2e}", flush=True)
+    assert e_m < (TOL32 if prec == 32 else TOL64)
+    assert np.all(m[d["flag"]] == 0)
+    got["plan"].close()
+
+
+# ------------------------------------------- 4096^2 precision variants
+@pytest.fixture(scope="module")
+def c5_subset(oracle):
+    d = make_subset(40, (0, 63), 4096)
+    return d, oracle_outputs(oracle, d, "cube")
+
+
+@pytest.mark.parametrize("variant", ["fp32", "fp32_compensated", "fp64"])
+def test_precision_sweep_workload_subset(gpu, oracle, c5_subset, variant):
+    d, ref = c5_subset
+    prec = 64 if variant == "fp64" else 32
+    got = run_gpu(gpu, d, "cube", prec, compensated=variant == "fp32_compensated")
+    check_outputs(got, ref, prec, f"4096^2 cube {variant}")
+    got["plan"].close()

@@ -1,8 +1,10 @@
 """GPU parity: libavgrid.so against the FP64 oracle on identical seeded inputs.
 
 Bars: integer geometry is exact (a misplaced sample shows up as an O(1)
-relative error); FP64 results within 1e-6 relative, FP32 within 1e-4
-relative (max |x - ref| / max |ref| per output).  Every test runs the native
+relative error); FP64 grids, sums of weights and models within 1e-12
+relative, FP64 images within 1e-9 (the taper correction magnifies FFT
+rounding up to 6e4 times at the image corners), FP32 within 1e-4 relative
+(max |x - ref| / max |ref| per output).  Every test runs the native
 kernels; nothing falls back to the CPU.
 """
 import math
@@ -15,7 +17,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 C = 299792458.0
-TOL64 = 1e-6
+TOL64 = 1e-12
+IMG64 = 1e-9
 TOL32 = 1e-4
 
 
@@ -53,12 +56,22 @@ def image_errors(img, ref, support=7):
     return float(d.max() / den), float(d[..., inner].max() / den)
 
 
+# FP32 full-field bar on the 27-antenna 256^2 images: the taper correction
+# divides by grdsf(1)^2 = 1.6e-5 at the corners, so one unit in the last
+# place of an FP32 grid value (6e-8 relative) becomes 3.7e-3 of the peak
+# there, whatever the accumulation order (rounding the exact FP64 grid to
+# complex64 alone gives 2.3e-4).  On these sparse grids a few such units
+# survive the FFT; on the dense 2048^2 and 4096^2 workloads they average
+# out and the full-field FP32 error is below 1e-4 (test_gpu_configs.py).
+FULL32_SPARSE = 1e-2
+
+
 def image_ok(img, ref, prec):
     full, inner = image_errors(img, ref)
     print(f"image error: full field {full:.2e}, taper >= 1e-2 {inner:.2e}")
     if prec == 32:
-        return inner < TOL32 and full < 3e-3
-    return full < TOL64
+        return inner < TOL32 and full < FULL32_SPARSE
+    return full < IMG64
 
 
 def c1_data(dense=False, flag_frac=0.0, weights=None, seed=1):
@@ -122,7 +135,8 @@ def test_ps_tables_match_oracle(gpu, oracle):
 @pytest.mark.parametrize("cube", [False, True])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("dense", [False, True])
-def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
+@pytest.mark.parametrize("tile", [(32, 32), (16, 16)])
+def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense, tile):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=dense, flag_frac=0.1,
                                                   weights=(0.5, 1.5))
     vis, w = as_prec(prec, vis, w)
@@ -130,7 +144,14 @@ def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense):
     w0[flag] = 0
     gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell,
                                        chan_mode="cube" if cube else "continuum")
-    g, sw, plan = device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec)
+    g, sw, plan = device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, tile=tile)
+    from astroviper_amd.device import grid_kernel_name
+    # FP32 and FP64 16 x 16 and FP32 32 x 32 tiles run the dense matrix-core
+    # kernel, FP64 32 x 32 at 2 pols the LDS kernel
+    kname = grid_kernel_name(tile, 2, fp64=prec == 64)
+    expect = {(32, (16, 16)): "k_grid_mfma", (32, (32, 32)): "k_grid_mfma",
+              (64, (16, 16)): "k_grid_mfma", (64, (32, 32)): "k_grid_tiles"}[(prec, tile)]
+    assert kname == expect
     st = plan.stats()
     assert st["grid_valid"] > 1000
     tol = TOL32 if prec == 32 else TOL64
@@ -474,3 +495,81 @@ def test_misaligned_inputs(gpu):
     assert rel(g1.cpu().numpy(), g0.cpu().numpy()) < 1e-6
     assert rel(s1.cpu().numpy(), s0.cpu().numpy()) < 1e-12
     assert torch.equal(m1, m0)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_degrid_skips_out_of_range_pol_and_zero_norm(gpu, oracle, prec):
+    """dgrid_numba skips a polarisation whose polmap entry is outside the
+    grid (standard_degrid.py:419-420) and adds nothing when the kernel-weight
+    sum is zero (:482).  Through av_degrid directly: such values are 0 when
+    overwriting and untouched when incremental; the other polarisation
+    matches the oracle."""
+    from astroviper_amd.device import GridPlan
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
+    rows, nchan = uvw.shape[0] * uvw.shape[1], freq.size
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    tol = TOL32 if prec == 32 else TOL64
+    rng = np.random.default_rng(21)
+    grid = rng.standard_normal((1, 1) + n_uv) + 1j * rng.standard_normal((1, 1) + n_uv)
+    grid = as_prec(prec, grid)
+    from astroviper_amd.device import degrid_kernel_name
+    assert degrid_kernel_name((16, 16), 2) == "k_degrid_mfma"
+    assert degrid_kernel_name((8, 64), 2) == "k_degrid_tiles"
+    for tile in ((16, 16), (8, 64)):  # the matrix-core and the LDS degridder
+        plan = GridPlan(rows, nchan, 1, n_uv, cell, tile=tile, device=gpu)
+        plan.set_uvw(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu),
+                     torch.from_numpy(freq).to(gpu))
+        gd = torch.from_numpy(grid).to(gpu, cdt)
+        pm = torch.tensor([0, 3], dtype=torch.int32)
+        m = plan.degrid(gd, n_pol=2, pol_map=pm).cpu().numpy()
+        assert np.all(m[:, 1] == 0)
+        vals = np.zeros((uvw.shape[0], uvw.shape[1], nchan, 2), dtype=np.complex128)
+        flag0 = np.zeros(vals.shape, bool)
+        sc = np.array([-cell[0] * n_uv[0], -cell[1] * n_uv[1], 0.0])
+        of = np.array([n_uv[0] / 2, n_uv[1] / 2, 0.0])
+        conv = oracle.create_prolate_spheroidal_kernel_1D(100, 7)
+        oracle.dgrid(uvw, vals, flag0, sc, of, grid, freq, 7, 100, conv,
+                     np.zeros(nchan, dtype=np.int64), np.array([0, 3], dtype=np.int64))
+        assert np.all(vals[..., 1] == 0)
+        assert rel(m[:, 0], vals[..., 0].reshape(-1)) < tol
+        start = torch.full((rows * nchan, 2), 1.5 + 0.5j, dtype=cdt, device=gpu)
+        m2 = plan.degrid(gd, vis_out=start, n_pol=2, pol_map=pm, incremental=True).cpu().numpy()
+        assert np.all(m2[:, 1] == np.asarray(1.5 + 0.5j, dtype=m2.dtype))
+        # a kernel table that vanishes everywhere: nothing is added
+        zero_tab = torch.zeros_like(plan.cgk)
+        keep = plan.cgk
+        plan.cgk = zero_tab
+        m3 = plan.degrid(gd, n_pol=2).cpu().numpy()
+        assert np.all(m3 == 0) and not np.isnan(m3).any()
+        start = torch.full((rows * nchan, 2), 2.0 - 1.0j, dtype=cdt, device=gpu)
+        m4 = plan.degrid(gd, vis_out=start, n_pol=2, incremental=True).cpu().numpy()
+        assert np.all(m4 == np.asarray(2.0 - 1.0j, dtype=m4.dtype))
+        plan.cgk = keep
+        plan.close()
+
+
+def test_plain_gridder_sumwt_reduced_per_wavefront(gpu, oracle):
+    """The plain gridder adds each wavefront's sum of weights per
+    polarisation with one atomic; the sums still equal the oracle's, for
+    1, 2 and 4 polarisations and a ragged last wavefront."""
+    from astroviper_amd.device import grid_plain
+    rng = np.random.default_rng(22)
+    for npol in (1, 2, 4):
+        nt, nb, nchan, n = 3, 97, 2, 128
+        uvw = rng.uniform(-1500, 1500, size=(nt, nb, 3))
+        freq = np.linspace(1.0e9, 1.2e9, nchan)
+        cell = (-0.5 * C / freq[-1] / 2600.0, 0.5 * C / freq[-1] / 2600.0)
+        vis = rng.standard_normal((nt, nb, nchan, npol)) + 1j * rng.standard_normal((nt, nb, nchan, npol))
+        w = rng.uniform(0.5, 1.5, size=(nt, nb, nchan, npol))
+        flag = rng.random(w.shape) < 0.2
+        w0 = np.where(flag, 0.0, w)
+        gref, swref = oracle.standard_grid(vis, uvw, w0, freq, (n, n), cell, chan_mode="cube")
+        rows = nt * nb
+        g, sw = grid_plain(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu),
+                           torch.from_numpy(freq).to(gpu),
+                           torch.from_numpy(vis.reshape(rows, nchan, npol)).to(gpu),
+                           torch.from_numpy(w.reshape(rows, nchan, npol)).to(gpu),
+                           torch.from_numpy(flag.reshape(rows, nchan, npol)).to(gpu), (n, n),
+                           cell, nchan, list(range(nchan)))
+        assert rel(sw.cpu().numpy(), swref) < 1e-12, npol
+        assert rel(g.cpu().numpy(), gref) < TOL64, npol

@@ -17,6 +17,11 @@ C = 299792458.0
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")
 
 
+def rel(a, b):
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - b)) / (den if den > 0 else 1.0))
+
+
 def gold(name):
     with open(GOLDEN) as f:
         return json.load(f)[name]
@@ -232,3 +237,201 @@ def test_fft_round_trip_and_known_answers(gpu):
     vis = np.zeros((200, 200), complex)
     vis[100, 100] = 10
     assert abs(ifft_uv_to_lm(vis, (0, 1)).max() - g["delta_10_max"]) < 1e-15
+
+
+# ------------------------------------ xradio-image gridding (A12 mirror)
+def lm_image(nx=200, ny=200, nchan=1, npol=1, ntime=1, which="RESIDUAL", cell=4.8e-6,
+             value=1.0):
+    """test_standard_image_grid_preparation.py:17-65: a sky image of ones
+    over (time, frequency, polarization, l, m), 4.8e-6 rad cells."""
+    from astroviper_amd.msv4 import DataArray, ImageDataset
+    coords = {"time": 5.7e4 + np.arange(ntime), "frequency": np.arange(1, nchan + 1) * 1e8 + 1e9,
+              "polarization": np.array(["RR", "LL", "LR", "RL"])[:npol],
+              "l": (np.arange(nx) - nx // 2) * -cell, "m": (np.arange(ny) - ny // 2) * cell}
+    im = ImageDataset({}, coords)
+    dims = ("time", "frequency", "polarization", "l", "m")
+    im[which] = DataArray(np.full((ntime, nchan, npol, nx, ny), value, dtype=np.float32),
+                          dims=dims, coords={k: coords[k] for k in dims})
+    return im
+
+
+def test_fft_to_uv_known_answers(gpu):
+    """test_standard_image_grid_preparation.py:127-135: the FFT of a 200 x
+    200 image of ones has DC 200*200, and 1.0 after correct_fft_to_uv; the
+    pixel off the origin stays 0."""
+    from astroviper_amd.standard_image_grid_preparation import (correct_fft_to_uv, fft_to_uv,
+                                                                make_empty_padded_uv_image)
+    a = lm_image()
+    b = make_empty_padded_uv_image(a, image_size=[a.sizes["l"], a.sizes["m"]])
+    assert b["VISIBILITY_RESIDUAL"].shape == (1, 1, 1, 200, 200)
+    fft_to_uv(b)
+    assert b["VISIBILITY_RESIDUAL"].sel(u=0, v=0).values[0] == 200 * 200
+    correct_fft_to_uv(b)
+    assert b["VISIBILITY_RESIDUAL"].sel(u=0, v=0).values[0] == 1.0
+    assert b["VISIBILITY_RESIDUAL"].isel(u=99, v=99).values[0] == 0.0
+    # padding keeps the image centred and the DC term
+    c = make_empty_padded_uv_image(a, image_size=[256, 256])
+    assert c["RESIDUAL"].shape == (1, 1, 1, 256, 256)
+    assert c["RESIDUAL"].data[0, 0, 0, 28:228, 28:228].min() == 1.0
+    assert c["RESIDUAL"].data.sum() == 200 * 200
+    fft_to_uv(c)
+    assert c["VISIBILITY_RESIDUAL"].sel(u=0, v=0).values[0] == 200 * 200
+    with pytest.raises(Exception, match="larger than requested"):
+        make_empty_padded_uv_image(a, image_size=[100, 100])
+
+
+def test_ifft_to_lm_known_answer(gpu):
+    """test_standard_image_grid_preparation.py:137-142: a delta of 10 at the
+    uv origin transforms to a flat image of 10/200/200."""
+    from astroviper_amd.standard_image_grid_preparation import (ifft_to_lm,
+                                                                make_empty_padded_uv_image)
+    a = lm_image()
+    b = make_empty_padded_uv_image(a, image_size=[a.sizes["l"], a.sizes["m"]])
+    b["VISIBILITY_RESIDUAL"].sel(u=0, v=0).values[0] = 10
+    ifft_to_lm(b)
+    r = b["RESIDUAL"].data
+    assert abs(r.max() - 10.0 / 200.0 / 200.0) < 1e-10
+    assert abs(r.min() - 10.0 / 200.0 / 200.0) < 1e-10
+    # the result replaces the variable's data (a real image)
+    assert not np.iscomplexobj(r)
+
+
+def test_fft_to_uv_replaces_data(gpu):
+    """fft_to_uv assigns the transform to the variable (the reference's
+    `.data = `), so a complex result is kept even where the target held
+    real values, and the variable takes the transform's dtype."""
+    from astroviper_amd.msv4 import DataArray
+    from astroviper_amd.standard_image_grid_preparation import fft_to_uv
+    a = lm_image(nx=64, ny=64, which="SKY", value=0.0)
+    a["SKY"].data[0, 0, 0, 10, 20] = 1.0
+    dims = ("time", "frequency", "polarization", "l", "m")
+    a["VISIBILITY_SKY"] = DataArray(np.zeros((1, 1, 1, 64, 64)), dims=dims)
+    fft_to_uv(a, "SKY")
+    v = a["VISIBILITY_SKY"].data
+    assert np.iscomplexobj(v)
+    ref = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a["SKY"].data.astype(np.float64),
+                                                       axes=[3, 4]), axes=[3, 4]), axes=[3, 4])
+    assert np.max(np.abs(v - ref)) < 1e-5
+    assert np.abs(v.imag).max() > 0.5
+
+
+def uv_image_for(ms, n, cell, ntime=2, data_var="VISIBILITY"):
+    """An image dataset with l/m and u/v coordinates for gridding `ms`
+    (cube: one plane per channel) at `n` x `n`, `ntime` time planes."""
+    from astroviper_amd.msv4 import DataArray, ImageDataset
+    nchan = len(ms.coords["frequency"].values)
+    npol = len(ms.coords["polarization"].values)
+    coords = {"time": np.arange(ntime, dtype=float), "frequency": ms.coords["frequency"].values,
+              "polarization": ms.coords["polarization"].values,
+              "l": (np.arange(n) - n // 2) * -cell, "m": (np.arange(n) - n // 2) * cell,
+              "u": (np.arange(n) - n // 2) / (n * cell), "v": (np.arange(n) - n // 2) / (n * cell)}
+    im = ImageDataset({}, coords)
+    uvd = ("time", "frequency", "polarization", "u", "v")
+    skd = ("time", "frequency", "polarization", "l", "m")
+    im[data_var] = DataArray(np.zeros((ntime, nchan, npol, n, n), np.complex64), dims=uvd)
+    im[data_var + "_NORMALIZATION"] = DataArray(np.zeros((ntime, nchan, npol)), dims=uvd[:3])
+    im["SKY"] = DataArray(np.zeros((ntime, nchan, npol, n, n), np.float32), dims=skd)
+    return im
+
+
+def test_grid2xradio_matches_oracle(gpu, oracle):
+    """grid2xradio_spheroid_ms4 (standard_image_grid_preparation.py:495-580)
+    grids every dataset of the list into every time plane of the complex64
+    uv image and its normalisation; ifft_to_lm then correct_ifft_to_lm with
+    doSpheroidCorr (:412-492) give the dirty image of
+    grid2image_spheroid_ms4.  Compared with the FP64 oracle at the FP32
+    bars (the uv image is complex64, so gridding and the FFT run in FP32)."""
+    from astroviper_amd.standard_image_grid_preparation import (correct_ifft_to_lm,
+                                                                grid2xradio_spheroid_ms4,
+                                                                ifft_to_lm)
+    cell, a = cube_dataset(nchan=3, seed=4)
+    _, b = cube_dataset(nchan=3, seed=5)
+    rng = np.random.default_rng(13)
+    for ds in (a, b):
+        ds.VISIBILITY.data[...] = (rng.standard_normal(ds.VISIBILITY.data.shape)
+                                   + 1j * rng.standard_normal(ds.VISIBILITY.data.shape))
+        ds.WEIGHT.data[...] = rng.uniform(0.5, 1.5, ds.WEIGHT.data.shape)
+        ds.FLAG.data[...] = rng.random(ds.FLAG.data.shape) < 0.1
+    uvw = np.concatenate([a.UVW.data, b.UVW.data], axis=1)
+    vis = np.concatenate([a.VISIBILITY.data, b.VISIBILITY.data], axis=1)
+    w = np.concatenate([a.WEIGHT.data, b.WEIGHT.data], axis=1)
+    flag = np.concatenate([a.FLAG.data, b.FLAG.data], axis=1)
+    freq = a.coords["frequency"].values
+    n = 200
+    w0 = np.where(flag, 0.0, w)
+    gref, swref = oracle.standard_grid(vis, uvw, w0, freq, (n, n), (cell, cell),
+                                       chan_mode="cube")
+    im = uv_image_for(a, n, cell)
+    grid2xradio_spheroid_ms4([a, b], im, data_var="VISIBILITY", chan_mode="cube")
+    # flagged weights are zeroed in the caller's datasets, as in the reference
+    assert np.all(a.WEIGHT.data[a.FLAG.data] == 0)
+    # a complex64 uv image grids in FP32, so the weights are rounded to
+    # float32 on the way in (the sums of weights are still formed in FP64)
+    for t in range(2):
+        e = rel(im["VISIBILITY"].data[t], gref)
+        assert e < 1e-4, (t, e)
+        np.testing.assert_allclose(im["VISIBILITY_NORMALIZATION"].data[t], swref, rtol=1e-6)
+    ifft_to_lm(im, "VISIBILITY")
+    correct_ifft_to_lm(im, "SKY", doSpheroidCorr=True)
+    iref = oracle.normalise(gref, swref)
+    _, corr = oracle.create_prolate_spheroidal_kernel(100, 7, np.array([n, n]))
+    inner = corr >= 1e-2
+    for t in range(2):
+        d = np.abs(im["SKY"].data[t] - iref)
+        full, inn = d.max() / np.abs(iref).max(), d[..., inner].max() / np.abs(iref).max()
+        print(f"grid2xradio dirty plane {t}: full field {full:.2e}, taper >= 1e-2 {inn:.2e}")
+        assert inn < 1e-4 and full < 1e-2
+    # UV_SAMPLING grids the weights (the PSF grid)
+    pim = uv_image_for(a, n, cell, ntime=1, data_var="UV_SAMPLING")
+    grid2xradio_spheroid_ms4([a, b], pim, data_var="UV_SAMPLING", chan_mode="cube")
+    pref, pswref = oracle.standard_grid(None, uvw, w0, freq, (n, n), (cell, cell),
+                                        chan_mode="cube", do_psf=True)
+    assert rel(pim["UV_SAMPLING"].data[0], pref) < 1e-4
+    np.testing.assert_allclose(pim["UV_SAMPLING_NORMALIZATION"].data[0], pswref, rtol=1e-6)
+    # continuum into a one-channel image, and the errors of the reference
+    cim = uv_image_for(a, n, cell, ntime=1)
+    cim["VISIBILITY"].data = np.zeros((1, 1, 2, n, n), np.complex64)
+    cim["VISIBILITY_NORMALIZATION"].data = np.zeros((1, 1, 2))
+    grid2xradio_spheroid_ms4(a, cim, data_var="VISIBILITY", chan_mode="continuum")
+    cref, _ = oracle.standard_grid(a.VISIBILITY.data, a.UVW.data, a.WEIGHT.data, freq, (n, n),
+                                   (cell, cell), chan_mode="continuum")
+    assert rel(cim["VISIBILITY"].data[0], cref) < 1e-4
+    with pytest.raises(Exception, match="VISIBILITY_MODEL is not in"):
+        grid2xradio_spheroid_ms4(a, cim, data_var="VISIBILITY_MODEL")
+    with pytest.raises(TypeError, match="not an xarray datatree"):
+        grid2xradio_spheroid_ms4([a, "x"], cim)
+    nosw = uv_image_for(a, n, cell, ntime=1)
+    del nosw.data_vars["VISIBILITY_NORMALIZATION"]
+    with pytest.raises(Exception, match="Do not have VISIBILITY_NORMALIZATION"):
+        correct_ifft_to_lm(nosw, "SKY", doSpheroidCorr=True)
+
+
+def test_correct_ifft_to_lm_plain_scaling(gpu):
+    """correct_ifft_to_lm without doSpheroidCorr multiplies by nx*ny."""
+    from astroviper_amd.standard_image_grid_preparation import correct_ifft_to_lm
+    a = lm_image(nx=32, ny=48, which="SKY", value=0.5)
+    correct_ifft_to_lm(a)
+    assert np.all(a["SKY"].data == 0.5 * 32 * 48)
+
+
+def test_make_psf_values_match_oracle(gpu, oracle):
+    """make_psf (make_point_spread_function.py:1-79) returns the PSF of
+    grid2image_spheroid_ms4(dopsf=True) with pixelincr (-cell_l, cell_m):
+    its values equal the oracle's grid2image(do_psf=True) on the same
+    weights and flags."""
+    from astroviper_amd.make_point_spread_function import make_psf
+    cell, ms = cube_dataset(nchan=3, seed=7)
+    rng = np.random.default_rng(17)
+    ms.WEIGHT.data[...] = rng.uniform(0.5, 1.5, ms.WEIGHT.data.shape)
+    ms.FLAG.data[...] = rng.random(ms.FLAG.data.shape) < 0.15
+    w = ms.WEIGHT.data.copy()
+    flag = ms.FLAG.data.copy()
+    psf = make_psf(ms, {"cell_size": (cell, cell), "image_size": (200, 200),
+                        "phase_center": (0.0, 0.5), "chan_mode": "cube"},
+                   {"sampling": 100, "support": 7})
+    p = psf["POINT_SPREAD_FUNCTION"].data
+    ref, _, _ = oracle.grid2image(None, ms.UVW.data, w, flag, ms.coords["frequency"].values,
+                                  (200, 200), (-cell, cell), chan_mode="cube", do_psf=True)
+    assert p.shape == (1,) + ref.shape
+    assert np.max(np.abs(p[0] - ref)) <= 1e-9 * np.max(np.abs(ref))
+    assert np.all(np.abs(p[0, :, :, 100, 100] - 1 / 0.9999996674 ** 2) < 1e-6)

@@ -343,6 +343,34 @@ def test_degrid_flags_and_nan_rows(oracle):
     np.testing.assert_array_equal(m2[mask], m[mask])
 
 
+def test_degrid_skips_zero_norm_and_foreign_pols(oracle):
+    """dgrid_numba adds nvalue / norm only when norm != 0
+    (standard_degrid.py:482-485) and skips a polarisation whose polmap entry
+    is outside the grid (:419-420): with a kernel table that is zero on the
+    samples' footprints the values stay as they were (no NaN), and a pol
+    mapped to plane 5 of a 1-plane grid is never written.  The second pol
+    (mapped in range) matches the brute-force sum through the same table."""
+    rng = np.random.default_rng(23)
+    uvw, freq, cell, vis, w, flag, n_uv = _random_problem(rng)
+    grid = rng.standard_normal((1, 1) + n_uv) + 1j * rng.standard_normal((1, 1) + n_uv)
+    nt, nb, nf = uvw.shape[0], uvw.shape[1], freq.size
+    sc = np.array([-cell[0] * n_uv[0], -cell[1] * n_uv[1], 0.0])
+    of = np.array([n_uv[0] / 2, n_uv[1] / 2, 0.0])
+    start = np.full((nt, nb, nf, 2), 0.25 - 0.5j)
+    vals = start.copy()
+    zero = np.zeros(400)
+    oracle.dgrid(uvw, vals, np.zeros(vals.shape, bool), sc, of, grid, freq, 7, 100, zero,
+                 np.zeros(nf, np.int64), np.array([0, 0], np.int64))
+    np.testing.assert_array_equal(vals, start)
+    vals = np.zeros((nt, nb, nf, 2), complex)
+    conv = oracle.create_prolate_spheroidal_kernel_1D(100, 7)
+    oracle.dgrid(uvw, vals, np.zeros(vals.shape, bool), sc, of, grid, freq, 7, 100, conv,
+                 np.zeros(nf, np.int64), np.array([5, 0], np.int64))
+    assert np.all(vals[..., 0] == 0)
+    ref = oracle.degrid(uvw, np.zeros((nt, nb, nf, 1), bool), freq, grid, cell)
+    np.testing.assert_array_equal(vals[..., 1], ref[..., 0])
+
+
 def test_fft_known_answers(oracle):
     """test_fft_ifft.py:9-24 and test_standard_image_grid_preparation.py:127-142."""
     sky = np.zeros((128, 128))
