@@ -53,7 +53,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 5
+#define AV_ABI_VERSION 6
 
 const char *av_last_error(void);
 int av_version(void);
@@ -165,6 +165,25 @@ int av_degrid(av_plan *plan, const void *grid, int32_t n_grid_pol,
               const int32_t *pol_map, const uint8_t *flag, int32_t n_pol,
               const double *cgk, void *vis_out, int32_t dtype, int32_t mode,
               void *stream);
+
+/* av_plan_create + av_plan_set_uvw + av_grid in one call that allocates no
+ * device memory (no counterpart in the reference; for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The plan's arrays live in a static 32 MiB device
+ * array of the library, one per GPU, held until the process ends; calls
+ * whose plan needs more are refused with AV_ERR_INVALID (use
+ * av_plan_create).  Arguments as av_plan_create (reference geometry, default
+ * unit cap), av_plan_set_uvw (chan_map, no row_chan) and av_grid.  Every
+ * call on a GPU reuses that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race. */
+int av_grid_noalloc(const double *uvw, int64_t n_rows, const double *freq,
+                    int32_t n_chan, const int32_t *chan_map, int32_t n_img_chan,
+                    int32_t n_u, int32_t n_v, double cell_u, double cell_v,
+                    int32_t support, int32_t oversampling, int32_t tile_u,
+                    int32_t tile_v, const void *vis, const void *weight,
+                    const uint8_t *flag, int32_t n_pol, const double *cgk,
+                    void *grid, double *sumwt, int32_t dtype, int32_t mode,
+                    void *stream);
 
 /* ------------------------------------------------- plain GPU baseline --
  * One thread per sample of ONE visibility channel, global atomics, no
