@@ -27,6 +27,7 @@ AV_GRID_ACCUMULATE = 2
 AV_GRID_COMPENSATED = 4
 AV_GRID_ADJOINT = 8
 AV_GRID_SUBTILE = 16
+AV_GRID_F32_PRODUCTS = 32
 AV_DEGRID_INCREMENTAL = 1
 
 AV_GEOM_REFERENCE = 0

@@ -438,8 +438,10 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int ov, int support, int tile
 // Fill the padded table: rows -1 .. S from the call's table, zeros elsewhere.
 // The zero rows are plain stores; only the call's rows are loaded, K per
 // thread in flight (one round trip for the common tables).
+// `scale` (a power of two, so exact) multiplies every copied weight.
 template <typename R>
-__device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P) {
+__device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P,
+                                        R scale = (R)1) {
   const R *src;
   if constexpr (sizeof(R) == 4)
     src = P.wtab32;
@@ -456,7 +458,7 @@ __device__ inline void copy_wtab_padded(R *w, const MfmaLds &L, const PlanDev &P
     for (int k = 0; k < K; ++k) v[k] = src[min(i0 + k * (int)blockDim.x, nsrc - 1)];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (i0 + k * (int)blockDim.x < nsrc) w[lo + i0 + k * blockDim.x] = v[k];
+      if (i0 + k * (int)blockDim.x < nsrc) w[lo + i0 + k * blockDim.x] = v[k] * scale;
   }
 }
 
@@ -844,6 +846,391 @@ static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
 }
 
 // ---------------------------------------------------------------------------
+// Split-FP16 matrix-core gridder (FP32 grids, 16 x 16 tiles, the default for
+// them).  The FP16 product v_mfma_f32_16x16x32_f16 takes 32 records per
+// instruction in about the time the FP32 16x16x4 product takes 4, so the
+// tile update out[u][(c,v)] += sum_k wu_k(u) * (wv_k(v) * d_k[c]) runs on it
+// with each FP32 operand x split into two FP16 halves, x = hi + lo with
+// hi = fp16(x) and lo = fp16(x - hi), and three products per block:
+// hi*hi + hi*lo + lo*hi, accumulated in FP32.  The split keeps 22 bits of
+// each operand (the dropped lo*lo term is below 2^-22 relative), against 24
+// for an FP32 product, and each product is exact in the FP32 accumulator.
+//
+// FP16 has a narrow range, so every operand is scaled by a power of two
+// (exactly): the weight table by 2^14 (weights are at most 1, so the table
+// stays below 2^14 and its values above the FP16 subnormals down to 4e-9),
+// and the data of a unit by 2^-E with E the exponent of the largest |d| the
+// unit has seen so far, which puts them below 1.  The products wu * wv * d
+// are then below 2^28 (the u- and v-weights both carry 2^14) and the FP32
+// accumulators hold the tile times 2^(28 - E).  When a batch of 64 records
+// raises E, the accumulators are rescaled by the power of two first, so
+// records of any magnitude mix; the flush multiplies by 2^(E - 28).  Errors are therefore relative to the
+// largest data of the unit, as for FP32 sums.  Data of +-inf make the cells
+// they touch NaN (the FP32 kernel gives inf there).
+//
+// Lane l of the 16x16x32 product holds A[l % 16][8 * (l / 16) + j] and
+// B[8 * (l / 16) + j][l % 16], j = 0..7, and result rows 4 * (l / 16) + r of
+// column l % 16 (checked on the GPU).  So lane l forms, for records
+// 8 * (l / 16) + j of the step, the u-weight of tile row l % 16 and the
+// v-weight of tile column l % 16 times each component.  Staged data are
+// component-major ([c][64 records]), so the eight records of a lane are two
+// 16-byte LDS reads per component.
+constexpr int kH3TabExp = 14;  // weight table scale 2^14
+constexpr int kH3NoScale = -100000;
+
+typedef _Float16 H3Half8 __attribute__((ext_vector_type(8)));
+typedef float H3Float4 __attribute__((ext_vector_type(4)));
+typedef uint32_t H3U4 __attribute__((ext_vector_type(4)));
+
+struct H3Lds {
+  int tab, nrm, st_d, st_gu, st_gv, wave_bytes, total, nw, pad, rows, nnrm;
+};
+
+// The padded FP32 weight table (as the 16 x 16 FP32 matrix-core kernel's,
+// times 2^14), the FP64 kernel-sum table, and per wavefront the staged data
+// [c][64 records] and the records' two table offsets in bytes.
+template <int NPOL>
+__host__ __device__ inline H3Lds h3_lds_layout(int ov, int support) {
+  const MfmaLds M = mfma_lds_layout<float, NPOL>(ov, support, 16);
+  H3Lds L;
+  L.tab = M.tab;
+  L.nrm = M.nrm;
+  L.nw = M.nw;
+  L.pad = M.pad;
+  L.rows = M.rows;
+  L.nnrm = M.nnrm;
+  L.st_d = M.st_d;
+  L.st_gu = 64 * 2 * NPOL * 4;
+  L.st_gv = L.st_gu + 64 * 4;
+  L.wave_bytes = L.st_gv + 64 * 4;
+  L.total = L.st_d + kGridMaxWaves * L.wave_bytes;
+  return L;
+}
+
+// Eight FP32 values a_j as FP16 halves, two per register: hi_j = fp16(a_j)
+// (v_cvt_pk_f16_f32, round to nearest even) and lo_j = fp16(a_j - hi_j).
+// v_fma_mix forms its FMA in FP32 and rounds the result to FP16; mixlo
+// writes bits 0-15 of its register and mixhi bits 16-31, keeping the other
+// half.  The closing s_nop gives the wait states a matrix-core read of a
+// VALU result needs (the compiler cannot see which instruction of the block
+// writes which register).
+__device__ inline void h3_split8(const float *a, H3U4 &hi, H3U4 &lo) {
+  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
+  asm("v_cvt_pk_f16_f32 %0, %8, %9\n\t"
+      "v_cvt_pk_f16_f32 %1, %10, %11\n\t"
+      "v_cvt_pk_f16_f32 %2, %12, %13\n\t"
+      "v_cvt_pk_f16_f32 %3, %14, %15\n\t"
+      "v_fma_mixlo_f16 %4, %8, 1.0, -%0 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %4, %9, 1.0, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixlo_f16 %5, %10, 1.0, -%1 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %5, %11, 1.0, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixlo_f16 %6, %12, 1.0, -%2 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %6, %13, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixlo_f16 %7, %14, 1.0, -%3 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %7, %15, 1.0, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+      "s_nop 1"
+      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
+      : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]),
+        "v"(a[7]));
+  hi = H3U4{h0, h1, h2, h3};
+  lo = H3U4{l0, l1, l2, l3};
+}
+
+// Eight FP32 products x_j = w_j * d_j as FP16 halves, two per register:
+// hi_j = fp16(x_j) and lo_j = fp16(x_j - hi_j), two v_fma_mix each.
+__device__ inline void h3_split_products8(const float *w, const float *d, H3U4 &hi, H3U4 &lo) {
+  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
+#define AV_H3_PAIR(H, L, W0, D0, W1, D1)                                      \
+  "v_fma_mixlo_f16 " H ", " W0 ", " D0 ", 0\n\t"                            \
+  "v_fma_mixhi_f16 " H ", " W1 ", " D1 ", 0\n\t"                            \
+  "v_fma_mixlo_f16 " L ", " W0 ", " D0 ", -" H " op_sel_hi:[0,0,1]\n\t"     \
+  "v_fma_mixhi_f16 " L ", " W1 ", " D1 ", -" H " op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+  asm(AV_H3_PAIR("%0", "%4", "%8", "%16", "%9", "%17")
+      AV_H3_PAIR("%1", "%5", "%10", "%18", "%11", "%19")
+      AV_H3_PAIR("%2", "%6", "%12", "%20", "%13", "%21")
+      AV_H3_PAIR("%3", "%7", "%14", "%22", "%15", "%23")
+      "s_nop 1"
+      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
+      : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "v"(w[6]),
+        "v"(w[7]), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]), "v"(d[4]), "v"(d[5]),
+        "v"(d[6]), "v"(d[7]));
+#undef AV_H3_PAIR
+  hi = H3U4{h0, h1, h2, h3};
+  lo = H3U4{l0, l1, l2, l3};
+}
+
+template <int NPOL, bool PSF>
+__global__ __launch_bounds__(kGridThreads, (NPOL == 4 ? 4 : 5))
+void k_grid_h3(PlanDev P, const float *__restrict__ vis, const float *__restrict__ weight,
+               const uint8_t *__restrict__ flag, float *__restrict__ grid, int mode) {
+  constexpr int C = 2 * NPOL;
+  extern __shared__ __align__(16) char smem[];
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const H3Lds L = h3_lds_layout<NPOL>(ov, S);
+  float *tab = (float *)(smem + L.tab);
+  const char *tabc = smem + L.tab;
+  double *nrm = (double *)(smem + L.nrm);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char *wbase = smem + L.st_d + wave * L.wave_bytes;
+  float *st_d = (float *)wbase;                      // [c][64 records]
+  uint32_t *st_gu = (uint32_t *)(wbase + L.st_gu);  // [64 records] u-tap table offsets, bytes
+  uint32_t *st_gv = (uint32_t *)(wbase + L.st_gv);  // [64 records] v-tap table offsets, bytes
+
+  const bool adjoint = (mode & AV_GRID_ADJOINT) != 0;
+  const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
+  const int half = ov / 2 + 2;
+
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  const uint32_t blk = xcd_grouped_block(blockIdx.x, kGridXcdGroup);
+  if (blk * kGridMaxWaves >= n_units) return;
+  copy_wtab_padded(tab, mfma_lds_layout<float, NPOL>(ov, S, 16), P, (float)(1 << kH3TabExp));
+  for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
+  __syncthreads();
+  const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blk * kGridMaxWaves + wave));
+  if (u >= n_units) return;
+
+  const uint4 unit = P.gunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * 16, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * 16;
+  const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
+  float *gplane = grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+  const int li = lane & 15, lk = lane >> 4;
+
+  if (cnt == 0) {
+    if (lane < NPOL) P.upart[(size_t)u * 4 + lane] = 0.0;
+    if (!accumulate) {
+      for (int idx = lane; idx < NPOL * 256; idx += 64) {
+        const int p = idx >> 8, c = idx & 255;
+        const int gu = u0 + (c >> 4), gv = v0 + (c & 15);
+        if (gu < P.n_u && gv < P.n_v)
+          *(Cplx<float> *)(gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2) =
+              Cplx<float>{0.f, 0.f};
+      }
+    }
+    return;
+  }
+
+  H3Float4 acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = H3Float4{0.f, 0.f, 0.f, 0.f};
+  int E = kH3NoScale;  // the unit's data exponent (uniform)
+  double sw[NPOL];
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+
+  const uint32_t end = first + cnt;
+  uint2 r0 = load_record(P, first + lane, end), r1 = load_record(P, first + 64 + lane, end);
+  RawSample<float, NPOL> raw0;
+  fetch_raw<float, NPOL>(raw0, r0.x, vis, weight, flag, PSF, adjoint);
+  for (uint32_t base = first; base < end; base += 64) {
+    // ---- this lane's record: data (zeros when skipped) and geometry
+    uint32_t geo = kInvalidGeo;
+    float d[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) d[k] = 0.f;
+    const uint32_t g0 = base + lane < end ? r0.y : kInvalidGeo;
+    if (g0 != kInvalidGeo) {
+      int au, ou, avv, ovv;
+      unpack_geo(g0, au, ou, avv, ovv);
+      const double norm = nrm[ou + half] * nrm[ovv + half];
+      const bool prim = base + lane < prim_end;
+      bool any = false;
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        float dre, dim;
+        if (adjoint) {
+          const float inv = (float)(1.0 / norm);
+          dre = raw0.vv[p].re * inv;
+          dim = raw0.vv[p].im * inv;
+        } else {
+          float w = raw0.wv[p];
+          if ((raw0.fbits >> (8 * p)) & 0xFFu) w = 0.f;
+          if (PSF) {
+            dre = w;
+            dim = 0.f;
+          } else {
+            dre = raw0.vv[p].re * w;
+            dim = raw0.vv[p].im * w;
+          }
+        }
+        // standard_grid.py:378: skip NaN or exactly-zero data
+        const bool ok = !(isnan(dre) || isnan(dim)) && !(dre == 0.f && dim == 0.f);
+        if (ok) {
+          any = true;
+          d[2 * p] = dre;
+          d[2 * p + 1] = dim;
+          if (prim && !adjoint) sw[p] += (double)raw0.wv[p] * norm;
+        }
+      }
+      if (any) geo = g0;
+    }
+    // ---- the unit's scale: raise E to this batch's largest |d| (a power of
+    // two, so rescaling the accumulators is exact)
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < C; ++k) m = fmaxf(m, fabsf(d[k]));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    m = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, m)));
+    if (m > 0.f) {
+      int e;
+      (void)frexpf(m, &e);  // m < 2^e
+      e = __builtin_amdgcn_readfirstlane(e);
+      if (e > E) {
+        if (E != kH3NoScale) {
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[c][r] = ldexpf(acc[c][r], E - e);
+        }
+        E = e;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) st_d[k * 64 + lane] = E == kH3NoScale ? 0.f : ldexpf(d[k], -E);
+    {
+      // byte offsets of tap (row) 0 of the record's u and v weights; a
+      // skipped record has zero data, so any in-range offset (0) serves it
+      uint32_t ou_b = 0u, ov_b = 0u;
+      if (geo != kInvalidGeo) {
+        int au, ou, avv, ovv;
+        unpack_geo(geo, au, ou, avv, ovv);
+        ou_b = 4u * (uint32_t)((sc - au + L.pad) * L.nw + ou + L.nw / 2);
+        ov_b = 4u * (uint32_t)((sc - avv + L.pad) * L.nw + ovv + L.nw / 2);
+      }
+      st_gu[lane] = ou_b;
+      st_gv[lane] = ov_b;
+    }
+    // ---- issue the next loads
+    RawSample<float, NPOL> raw1;
+    fetch_raw<float, NPOL>(raw1, r1.x, vis, weight, flag, PSF, adjoint);
+    const uint2 r2 = load_record(P, base + 128 + lane, end);
+    wave_lds_sync();
+    // ---- 32 records per step: records s + 8 * lk + j of the batch
+    const int nst = (int)min((uint32_t)64, end - base);
+    const uint32_t li_b = 4u * (uint32_t)(li * L.nw);  // this lane's tile row / column
+    for (int s = 0; s < nst; s += 32) {
+      const int rb = s + 8 * lk;
+      const uint4 ua = *(const uint4 *)(st_gu + rb), ub = *(const uint4 *)(st_gu + rb + 4);
+      const uint4 va = *(const uint4 *)(st_gv + rb), vb = *(const uint4 *)(st_gv + rb + 4);
+      const uint32_t uo[8] = {ua.x, ua.y, ua.z, ua.w, ub.x, ub.y, ub.z, ub.w};
+      const uint32_t vo[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+      float a[8], wv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        a[j] = *(const float *)(tabc + uo[j] + li_b);
+        wv[j] = *(const float *)(tabc + vo[j] + li_b);
+      }
+      H3U4 ahu, alu;
+      h3_split8(a, ahu, alu);
+      const H3Half8 ah = __builtin_bit_cast(H3Half8, ahu), al = __builtin_bit_cast(H3Half8, alu);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if constexpr (PSF) {
+          if (c & 1) continue;
+        }
+        const H3Float4 da = *(const H3Float4 *)(st_d + c * 64 + rb);
+        const H3Float4 db = *(const H3Float4 *)(st_d + c * 64 + rb + 4);
+        const float dv[8] = {da.x, da.y, da.z, da.w, db.x, db.y, db.z, db.w};
+        H3U4 bhu, blu;
+        h3_split_products8(wv, dv, bhu, blu);
+        const H3Half8 bh = __builtin_bit_cast(H3Half8, bhu), bl = __builtin_bit_cast(H3Half8, blu);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[c], 0, 0, 0);
+      }
+    }
+    wave_lds_sync();
+    raw0 = raw1;
+    r0 = r1;
+    r1 = r2;
+  }
+
+  // ---- flush: undo the scales; lane holds rows 4 * lk + r, column li
+  const int fexp = E - 2 * kH3TabExp;  // both operands carry the table scale
+  const bool multi = (fl & kUnitMulti) != 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int gu = u0 + 4 * lk + r, gv = v0 + li;
+    if (gu >= P.n_u || gv >= P.n_v) continue;
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p) {
+      float re = 0.f, im = 0.f;
+      if (E != kH3NoScale) {
+        re = ldexpf(acc[2 * p][r], fexp);
+        im = ldexpf(acc[2 * p + 1][r], fexp);
+      }
+      float *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+      if (multi) {
+        if (re != 0.f) atomicAdd(dst, re);
+        if (im != 0.f) atomicAdd(dst + 1, im);
+      } else if (accumulate) {
+        Cplx<float> old = *(Cplx<float> *)dst;
+        old.re += re;
+        old.im += im;
+        *(Cplx<float> *)dst = old;
+      } else {
+        *(Cplx<float> *)dst = Cplx<float>{re, im};
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) {
+    double sv = adjoint ? 0.0 : sw[p];
+    for (int off = 32; off >= 1; off >>= 1) sv += __shfl_xor(sv, off);
+    if (lane == 0) P.upart[(size_t)u * 4 + p] = sv;
+  }
+}
+
+template <int NPOL, bool PSF>
+static int launch_grid_h3(av_plan *plan, const void *vis, const void *weight,
+                          const uint8_t *flag, void *grid, double *sumwt, int mode,
+                          hipStream_t s) {
+  PlanDev &d = plan->d;
+  const H3Lds L = h3_lds_layout<NPOL>(d.ov, d.support);
+  {
+    const int rc_lds =
+        ensure_lds_limit<k_grid_h3<NPOL, PSF>>(L.total, "av_grid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
+  const unsigned nwg = (unsigned)xcd_grouped_grid(
+      (d.max_gunits + kGridMaxWaves - 1) / kGridMaxWaves, kGridXcdGroup);
+  hipLaunchKernelGGL((k_grid_h3<NPOL, PSF>), dim3(nwg), dim3(kGridThreads), L.total, s, d,
+                     (const float *)vis, (const float *)weight, flag, (float *)grid, mode);
+  AV_LAUNCH_CHECK("k_grid_h3");
+  if (!(mode & AV_GRID_ADJOINT)) {
+    hipLaunchKernelGGL(k_sumwt_reduce, dim3((unsigned)d.n_img_chan), dim3(256), 0, s, d, NPOL,
+                       sumwt);
+    AV_LAUNCH_CHECK("k_sumwt_reduce");
+  }
+  return AV_OK;
+}
+
+// The split-FP16 kernel serves FP32 16 x 16 tiles without compensated sums
+// whenever its padded weight table fits in a workgroup's LDS.
+template <int NPOL>
+static int try_grid_h3(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
+                       const double *cgk, void *grid, double *sumwt, int mode, hipStream_t s,
+                       bool &done) {
+  const PlanDev &d = plan->d;
+  if (d.tile_u != 16 || d.tile_v != 16 || (mode & AV_GRID_COMPENSATED)) return AV_OK;
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_grid: device properties");
+  if ((size_t)h3_lds_layout<NPOL>(d.ov, d.support).total > prop.lds_per_block) return AV_OK;
+  int rc = prepare_tables(d, cgk, s);
+  if (rc) return rc;
+  done = true;
+  if (mode & AV_GRID_PSF)
+    return launch_grid_h3<NPOL, true>(plan, vis, weight, flag, grid, sumwt, mode, s);
+  return launch_grid_h3<NPOL, false>(plan, vis, weight, flag, grid, sumwt, mode, s);
+}
+
+// ---------------------------------------------------------------------------
 // Sub-tile scheduled matrix-core gridder (FP32, 16 x 16 tiles).  A record's
 // footprint (at most S x S cells) covers only a few of the tile's sixteen
 // 4 x 4 sub-tiles, yet a 16 x 16 x 4 product spends its full 16 x 16 block
@@ -920,7 +1307,7 @@ __device__ inline uint32_t subtile_mask(int au, int av, int sc) {
 // (the adjoint of the degridder).  A template parameter, so that no mode's
 // arithmetic is evaluated speculatively in another's.
 template <int NPOL, int MODE>
-__global__ __launch_bounds__(kGridThreads, 6)
+__global__ __launch_bounds__(kGridThreads, 5)
 void k_grid_sub(PlanDev P, const float *__restrict__ vis, const float *__restrict__ weight,
                 const uint8_t *__restrict__ flag, float *__restrict__ grid, int mode) {
   typedef float V4 __attribute__((ext_vector_type(4)));
@@ -1068,7 +1455,10 @@ void k_grid_sub(PlanDev P, const float *__restrict__ vis, const float *__restric
     }
     // ---- the blocks' lists: every entry starts as the null record, then
     // each record is appended to the list of every sub-tile it touches
-    *(uint4 *)(list + lane * 16) = make_uint4(0x40404040u, 0x40404040u, 0x40404040u, 0x40404040u);
+    // (the 16 bytes past the lists are read one round ahead by the last block)
+    const uint4 nulls = make_uint4(0x40404040u, 0x40404040u, 0x40404040u, 0x40404040u);
+    *(uint4 *)(list + lane * 16) = nulls;
+    if (lane == 0) *(uint4 *)(list + 1024) = nulls;
     wave_lds_sync();
     uint32_t rounds = 0;
 #pragma unroll
@@ -1085,19 +1475,33 @@ void k_grid_sub(PlanDev P, const float *__restrict__ vis, const float *__restric
     fetch_raw<float, NPOL>(raw1, r1.x, vis, weight, flag, psf, adjoint);
     const uint2 r2 = load_record(P, base + 128 + lane, end);
     wave_lds_sync();
-    // ---- rounds: every block applies the next record of its list
-    for (uint32_t i = 0; i < rounds; ++i) {
-      const int k = list[lbase + i];
-      const uint32_t tb = st_geo[k];
-      float dd[C];
+    // ---- rounds: every block applies the next record of its list.  The
+    // next round's slot, table bases and data are read while this round's
+    // products run (a list entry past a block's records is the null slot).
+    const int nrounds = __builtin_amdgcn_readfirstlane((int)rounds);
+    auto load_slot = [&](int k, uint32_t &tb, float *dd) {
+      tb = st_geo[k];
 #pragma unroll
       for (int q = 0; q < C; q += CL::kCC) {
         const Vec<float, CL::kCC> t = *(const Vec<float, CL::kCC> *)(st_d + k * C + q);
 #pragma unroll
         for (int c = 0; c < CL::kCC; ++c) dd[q + c] = t.v[c];
       }
+    };
+    uint32_t tb_n;
+    float dd_n[C];
+    load_slot(list[lbase], tb_n, dd_n);
+    for (int i = 0; i < nrounds; ++i) {
+      const uint32_t tb = tb_n;
+      float dd[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) dd[c] = dd_n[c];
       const float a = tab[(tb & 0xFFFFu) + rowoff];
       const float wv = tab[(tb >> 16) + coloff];
+      // LDS reads return in order: keep this round's table reads ahead of
+      // the next round's, so the products wait for them alone
+      __builtin_amdgcn_sched_barrier(0);
+      load_slot(list[lbase + i + 1], tb_n, dd_n);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         if constexpr (psf) {
@@ -1248,6 +1652,10 @@ static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, con
   if constexpr (sizeof(R) == 4) {
     if (!comp && (mode & kGridSubtile)) {
       rc = try_grid_sub<NPOL>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s, done);
+      if (done || rc) return rc;
+    }
+    if (!comp && !(mode & AV_GRID_F32_PRODUCTS)) {
+      rc = try_grid_h3<NPOL>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s, done);
       if (done || rc) return rc;
     }
     if (comp)

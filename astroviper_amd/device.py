@@ -15,7 +15,8 @@ import torch
 from . import _lib
 from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_DEGRID, AV_PLAN_ROW_CHANNELS,
                    AV_GEOM_REFERENCE, AV_GRID_ACCUMULATE, AV_GRID_ADJOINT,
-                   AV_GRID_COMPENSATED, AV_GRID_PSF, AV_GRID_SUBTILE, call)
+                   AV_GRID_COMPENSATED, AV_GRID_F32_PRODUCTS, AV_GRID_PSF, AV_GRID_SUBTILE,
+                   call)
 
 
 def require_gpu(device=None) -> torch.device:
@@ -196,13 +197,15 @@ class GridPlan:
              sumwt: Optional[torch.Tensor] = None, *, n_pol: Optional[int] = None,
              psf: bool = False, accumulate: bool = False, compensated: bool = False,
              adjoint: bool = False, dtype: Optional[torch.dtype] = None,
-             subtile: bool = False):
+             subtile: bool = False, f32_products: bool = False):
         """Grid samples onto [n_img_chan, n_pol, n_u, n_v] complex planes.
 
         vis [samples, n_pol] complex, weight [samples, n_pol] real, flag
         [samples, n_pol] uint8/bool or None.  Returns (grid, sumwt).
-        ``subtile`` runs FP32 16 x 16 tiles through the sub-tile scheduled
-        4x4x1 matrix-core kernel instead of the dense 16x16x4 one.
+        FP32 16 x 16 tiles run the split-FP16 matrix-core kernel (FP16
+        halves of every operand, FP32 sums) unless ``f32_products`` asks for
+        the FP32 16x16x4 kernel or ``subtile`` for the sub-tile scheduled
+        4x4x1 one.
         """
         ref = vis if vis is not None else weight
         if ref is None:
@@ -243,7 +246,8 @@ class GridPlan:
             return grid, sumwt
         mode = ((AV_GRID_PSF if psf else 0) | (AV_GRID_ACCUMULATE if accumulate else 0)
                 | (AV_GRID_COMPENSATED if compensated else 0)
-                | (AV_GRID_ADJOINT if adjoint else 0) | (AV_GRID_SUBTILE if subtile else 0))
+                | (AV_GRID_ADJOINT if adjoint else 0) | (AV_GRID_SUBTILE if subtile else 0)
+                | (AV_GRID_F32_PRODUCTS if f32_products else 0))
         call("av_grid", self._h, _ptr(None if psf else vis),
              _ptr(None if adjoint else weight), _ptr(flag), int(n_pol), _ptr(self.cgk),
              _ptr(grid), _ptr(sumwt), code, mode, _stream())
@@ -280,10 +284,13 @@ class GridPlan:
 # ------------------------------------------------------ kernel choice --
 def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = False,
                      support: int = 7, oversampling: int = 100,
-                     lds_bytes: int = 160 * 1024, subtile: bool = False) -> str:
+                     lds_bytes: int = 160 * 1024, subtile: bool = False,
+                     f32_products: bool = False) -> str:
     """Name of the tile kernel av_grid runs for this plan shape: for FP32
     16 x 16 tiles without compensated sums, the sub-tile scheduled kernel
-    (4x4x1 products) when ``subtile`` is asked for, else the dense
+    (4x4x1 products) when ``subtile`` is asked for, else the split-FP16
+    kernel unless ``f32_products`` is asked for (both when their LDS
+    fits), else the dense
     matrix-core kernel for tiles of 16 x 16 multiples whose result blocks
     ((tile_u/16) * 2*n_pol * (tile_v/16)) fit its register budget (16 blocks
     for FP32, 8 for FP64 and for compensated sums) and whose LDS (a weight
@@ -306,6 +313,8 @@ def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = F
         up16 = lambda b: (b + 15) & ~15
         total = up16((support + 2 * pad) * nw * r) + up16(nw * 8) + 4 * (64 * 2 * n_pol * r + 64 * 8)
         if total <= lds_bytes:
+            if (tu, tv) == (16, 16) and not fp64 and not compensated and not f32_products:
+                return "k_grid_h3"
             return "k_grid_mfma"
     return "k_grid_tiles"
 

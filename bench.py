@@ -265,6 +265,9 @@ def main():
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--subtile", action="store_true",
                     help="grid FP32 16x16 tiles with the sub-tile scheduled 4x4x1 kernel")
+    ap.add_argument("--f32-products", action="store_true",
+                    help="grid FP32 16x16 tiles with FP32 products (16x16x4 kernel) instead "
+                         "of split-FP16 operands")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -318,7 +321,7 @@ def main():
         if ev is not None:
             ev[1].record()
         plan.grid(vis2, w2, fl2, grid=grid, sumwt=sumwt, compensated=args.compensated,
-                  subtile=args.subtile)
+                  subtile=args.subtile, f32_products=args.f32_products)
         if ev is not None:
             ev[2].record()
         plan.degrid(grid, fl2, model)
@@ -355,7 +358,7 @@ def main():
     tdom, bdom = kern[dom]
     achieved = bdom / (tdom / 1e3) / 1e9
     names = {"grid": grid_kernel_name(args.tile, npol, compensated=args.compensated,
-                                      subtile=args.subtile),
+                                      subtile=args.subtile, f32_products=args.f32_products),
              "degrid": degrid_kernel_name(args.tile, npol, 7)}
     roofline = {"bound": "hbm", "kernel": names[dom],
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -423,7 +426,8 @@ def main():
 
         errs["dirty"] = img_err(dirty_image(grid[:1], sumwt[:1]), dirty_image(g64[:1], sw64[:1]))
         del g64
-        p32, ps32 = plan.grid(None, w2, fl2, psf=True, subtile=args.subtile)
+        p32, ps32 = plan.grid(None, w2, fl2, psf=True, compensated=args.compensated,
+                              subtile=args.subtile, f32_products=args.f32_products)
         p64, ps64 = plan.grid(None, w2.to(torch.float64), fl2, psf=True)
         errs["psf"] = img_err(dirty_image(p32[:1], ps32[:1]), dirty_image(p64[:1], ps64[:1]))
         errs["images_of"] = f"image channel 0 ({npol} planes)"
@@ -465,7 +469,8 @@ def main():
                                f"{n_uv[0]}x{n_uv[1]} {args.mode}, support 7, oversampling 100, "
                                f"10% flagged",
                    "mode": args.mode, "tile": list(args.tile), "visibilities": V,
-                   "compensated": bool(args.compensated), "subtile": bool(args.subtile)},
+                   "compensated": bool(args.compensated), "subtile": bool(args.subtile),
+                   "grid_kernel": names["grid"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -38,6 +38,9 @@ extern "C" {
 #define AV_GRID_SUBTILE 16    /* FP32 16 x 16 tiles: run the sub-tile
                                  scheduled 4x4x1 matrix-core kernel instead of
                                  the dense 16x16x4 one (same results) */
+#define AV_GRID_F32_PRODUCTS 32 /* FP32 16 x 16 tiles: form every product in
+                                   FP32 (16x16x4 FP32 matrix-core kernel)
+                                   instead of from split FP16 operands */
 /* av_degrid mode bits */
 #define AV_DEGRID_INCREMENTAL 1 /* add into vis_out instead of overwriting */
 

@@ -5,8 +5,10 @@ instantiations the benchmark and the precision sweep time.
   16 x 16 tiles, 10% flagged, weights U(0.5, 1.5): the benchmark's generator
   (synth.CONFIG2) over 100 of its 1000 time samples and 2 of its 64 channels
   (1.0 and 1.4 GHz), cube and continuum, FP32 and FP64.  These plans run
-  k_grid_mfma<float,4,1,1> / <double,4,1,1> and the v-half degridder
-  k_degrid_mfma (4 polarisations, 12 column blocks, 8 per half).
+  the split-FP16 kernel k_grid_h3<4> (the benchmark's FP32 gridder), the
+  FP32-products kernel k_grid_mfma<float,4,1,1>, the sub-tile kernel,
+  k_grid_mfma<double,4,1,1> and the v-half degridder k_degrid_mfma
+  (4 polarisations, 12 column blocks, 8 per half).
 * The same generator at 4096 x 4096 (the precision sweep's grid), 40 time
   samples x 2 channels, cube: FP32, FP32 compensated and FP64.
 
@@ -80,17 +82,21 @@ def oracle_outputs(oracle, d, mode):
     return out
 
 
-def run_gpu(gpu, d, mode, prec, compensated=False, tile=(16, 16), subtile=False):
+def run_gpu(gpu, d, mode, prec, compensated=False, tile=(16, 16), subtile=False,
+            f32_products=False):
     from astroviper_amd.device import (GridPlan, degrid_kernel_name, dirty_image,
                                        grid_kernel_name)
     nt, nb, nchan, npol = d["w"].shape
     rows = nt * nb
     cube = mode == "cube"
     # FP32 without compensated sums runs the sub-tile kernel (4x4x1 products)
-    # when asked for, else the dense 16x16x4 kernel
-    sub = prec == 32 and not compensated and subtile
+    # or the FP32-products kernel when asked for, else the split-FP16 kernel;
+    # FP64 and compensated FP32 the dense 16x16x4 kernel
+    plain32 = prec == 32 and not compensated
+    expect = ("k_grid_sub" if plain32 and subtile else
+              "k_grid_mfma" if not plain32 or f32_products else "k_grid_h3")
     assert grid_kernel_name(tile, npol, fp64=prec == 64, compensated=compensated,
-                            subtile=subtile) == ("k_grid_sub" if sub else "k_grid_mfma")
+                            subtile=subtile, f32_products=f32_products) == expect
     assert degrid_kernel_name(tile, npol, 7) == "k_degrid_mfma"
     plan = GridPlan(rows, nchan, nchan if cube else 1, d["n_uv"], d["cell"], 7, 100,
                     tile=tile, device=gpu)
@@ -101,9 +107,11 @@ def run_gpu(gpu, d, mode, prec, compensated=False, tile=(16, 16), subtile=False)
     v = torch.from_numpy(d["vis"].reshape(rows * nchan, npol)).to(gpu, cdt)
     w = torch.from_numpy(d["w"].reshape(rows * nchan, npol)).to(gpu, rdt)
     fl = torch.from_numpy(d["flag"].reshape(rows * nchan, npol).astype(np.uint8)).to(gpu)
-    g, sw = plan.grid(v, w, fl, compensated=compensated, subtile=subtile)
+    g, sw = plan.grid(v, w, fl, compensated=compensated, subtile=subtile,
+                      f32_products=f32_products)
     dirty = dirty_image(g, sw)
-    pg, psw = plan.grid(None, w, fl, psf=True, compensated=compensated, subtile=subtile)
+    pg, psw = plan.grid(None, w, fl, psf=True, compensated=compensated, subtile=subtile,
+                        f32_products=f32_products)
     psf = dirty_image(pg, psw)
     torch.cuda.synchronize()
     st = plan.stats()
@@ -140,14 +148,16 @@ def c2_subset(oracle):
 
 
 @pytest.mark.parametrize("mode", ["cube", "continuum"])
-@pytest.mark.parametrize("prec,subtile", [(32, False), (32, True), (64, False)])
-def test_benchmark_workload_subset(gpu, oracle, c2_subset, mode, prec, subtile):
+@pytest.mark.parametrize("prec,variant", [(32, "split16"), (32, "f32_products"), (32, "subtile"),
+                                          (64, "")])
+def test_benchmark_workload_subset(gpu, oracle, c2_subset, mode, prec, variant):
     d, refs = c2_subset
     ref = refs[mode]
-    got = run_gpu(gpu, d, mode, prec, subtile=subtile)
+    got = run_gpu(gpu, d, mode, prec, subtile=variant == "subtile",
+                  f32_products=variant == "f32_products")
     st = got["stats"]
     assert st["grid_valid"] > 0.6 * st["samples"]
-    check_outputs(got, ref, prec, f"2048^2 {mode} FP{prec}{' subtile' if subtile else ''}")
+    check_outputs(got, ref, prec, f"2048^2 {mode} FP{prec} {variant}")
     # degrid the oracle's grid (rounded to the run's precision) with the
     # benchmark's plan, against the oracle's degridder on that same grid
     cdt = torch.complex64 if prec == 32 else torch.complex128
@@ -169,10 +179,11 @@ def c5_subset(oracle):
     return d, oracle_outputs(oracle, d, "cube")
 
 
-@pytest.mark.parametrize("variant", ["fp32", "fp32_compensated", "fp64"])
+@pytest.mark.parametrize("variant", ["fp32", "fp32_products", "fp32_compensated", "fp64"])
 def test_precision_sweep_workload_subset(gpu, oracle, c5_subset, variant):
     d, ref = c5_subset
     prec = 64 if variant == "fp64" else 32
-    got = run_gpu(gpu, d, "cube", prec, compensated=variant == "fp32_compensated")
+    got = run_gpu(gpu, d, "cube", prec, compensated=variant == "fp32_compensated",
+                  f32_products=variant == "fp32_products")
     check_outputs(got, ref, prec, f"4096^2 cube {variant}")
     got["plan"].close()

@@ -90,7 +90,8 @@ def c1_data(dense=False, flag_frac=0.0, weights=None, seed=1):
 
 
 def device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, psf=False, support=7,
-                ov=100, tile=(32, 32), unit_cap=0, compensated=False, geometry="reference"):
+                ov=100, tile=(32, 32), unit_cap=0, compensated=False, geometry="reference",
+                f32_products=False):
     from astroviper_amd.device import GridPlan
     nt, nb, nchan, npol = w.shape
     rows = nt * nb
@@ -106,7 +107,7 @@ def device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, psf=False,
     if flag is not None:
         fl = torch.from_numpy(np.ascontiguousarray(flag.reshape(rows * nchan, npol))
                               .astype(np.uint8)).to(gpu)
-    g, sw = plan.grid(v, wt, fl, psf=psf, compensated=compensated)
+    g, sw = plan.grid(v, wt, fl, psf=psf, compensated=compensated, f32_products=f32_products)
     return g, sw, plan
 
 
@@ -135,8 +136,10 @@ def test_ps_tables_match_oracle(gpu, oracle):
 @pytest.mark.parametrize("cube", [False, True])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("dense", [False, True])
-@pytest.mark.parametrize("tile", [(32, 32), (16, 16)])
-def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense, tile):
+@pytest.mark.parametrize("tile,f32p", [((32, 32), False), ((16, 16), False), ((16, 16), True)])
+def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense, tile, f32p):
+    if f32p and prec == 64:
+        pytest.skip("FP32 products are an FP32 option")
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=dense, flag_frac=0.1,
                                                   weights=(0.5, 1.5))
     vis, w = as_prec(prec, vis, w)
@@ -144,13 +147,16 @@ def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense, tile):
     w0[flag] = 0
     gref, swref = oracle.standard_grid(vis, uvw, w0, freq, n_uv, cell,
                                        chan_mode="cube" if cube else "continuum")
-    g, sw, plan = device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, tile=tile)
+    g, sw, plan = device_grid(gpu, vis, uvw, w, flag, freq, n_uv, cell, cube, prec, tile=tile,
+                              f32_products=f32p)
     from astroviper_amd.device import grid_kernel_name
-    # FP32 and FP64 16 x 16 and FP32 32 x 32 tiles run the dense matrix-core
-    # kernel, FP64 32 x 32 at 2 pols the LDS kernel
-    kname = grid_kernel_name(tile, 2, fp64=prec == 64)
-    expect = {(32, (16, 16)): "k_grid_mfma", (32, (32, 32)): "k_grid_mfma",
-              (64, (16, 16)): "k_grid_mfma", (64, (32, 32)): "k_grid_tiles"}[(prec, tile)]
+    # FP32 16 x 16 tiles run the split-FP16 kernel (or, asked for, the FP32
+    # products kernel), FP64 16 x 16 and FP32 32 x 32 tiles the dense
+    # matrix-core kernel, FP64 32 x 32 at 2 pols the LDS kernel
+    kname = grid_kernel_name(tile, 2, fp64=prec == 64, f32_products=f32p)
+    expect = {(32, (16, 16), False): "k_grid_h3", (32, (16, 16), True): "k_grid_mfma",
+              (32, (32, 32), False): "k_grid_mfma", (64, (16, 16), False): "k_grid_mfma",
+              (64, (32, 32), False): "k_grid_tiles"}[(prec, tile, f32p)]
     assert kname == expect
     st = plan.stats()
     assert st["grid_valid"] > 1000
