@@ -33,6 +33,7 @@ AV_DEGRID_INCREMENTAL = 1
 AV_GEOM_REFERENCE = 0
 AV_GEOM_DEGRID = 1
 AV_PLAN_ROW_CHANNELS = 16
+AV_GEOM_CONJUGATE = 32
 
 _c = ctypes
 _P = _c.c_void_p

@@ -137,6 +137,7 @@ struct PlanDev {
   double cell_u, cell_v;
   double dscale_u, dscale_v, doff_u, doff_v;  // degrid pixel transform
   int32_t support, ov, tile_u, tile_v, geometry, unit_cap;
+  int32_t conjugate;  // AV_GEOM_CONJUGATE: samples at their conjugate centres
   // x / tile as __umulhi(x, tile_*_mul), exact for 0 <= x < 2^24 (tiles <= 128)
   uint32_t tile_u_mul, tile_v_mul;
   int32_t row_channels;  // every row is one sample at its own channel

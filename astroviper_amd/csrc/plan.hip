@@ -58,6 +58,17 @@ __device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, 
     g.gvalid = grid_geometry(uu, vv, P.gscale_u[ch], P.gscale_v[ch], P.n_u, P.n_v, sc, P.ov,
                              g.iu, g.iv, g.ou, g.ov) &&
                chan_ok;
+    if (P.conjugate) {
+      // the second scatter of do_imaging_weight (standard_grid.py:330-340,
+      // 389-425): the conjugate centre int(-u + N/2 + 0.5), the sample's own
+      // kernel offsets and bounds test, no degrid record
+      g.dvalid = false;
+      if (g.gvalid) {
+        const double u = uu * P.gscale_u[ch], v = vv * P.gscale_v[ch];
+        g.iu = (int)(-u + (double)(P.n_u / 2) + 0.5);
+        g.iv = (int)(-v + (double)(P.n_v / 2) + 0.5);
+      }
+    }
   }
 }
 
@@ -110,14 +121,15 @@ __device__ inline RecPlan record_plan(const PlanDev &P, const ItemGeo &g) {
   r.tu = r.tv = r.du_t = r.dv_t = 0;
   r.tu0 = r.tu1 = r.tv0 = r.tv1 = 0;
   if (g.gvalid) {
-    // positions of a valid footprint are in [0, n) (n < 2^24): tile
-    // numbers by multiply-high
-    r.tu = tile_u_of(P, g.iu);
-    r.tv = tile_v_of(P, g.iv);
-    r.tu0 = tile_u_of(P, g.iu - sc);
-    r.tu1 = tile_u_of(P, g.iu + P.support - sc - 1);
-    r.tv0 = tile_v_of(P, g.iv - sc);
-    r.tv1 = tile_v_of(P, g.iv + P.support - sc - 1);
+    // tile numbers by multiply-high of positions clamped into [0, n) (n <
+    // 2^24): a valid footprint lies inside the grid, a conjugate one may
+    // reach past an edge, whose cells are dropped
+    r.tu = tile_u_of(P, min(max(g.iu, 0), P.n_u - 1));
+    r.tv = tile_v_of(P, min(max(g.iv, 0), P.n_v - 1));
+    r.tu0 = tile_u_of(P, max(g.iu - sc, 0));
+    r.tu1 = tile_u_of(P, min(g.iu + P.support - sc - 1, P.n_u - 1));
+    r.tv0 = tile_v_of(P, max(g.iv - sc, 0));
+    r.tv1 = tile_v_of(P, min(g.iv + P.support - sc - 1, P.n_v - 1));
   }
   if (r.dsep) {
     r.du_t = tile_u_of(P, g.lu);
@@ -600,9 +612,12 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   if (n_u >= (1 << 24) || n_v >= (1 << 24))
     return fail(AV_ERR_UNSUPPORTED, "av_plan_create: grid sides must be below 2^24");
   const bool row_channels = (geometry & AV_PLAN_ROW_CHANNELS) != 0;
-  geometry &= ~AV_PLAN_ROW_CHANNELS;
+  const bool conjugate = (geometry & AV_GEOM_CONJUGATE) != 0;
+  geometry &= ~(AV_PLAN_ROW_CHANNELS | AV_GEOM_CONJUGATE);
   if (geometry != AV_GEOM_REFERENCE && geometry != AV_GEOM_DEGRID)
     return fail(AV_ERR_INVALID, "av_plan_create: unknown geometry");
+  if (conjugate && geometry != AV_GEOM_REFERENCE)
+    return fail(AV_ERR_INVALID, "av_plan_create: AV_GEOM_CONJUGATE needs AV_GEOM_REFERENCE");
   if (unit_cap == 0) unit_cap = 8192;
   if (unit_cap < 64) return fail(AV_ERR_INVALID, "av_plan_create: unit_cap < 64");
   int64_t n_items = row_channels ? n_rows : n_rows * (int64_t)n_chan;
@@ -629,6 +644,7 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   d.tile_u_mul = 0xFFFFFFFFu / (uint32_t)tile_u + 1u;
   d.tile_v_mul = 0xFFFFFFFFu / (uint32_t)tile_v + 1u;
   d.geometry = geometry;
+  d.conjugate = conjugate ? 1 : 0;
   d.row_channels = row_channels;
   d.unit_cap = unit_cap;
   d.ntu = (n_u + tile_u - 1) / tile_u;

@@ -13,7 +13,8 @@ from typing import Optional, Tuple
 import torch
 
 from . import _lib
-from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_DEGRID, AV_PLAN_ROW_CHANNELS,
+from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_CONJUGATE, AV_GEOM_DEGRID,
+                   AV_PLAN_ROW_CHANNELS,
                    AV_GEOM_REFERENCE, AV_GRID_ACCUMULATE, AV_GRID_ADJOINT,
                    AV_GRID_COMPENSATED, AV_GRID_F32_PRODUCTS, AV_GRID_PSF, AV_GRID_SUBTILE,
                    call)
@@ -112,8 +113,11 @@ class GridPlan:
     (axis 2, axis 3), ``cell`` the signed pixel increment in radians,
     ``support`` / ``oversampling`` the kernel table.  ``geometry`` is
     ``"reference"`` (gridding and degridding each use their reference
-    rounding rule) or ``"degrid"`` (both use the degridder's, making
-    ``grid(adjoint=True)`` the exact adjoint of ``degrid``).  With
+    rounding rule), ``"degrid"`` (both use the degridder's, making
+    ``grid(adjoint=True)`` the exact adjoint of ``degrid``) or
+    ``"conjugate"`` (the reference grid rule with every sample placed at its
+    conjugate centre and no degrid records: the second scatter of
+    ``do_imaging_weight``, standard_grid.py:389-430).  With
     ``row_channels`` every row is one sample at its own channel (``row_chan``
     in :meth:`set_uvw`), and the plan holds n_rows samples instead of
     n_rows * n_chan.
@@ -129,7 +133,8 @@ class GridPlan:
         self.cell = (float(cell[0]), float(cell[1]))
         self.support, self.oversampling = int(support), int(oversampling)
         self.tile = (int(tile[0]), int(tile[1]))
-        geo = {"reference": AV_GEOM_REFERENCE, "degrid": AV_GEOM_DEGRID}[geometry]
+        geo = {"reference": AV_GEOM_REFERENCE, "degrid": AV_GEOM_DEGRID,
+               "conjugate": AV_GEOM_REFERENCE | AV_GEOM_CONJUGATE}[geometry]
         if row_channels:
             geo |= AV_PLAN_ROW_CHANNELS
         self.geometry = geometry

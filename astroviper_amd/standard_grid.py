@@ -40,9 +40,10 @@ def _cgk_device(cgk_1D, oversampling, support, dev):
 
 def grid_to_device(vis_data, uvw, weight, flag, freq_chan, chan_map, n_img_chan, n_uv,
                    delta_lm, support, oversampling, do_psf, prec, cgk_dev,
-                   grid_dev=None, sumwt_dev=None, tile=(16, 16)):
+                   grid_dev=None, sumwt_dev=None, tile=(16, 16), geometry="reference"):
     """Grid host arrays [nt, nb, nchan, npol] on the GPU, accumulating into
-    grid_dev [n_img_chan, npol, n_u, n_v] / sumwt_dev when given."""
+    grid_dev [n_img_chan, npol, n_u, n_v] / sumwt_dev when given.
+    ``geometry="conjugate"`` places every sample at its conjugate centre."""
     dev = _dev.require_gpu()
     weight = np.asarray(weight)
     nt, nb, nchan, npol = weight.shape
@@ -64,7 +65,7 @@ def grid_to_device(vis_data, uvw, weight, flag, freq_chan, chan_map, n_img_chan,
     for r0 in range(0, rows, step):
         r1 = min(rows, r0 + step)
         plan = _dev.GridPlan(r1 - r0, nchan, n_img_chan, n_uv, delta_lm, support,
-                             oversampling, tile=tile, device=dev)
+                             oversampling, tile=tile, geometry=geometry, device=dev)
         plan.cgk = cgk_dev
         plan.set_uvw(torch.from_numpy(uvw_h[r0:r1]).to(dev), freq_t, cmap_t)
         w_t = torch.from_numpy(np.ascontiguousarray(w_h[r0:r1])).to(dev, rdt)
@@ -83,10 +84,15 @@ def grid_to_device(vis_data, uvw, weight, flag, freq_chan, chan_map, n_img_chan,
 def standard_grid_jit(grid, sum_weight, do_psf, do_imaging_weight, vis_data, uvw, freq_chan,
                       chan_map, pol_map, weight, cgk_1D, n_uv, delta_lm, support, oversampling):
     """standard_grid.py:246-432 on the GPU: accumulate into `grid`
-    (n_chan, n_pol, n_u, n_v) and `sum_weight` (n_chan, n_pol) in place."""
-    if do_imaging_weight:
-        raise NotImplementedError("do_imaging_weight (conjugate scatter) is not implemented "
-                                  "on the GPU path")
+    (n_chan, n_pol, n_u, n_v) and `sum_weight` (n_chan, n_pol) in place.
+
+    With ``do_imaging_weight`` every sample is gridded twice
+    (standard_grid.py:389-430): at its centre and at its conjugate centre
+    int(-u + N/2 + 0.5) with the same kernel weights (a second plan with the
+    conjugate geometry); conjugate cells outside the grid are dropped, and
+    the sum of weights counts both.  For the PSF with two or more
+    polarisations every polarisation then grids (w0 + w1) / 2
+    (standard_grid.py:356-360)."""
     weight = np.asarray(weight)
     npol = len(pol_map)
     if not np.array_equal(np.asarray(pol_map), np.arange(npol)) or weight.shape[3] != npol:
@@ -103,9 +109,14 @@ def standard_grid_jit(grid, sum_weight, do_psf, do_imaging_weight, vis_data, uvw
     cdt = torch.complex64 if prec == _dev.AV_F32 else torch.complex128
     g_dev = torch.from_numpy(np.ascontiguousarray(grid)).to(dev).to(cdt)
     s_dev = torch.from_numpy(np.ascontiguousarray(sum_weight, dtype=np.float64)).to(dev)
-    g_dev, s_dev = grid_to_device(vis_data, uvw, weight, None, freq_chan, chan_map, n_img_chan,
-                                  n_uv, delta_lm, support, oversampling, do_psf, prec, cgk,
-                                  g_dev, s_dev)
+    if do_imaging_weight and do_psf and npol >= 2:
+        w2 = (weight[..., 0] + weight[..., 1]) / 2.0
+        weight = np.ascontiguousarray(np.broadcast_to(w2[..., None], weight.shape))
+    geometries = ("reference", "conjugate") if do_imaging_weight else ("reference",)
+    for geo in geometries:
+        g_dev, s_dev = grid_to_device(vis_data, uvw, weight, None, freq_chan, chan_map,
+                                      n_img_chan, n_uv, delta_lm, support, oversampling, do_psf,
+                                      prec, cgk, g_dev, s_dev, geometry=geo)
     g_h = g_dev.cpu().numpy()
     grid[...] = g_h if np.iscomplexobj(grid) else g_h.real
     sum_weight[...] = s_dev.cpu().numpy()

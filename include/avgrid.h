@@ -53,6 +53,13 @@ extern "C" {
 #define AV_PLAN_ROW_CHANNELS 16 /* or'ed into geometry: every row is one sample
                                    at its own channel (row_chan), so the plan
                                    holds n_rows samples, not n_rows * n_chan */
+#define AV_GEOM_CONJUGATE 32    /* or'ed into AV_GEOM_REFERENCE: every
+                                   grid-valid sample is placed at its conjugate
+                                   centre int(-u + N/2 + 0.5) with its own
+                                   kernel offsets, cells outside the grid
+                                   dropped, and gets no degrid record: the
+                                   second scatter of do_imaging_weight
+                                   (standard_grid.py:330-340, 389-430) */
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
@@ -106,7 +113,8 @@ int av_ps_outer(const double *k1, int32_t nrow, int32_t nfill,
  * cell_u/v    pixelincr in radians (signed)
  * tile_u/v    tile shape in pixels (0 = default 32)
  * geometry    AV_GEOM_REFERENCE or AV_GEOM_DEGRID, optionally or'ed with
- *             AV_PLAN_ROW_CHANNELS (then av_plan_set_uvw needs row_chan)
+ *             AV_PLAN_ROW_CHANNELS (then av_plan_set_uvw needs row_chan);
+ *             AV_GEOM_REFERENCE may also be or'ed with AV_GEOM_CONJUGATE
  * unit_cap    records per work unit (0 = default 8192)
  * The plan is device-resident; create it once per (shape, tiling). */
 typedef struct av_plan av_plan;

@@ -435,3 +435,48 @@ def test_make_psf_values_match_oracle(gpu, oracle):
     assert p.shape == (1,) + ref.shape
     assert np.max(np.abs(p[0] - ref)) <= 1e-9 * np.max(np.abs(ref))
     assert np.all(np.abs(p[0, :, :, 100, 100] - 1 / 0.9999996674 ** 2) < 1e-6)
+
+
+# ------------------------------------------------- do_imaging_weight mode
+@pytest.mark.parametrize("n", [64, 63])
+@pytest.mark.parametrize("chan_mode", ["continuum", "cube"])
+def test_psf_imaging_weight_matches_oracle(gpu, oracle, n, chan_mode):
+    """standard_grid_psf_numpy_wrap with do_imaging_weight (standard_grid.py:
+    168-239, 318-430): the weight grid plus its conjugate scatter, through a
+    plan of conjugate centres, against the oracle; the 64 grid has conjugate
+    footprints that reach past its edge."""
+    from test_oracle import imaging_weight_edge_problem
+    from astroviper_amd.gcf_prolate_spheroidal import create_prolate_spheroidal_kernel_1D
+    from astroviper_amd.standard_grid import standard_grid_psf_numpy_wrap
+    uvw, freq, cell, w, n_uv = imaging_weight_edge_problem(n)
+    if chan_mode == "cube":  # two channels at f and 1.25 f
+        freq = np.array([C, 1.25 * C])
+        w = np.concatenate([w, 0.5 * w], axis=2)
+    cgk = create_prolate_spheroidal_kernel_1D(100, 7)
+    params = {"chan_mode": chan_mode, "image_size_padded": np.array(n_uv), "cell_size": cell,
+              "oversampling": 100, "support": 7, "do_psf": True, "do_imaging_weight": True}
+    g, sw = standard_grid_psf_numpy_wrap(uvw, w, freq, cgk, params)
+    gref, swref = oracle.standard_grid(None, uvw, w, freq, n_uv, cell, chan_mode=chan_mode,
+                                       do_psf=True, do_imaging_weight=True, complex_grid=False)
+    assert g.dtype == np.float64 and g.shape == gref.shape
+    assert rel(g, gref) < 1e-12
+    np.testing.assert_allclose(sw, swref, rtol=1e-12)
+
+
+def test_grid_imaging_weight_with_data_matches_oracle(gpu, oracle):
+    """standard_grid_jit with do_imaging_weight on visibilities (complex
+    grid): both scatters carry vis * weight."""
+    from test_oracle import imaging_weight_edge_problem
+    from astroviper_amd.gcf_prolate_spheroidal import create_prolate_spheroidal_kernel_1D
+    from astroviper_amd.standard_grid import standard_grid_jit
+    uvw, freq, cell, w, n_uv = imaging_weight_edge_problem(64)
+    rng = np.random.default_rng(5)
+    vis = rng.standard_normal(w.shape) + 1j * rng.standard_normal(w.shape)
+    grid = np.zeros((1, w.shape[3]) + n_uv, dtype=np.complex128)
+    sw = np.zeros((1, w.shape[3]))
+    standard_grid_jit(grid, sw, False, True, vis, uvw, freq, np.zeros(1, dtype=int),
+                      np.arange(w.shape[3]), w, create_prolate_spheroidal_kernel_1D(100, 7),
+                      np.array(n_uv), cell, 7, 100)
+    gref, swref = oracle.standard_grid(vis, uvw, w, freq, n_uv, cell, do_imaging_weight=True)
+    assert rel(grid, gref) < 1e-12
+    np.testing.assert_allclose(sw, swref, rtol=1e-12)

@@ -182,6 +182,90 @@ def test_grid_matches_brute_force(oracle, cube):
     np.testing.assert_allclose(sw, swb, rtol=1e-12)
 
 
+def brute_force_imaging_weight_psf(oracle, uvw, w, freq, cell, n_uv, support=7, ov=100):
+    """Cell-centric sum of do_imaging_weight's PSF (standard_grid.py:318-430)
+    written from its definition: every sample adds its kernel around its grid
+    centre and, with the same kernel weights, around its conjugate centre
+    int(-u + N/2 + 0.5) (cells outside the grid dropped); for two or more
+    polarisations the gridded value is (w0 + w1) / 2 for every polarisation,
+    and the sum of weights counts both scatters."""
+    nt, nb, nchan, npol = w.shape
+    grid = np.zeros((1, npol, n_uv[0], n_uv[1]))
+    sw = np.zeros((1, npol))
+    cgk = oracle.create_prolate_spheroidal_kernel_1D(ov, support)
+    sc = support // 2
+    uu, vv = np.meshgrid(np.arange(n_uv[0]), np.arange(n_uv[1]), indexing="ij")
+
+    def footprint(cu, cv, ou, ovv):
+        du, dv = uu - cu, vv - cv
+        inside = (du >= -sc) & (du < support - sc) & (dv >= -sc) & (dv < support - sc)
+        ku = cgk[np.abs(ov * np.clip(du, -sc, sc) + ou)]
+        kv = cgk[np.abs(ov * np.clip(dv, -sc, sc) + ovv)]
+        return np.where(inside, ku * kv, 0.0)
+
+    for t in range(nt):
+        for b in range(nb):
+            for c in range(nchan):
+                geo = oracle.grid_geometry(uvw[t, b, 0], uvw[t, b, 1], freq[c], cell, n_uv,
+                                           support, ov)
+                if geo is None:
+                    continue
+                iu, iv, ou, ovv = geo
+                u = uvw[t, b, 0] * (-(freq[c] * cell[0] * n_uv[0]) / C)
+                v = uvw[t, b, 1] * (-(freq[c] * cell[1] * n_uv[1]) / C)
+                cu, cv = int(-u + n_uv[0] // 2 + 0.5), int(-v + n_uv[1] // 2 + 0.5)
+                k0 = footprint(iu, iv, ou, ovv)
+                k1 = footprint(cu, cv, ou, ovv)
+                norm = k0.sum()
+                for p in range(npol):
+                    d = (w[t, b, c, 0] + w[t, b, c, 1]) / 2.0 if npol >= 2 else w[t, b, c, p]
+                    if d == 0 or np.isnan(d):
+                        continue
+                    grid[0, p] += (k0 + k1) * d
+                    sw[0, p] += 2 * d * norm
+    return grid, sw
+
+
+def imaging_weight_edge_problem(n):
+    """Samples at f = c with 1/n cells (u = -uvw_u pixels): a random set plus
+    rows whose conjugate footprints reach past the grid edge (u in [-29.5,
+    -28.5] puts the conjugate centre at 61..62 of 64)."""
+    rng = np.random.default_rng(21)
+    uvw, freq, cell, vis, w, flag, n_uv = _random_problem(rng, nt=2, nb=40, nchan=1, n=n)
+    cell = (1.0 / n, 1.0 / n)
+    freq = np.array([C])
+    uvw[..., :2] = rng.uniform(-28.0, 28.0, size=uvw[..., :2].shape)
+    edge = np.array([29.0, 29.3, 28.6, 29.5, -28.4, 0.5])
+    uvw[0, :edge.size, 0] = edge
+    uvw[1, :edge.size, 1] = edge
+    return uvw, freq, cell, w[:, :, :1], (n, n)
+
+
+@pytest.mark.parametrize("n", [64, 63])
+def test_grid_imaging_weight_matches_brute_force(oracle, n):
+    """do_imaging_weight (the conjugate scatter of make_gridding_convolution_function's
+    PSF): the oracle against the cell-centric sum, on an even and an odd grid,
+    with conjugate footprints that reach past the grid edge."""
+    uvw, freq, cell, w, n_uv = imaging_weight_edge_problem(n)
+    g, sw = oracle.standard_grid(None, uvw, w, freq, n_uv, cell, do_psf=True,
+                                 do_imaging_weight=True, complex_grid=False)
+    gb, swb = brute_force_imaging_weight_psf(oracle, uvw, w, freq, cell, n_uv)
+    assert np.max(np.abs(g - gb)) <= 1e-12 * np.max(np.abs(gb))
+    np.testing.assert_allclose(sw, swb, rtol=1e-12)
+    # the conjugate half is a second copy of every footprint: the sum of
+    # weights doubles that of the plain PSF of the averaged weights, and the
+    # grid sum falls short of twice the plain one by the clipped cells only
+    wa = np.broadcast_to(((w[..., 0] + w[..., 1]) / 2.0)[..., None], w.shape).copy()
+    g1, sw1 = oracle.standard_grid(None, uvw, wa, freq, n_uv, cell, do_psf=True,
+                                   complex_grid=False)
+    np.testing.assert_allclose(sw, 2 * sw1, rtol=1e-12)
+    clipped = 2 * g1.sum() - g.sum()
+    if n == 64:
+        assert clipped > 1e-3 * g1.sum() / (w.shape[0] * w.shape[1])  # some cells fell off
+    else:
+        assert abs(clipped) <= 1e-12 * g1.sum()
+
+
 def test_grid_invariants(oracle):
     rng = np.random.default_rng(12)
     uvw, freq, cell, vis, w, flag, n_uv = _random_problem(rng, nt=4, nb=60)
