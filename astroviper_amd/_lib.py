@@ -29,6 +29,7 @@ AV_GRID_ADJOINT = 8
 AV_GRID_SUBTILE = 16
 AV_GRID_F32_PRODUCTS = 32
 AV_DEGRID_INCREMENTAL = 1
+AV_DEGRID_F32_PRODUCTS = 2
 
 AV_GEOM_REFERENCE = 0
 AV_GEOM_DEGRID = 1

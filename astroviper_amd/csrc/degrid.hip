@@ -735,6 +735,383 @@ void k_degrid_mfma(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split-FP16 matrix-core degridder (FP32 grids, 16 x 16 tiles, the default
+// for them).  For a group of 16 records the products T[(v,c)][k] =
+// sum_u G[u][(v,c)] * wu_k(u) run on v_mfma_f32_16x16x32_f16 with K = 32
+// region rows at once (the 24 rows of a 16-row tile plus its halo; the other
+// 8 read a zero block), each operand split into FP16 hi + lo halves and three
+// products per block (hi*hi + hi*lo + lo*hi) with FP32 accumulation.  So no
+// row band, no sort by row and no row-step loop: a group is one table read
+// per lane per row group, one split of its u-weights, and three products per
+// column block of its v-half window.
+//
+// Scales (powers of two, so exact): the weight table carries 2^14 (u- and
+// v-weights at most 1, kept clear of the FP16 subnormals), and the region of
+// a unit is scaled by 2^-E with E the exponent of its largest |value|, which
+// puts it below 1; both cancel in the division by the kernel-weight sum (the
+// ones block and the same v-weights) and the result is multiplied by 2^E.  A
+// constant grid g degrids to g exactly: its scaled region is a power-of-two
+// multiple m of 1, hi = m and lo = 0, so every product and sum of the data
+// is m times the same one of the weight sums.  Grid values of +-inf give NaN
+// (the FP32 kernel gives inf).
+//
+// Region layout in LDS, for the A operands (lane l holds A[l % 16][8 (l/16)
+// + j], j = 0..7): hi and lo planes of [row group 0..2][region entry n =
+// column * 2npol + component][8 rows] FP16, so the eight rows of a lane are
+// one 16-byte read; lanes of row group 3 read a zero block.  The records of a
+// unit are split by v-half only (keys: half 0, half 1, no geometry).
+struct DegridH3Lds {
+  int tab, ahi, alo, zero, sgeo, sitem, sflag, scnt, red, total, ru, rv, rvp, nbn, halo;
+};
+
+template <int NPOL, int NB, int NBW>
+__host__ __device__ inline DegridH3Lds degrid_h3_layout(int ov, int support, int tu, int tv) {
+  DegridH3Lds L;
+  L.halo = support / 2 + 1;
+  L.ru = tu + 2 * L.halo;
+  L.rv = tv + 2 * L.halo;
+  L.rvp = NB * 16 / (2 * NPOL);
+  L.nbn = NB * 16;
+  int o = 0;
+  auto take = [&](int bytes) {
+    const int r = o;
+    o += (bytes + 15) & ~15;
+    return r;
+  };
+  L.tab = take(WTab(ov, support).size() * 4);
+  L.ahi = take(3 * L.nbn * 16);
+  L.alo = take(3 * L.nbn * 16);
+  L.zero = take(16);
+  L.sgeo = take((kSortCap + 16) * 4);
+  L.sitem = take((kSortCap + 16) * 4);
+  L.sflag = take((kSortCap + 16) * 4);
+  L.scnt = take(8 * 4);
+  L.red = take(kDegridWaves * 4);
+  L.total = o;
+  return L;
+}
+
+// The tiles the split-FP16 degridder serves: 16 region rows plus the halo fit
+// its 24 rows, and every region value of a pass is loaded by one item of at
+// most two per thread.
+template <int NPOL, int NB>
+__host__ __device__ constexpr bool degrid_h3_fits(int tu, int tv, int support) {
+  return tu == 16 && tv == 16 && support >= 2 && tu + 2 * (support / 2 + 1) <= 24 &&
+         3 * (NB * 16 / (2 * NPOL)) * NPOL <= 2 * kDegridThreads;
+}
+
+template <int NPOL, int NB, int NBW>
+__global__ __launch_bounds__(kDegridThreads, 6)
+void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
+                 const int32_t *__restrict__ pol_map, const uint8_t *__restrict__ flag,
+                 float *__restrict__ vis_out, int mode) {
+  using MV = H3Float4;
+  constexpr int C = 2 * NPOL;
+  constexpr int CPB = 16 / C;  // cells per 16-entry block
+  constexpr bool kHalves = NBW < NB;
+  constexpr int KB1 = NB - NBW;  // first block of the second half's window
+  constexpr int HK = kSortCap / kDegridThreads;
+  extern __shared__ __align__(16) char smem[];
+  const uint32_t u = xcd_grouped_block(blockIdx.x, kDegridXcdGroup);
+  if (u >= P.dunit_start[P.n_bins]) return;
+  const int S = P.support, sc = S / 2, ov = P.ov;
+  const int Tu = P.tile_u, Tv = P.tile_v;
+  const DegridH3Lds L = degrid_h3_layout<NPOL, NB, NBW>(ov, S, Tu, Tv);
+  float *tab = (float *)(smem + L.tab);
+  _Float16 *ahi = (_Float16 *)(smem + L.ahi), *alo = (_Float16 *)(smem + L.alo);
+  uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
+  uint32_t *s_item = (uint32_t *)(smem + L.sitem);
+  uint32_t *s_flag = (uint32_t *)(smem + L.sflag);
+  uint32_t *s_cnt = (uint32_t *)(smem + L.scnt);
+  float *red = (float *)(smem + L.red);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool incremental = (mode & AV_DEGRID_INCREMENTAL) != 0;
+  const uint32_t polbad = pol_map_skips<NPOL>(pol_map, npol_g);
+  const int Hh = L.halo;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr int nkeys = kHalves ? 3 : 2;
+  const int half_v = Tv / 2;
+
+  const uint4 unit = P.dunits[u];
+  const int64_t b = unit.x;
+  const uint32_t first = unit.y, end = unit.y + unit.z;
+  auto key_of = [&](uint32_t g) {
+    if (g == kInvalidGeo) return nkeys - 1;
+    if constexpr (kHalves) {
+      const int av = (int)((g >> 16) & 0xFFu) - kPosBias;
+      return av >= half_v ? 1 : 0;
+    }
+    return 0;
+  };
+  // a record's flags are loaded with it (in flight during the region load)
+  auto load_flags = [&](uint32_t item) -> uint32_t {
+    if (!flag) return 0u;
+    if (NPOL == 4) return *(const uint32_t *)(flag + (size_t)item * 4);
+    if (NPOL == 2) return *(const uint16_t *)(flag + (size_t)item * 2);
+    return flag[item];
+  };
+  uint32_t pg[HK], pi[HK], pf[HK];
+#pragma unroll
+  for (int k = 0; k < HK; ++k) {
+    const uint32_t r = min(first + tid + k * kDegridThreads, end - 1);
+    const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
+    pg[k] = rw.z;
+    pi[k] = rw.x;
+    pf[k] = load_flags(rw.x);
+  }
+
+  const WTab W(ov, S);
+  {
+    // the weight table times 2^14
+    const int n = W.size();
+    for (int i = tid; i < n; i += kDegridThreads) tab[i] = P.wtab32[i] * 16384.0f;
+  }
+  if (tid < 4) ((uint32_t *)(smem + L.zero))[tid] = 0u;
+  const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
+  const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * Tu, v0 = (int)(tile - tu_i * (uint32_t)P.ntv) * Tv;
+  const float *gplane = grid + (size_t)ic * npol_g * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+  // ---- region: items (row group, column, polarisation), each eight rows of
+  // one complex value; at most two items per thread (checked on the host)
+  constexpr int kItems = 3 * (NB * 16 / (2 * NPOL)) * NPOL;
+  constexpr int kIt = (kItems + kDegridThreads - 1) / kDegridThreads;
+  float vre[kIt][8], vim[kIt][8];
+  float mx = 0.f;
+  {
+    const int rvp = L.rvp;
+    const int ulo = u0 - Hh, vlo = v0 - Hh;
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int e = tid + it * kDegridThreads;
+      const int rv = e % rvp, rest = e / rvp, p = rest % NPOL, kg = rest / NPOL;
+      int q = pol_map ? pol_map[p] : p % npol_g;
+      q = q < 0 ? 0 : (q >= npol_g ? npol_g - 1 : q);
+      const int gv = vlo + rv;
+      const bool colok = e < kItems && rv < L.rv && gv >= 0 && gv < P.n_v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int ru = 8 * kg + j, gu = ulo + ru;
+        const bool ok = colok && ru < L.ru && gu >= 0 && gu < P.n_u;
+        Cplx2<float> val{0.f, 0.f};
+        if (ok) val = *(const Cplx2<float> *)(gplane + q * pstride + ((size_t)gu * P.n_v + gv) * 2);
+        vre[it][j] = val.re;
+        vim[it][j] = val.im;
+        mx = fmaxf(mx, fmaxf(fabsf(val.re), fabsf(val.im)));
+      }
+    }
+  }
+  // the unit's scale: the exponent of the largest |value|
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  if (lane == 0) red[tid >> 6] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  int E;
+  (void)frexpf(mx, &E);  // |value| < 2^E (E = 0 for an all-zero region)
+  E = __builtin_amdgcn_readfirstlane(E);
+  {
+    const int rvp = L.rvp;
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int e = tid + it * kDegridThreads;
+      if (e >= kItems) continue;
+      const int rv = e % rvp, rest = e / rvp, p = rest % NPOL, kg = rest / NPOL;
+      const int n = rv * C + 2 * p;  // region entries n (real), n + 1 (imaginary)
+      float sre[8], sim[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        sre[j] = ldexpf(vre[it][j], -E);
+        sim[j] = ldexpf(vim[it][j], -E);
+      }
+      H3U4 hr, lr, hm, lm;
+      h3_split8(sre, hr, lr);
+      h3_split8(sim, hm, lm);
+      const int o = (kg * L.nbn + n) * 8;
+      *(H3U4 *)(ahi + o) = hr;
+      *(H3U4 *)(alo + o) = lr;
+      *(H3U4 *)(ahi + o + 8) = hm;
+      *(H3U4 *)(alo + o + 8) = lm;
+    }
+  }
+
+  for (uint32_t c0 = first; c0 < end; c0 += kSortCap) {
+    const int n = (int)min((uint32_t)kSortCap, end - c0);
+    if (c0 != first) {  // later passes of a large unit
+#pragma unroll
+      for (int k = 0; k < HK; ++k) {
+        const uint32_t r = c0 + min(tid + k * kDegridThreads, n - 1);
+        const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
+        pg[k] = rw.z;
+        pi[k] = rw.x;
+        pf[k] = load_flags(rw.x);
+      }
+    }
+    // ---- counting sort of the pass's records by v-half, in LDS
+    if (tid < nkeys) s_cnt[tid] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < HK; ++k)
+      if (tid + k * kDegridThreads < n) atomicAdd(&s_cnt[key_of(pg[k])], 1u);
+    __syncthreads();
+    if (tid == 0) {
+      // the second half starts on a group boundary b1; slots [h0, b1) hold
+      // no record
+      const uint32_t h0 = s_cnt[0];
+      const uint32_t b1 = kHalves ? ((h0 + 15u) & ~15u) : h0;
+      uint32_t run = 0;
+      for (int k = 0; k < nkeys; ++k) {
+        const uint32_t cnt = s_cnt[k];
+        s_cnt[k] = run + (k >= 1 ? b1 - h0 : 0u);
+        run += cnt;
+      }
+      s_cnt[nkeys] = b1;
+      s_cnt[nkeys + 1] = (uint32_t)n + (b1 - h0);
+      s_cnt[nkeys + 2] = h0;
+    }
+    __syncthreads();
+    const int b1 = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys]);
+    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys + 1]);
+    const int h0 = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys + 2]);
+    if (tid < b1 - h0) {
+      s_geo[h0 + tid] = kInvalidGeo;
+      s_item[h0 + tid] = 0u;
+      s_flag[h0 + tid] = 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < HK; ++k)
+      if (tid + k * kDegridThreads < n) {
+        const uint32_t pos = atomicAdd(&s_cnt[key_of(pg[k])], 1u);
+        s_geo[pos] = pg[k];
+        s_item[pos] = pi[k];
+        s_flag[pos] = pf[k];
+      }
+    __syncthreads();
+
+    // ---- 16-record groups of the pass
+    const int ngroups = (np + 15) / 16;
+    const H3Half8 ones = {1, 1, 1, 1, 1, 1, 1, 1};
+    for (int g = wave; g < ngroups; g += kDegridWaves) {
+      const int r = g * 16 + li;
+      const uint32_t geo = r < np ? s_geo[r] : kInvalidGeo;
+      const uint32_t item = r < np ? s_item[r] : 0u;
+      const int kb0 = (kHalves && g * 16 >= b1) ? KB1 : 0;
+      const uint32_t fbits = polbad | (r < np ? s_flag[r] : 0u);
+      int au, ou, avv, ovv;
+      unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
+      const int urow0 = au + Hh - sc;   // region row of the first u-tap
+      const int vcol0 = avv + Hh - sc;  // region column of the first v-tap
+      const int nw = W.nw, tcap = (S + 1) * nw;
+      auto tap_read = [&](int x, int offh) { return tab[min(max(x, 0), tcap) + offh]; };
+      // ---- B: this lane's record's u-weights at region rows 8 lk + j
+      float wu[8];
+      {
+        const int uofs = ou + W.half;
+        const int ub = (8 * lk - urow0 + 1) * nw;  // (tap + 1) * nw of row 8 lk
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wu[j] = tap_read(ub + j * nw, uofs);
+      }
+      H3U4 bhu, blu;
+      h3_split8(wu, bhu, blu);
+      const H3Half8 bh = __builtin_bit_cast(H3Half8, bhu), bl = __builtin_bit_cast(H3Half8, blu);
+      // ---- A: the window's blocks of the region (row group 3 reads zeros)
+      const _Float16 *ah_p = lk < 3 ? ahi + (lk * L.nbn + 16 * kb0 + li) * 8
+                                    : (const _Float16 *)(smem + L.zero);
+      const _Float16 *al_p = lk < 3 ? alo + (lk * L.nbn + 16 * kb0 + li) * 8
+                                    : (const _Float16 *)(smem + L.zero);
+      const int astep = lk < 3 ? 128 : 0;
+      MV T[NBW];
+#pragma unroll
+      for (int k = 0; k < NBW; ++k) {
+        const H3Half8 ah = *(const H3Half8 *)(ah_p + k * astep);
+        const H3Half8 al = *(const H3Half8 *)(al_p + k * astep);
+        const MV z = {0.f, 0.f, 0.f, 0.f};
+        MV t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, z, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t, 0, 0, 0);
+        T[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, t, 0, 0, 0);
+      }
+      MV T1;
+      {
+        const MV z = {0.f, 0.f, 0.f, 0.f};
+        T1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ones, bh, z, 0, 0, 0);
+        T1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ones, bl, T1, 0, 0, 0);
+      }
+      // ---- weighted sums over v (lane: record li, entries 4 lk + r of each
+      // block: one cell, components 4 lk % C + r, for 2npol >= 4)
+      const int vcolw = vcol0 - kb0 * CPB;
+      const int vofs = ovv + W.half;
+      constexpr bool kOneCell = C >= 4;
+      constexpr int NVB = kOneCell ? 1 : 4;
+      int vb0[NVB];
+#pragma unroll
+      for (int r2 = 0; r2 < NVB; ++r2) {
+        vb0[r2] = ((4 * lk + r2) / C - vcolw + 1) * nw;
+        asm("" : "+v"(vb0[r2]));
+      }
+      auto wv_at = [&](int k, int r2) { return tap_read(vb0[r2] + k * (CPB * nw), vofs); };
+      float part[4], npart[4];
+#pragma unroll
+      for (int r2 = 0; r2 < 4; ++r2) part[r2] = npart[r2] = 0.f;
+#pragma unroll
+      for (int k = 0; k < NBW; ++k) {
+        if constexpr (kOneCell) {
+          const float wv = wv_at(k, 0);
+#pragma unroll
+          for (int r2 = 0; r2 < 4; ++r2) part[r2] = fmaf(T[k][r2], wv, part[r2]);
+          npart[0] = fmaf(T1[0], wv, npart[0]);
+        } else {
+#pragma unroll
+          for (int r2 = 0; r2 < 4; ++r2) {
+            const float wv = wv_at(k, r2);
+            part[r2] = fmaf(T[k][r2], wv, part[r2]);
+            npart[r2] = fmaf(T1[0], wv, npart[r2]);
+          }
+        }
+      }
+      if constexpr (kOneCell) {
+#pragma unroll
+        for (int r2 = 1; r2 < 4; ++r2) npart[r2] = npart[0];
+      }
+      auto combine = [&](float *x) {
+        if constexpr (C == 2) {
+          x[0] += x[2];
+          x[1] += x[3];
+        }
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) {
+          if constexpr (C <= 4) x[r2] = sum_xor16(x[r2]);
+          x[r2] = sum_xor32(x[r2]);
+        }
+      };
+      combine(part);
+      combine(npart);
+      constexpr int kVals = C == 2 ? 2 : 4;
+      const bool writer = C == 8 ? lk < 2 : lk == 0;
+      auto comp_of = [&](int r2) { return C == 8 ? 4 * lk + r2 : r2; };
+      if (writer && geo != kInvalidGeo) {
+        float *base = vis_out + (size_t)item * C;
+#pragma unroll
+        for (int r2 = 0; r2 < kVals; ++r2) {
+          const int c = comp_of(r2);
+          const float val = ldexpf(div_by_norm(part[r2], npart[r2]), E);
+          // a zero kernel sum adds nothing (standard_degrid.py:482-485)
+          const bool skip = ((fbits >> (8 * (c >> 1))) & 0xFFu) || npart[r2] == 0.f;
+          if (incremental) {
+            if (!skip) base[c] += val;
+          } else {
+            base[c] = skip ? 0.f : val;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next pass rewrites the sorted records
+  }
+}
+
 // Finishing pass: zero the samples no degrid unit visits (off-grid, NaN uvw,
 // channel outside the grid) and, when `flag` is given, the flagged
 // polarisations of every sample.  A workgroup covers 64 rows x 64 channels:
@@ -857,6 +1234,9 @@ static int launch_degrid(av_plan *plan, const void *grid, int npol_g, const int3
   return AV_OK;
 }
 
+template <typename R, int NPOL>
+static int launch_degrid_finish(const PlanDev &d, void *vis_out, int mode, hipStream_t s);
+
 template <typename R, int NPOL, int NB, int NBW = NB>
 static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
                               const uint8_t *flag, const double *cgk, void *vis_out, int mode,
@@ -881,12 +1261,18 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
                      dim3(kDegridThreads), L.total, s, d, (const R *)grid, npol_g, pol_map, flag,
                      cgk, (R *)vis_out, mode);
   AV_LAUNCH_CHECK("k_degrid_mfma");
+  return launch_degrid_finish<R, NPOL>(d, vis_out, mode, s);
+}
+
+// The finishing pass after a matrix-core degridder: zero the samples no unit
+// visited (overwrite mode; the kernels zero the flagged polarisations of the
+// samples they visit).
+template <typename R, int NPOL>
+static int launch_degrid_finish(const PlanDev &d, void *vis_out, int mode, hipStream_t s) {
   if (d.n_rows > 0 && !(mode & AV_DEGRID_INCREMENTAL)) {
     const int64_t nch = d.row_chan ? 1 : d.n_chan;
     if (nch >= 32) {
       const int64_t blocks = d.mask_words_per_chan * ((nch + 63) / 64);
-      // the matrix-core kernel has zeroed the flagged polarisations of every
-      // sample it visited: only the unvisited samples are left
       hipLaunchKernelGGL((k_degrid_finish<R, NPOL>), dim3((unsigned)blocks), dim3(256), 0, s, d,
                          (const uint8_t *)nullptr, (R *)vis_out);
       AV_LAUNCH_CHECK("k_degrid_finish");
@@ -900,12 +1286,60 @@ static int launch_degrid_mfma(av_plan *plan, const void *grid, int npol_g, const
   return AV_OK;
 }
 
+// The split-FP16 degridder, when the plan's tiles suit it and its LDS fits.
+template <int NPOL, int NB, int NBW>
+static int try_degrid_h3(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
+                         const uint8_t *flag, const double *cgk, void *vis_out, int mode,
+                         hipStream_t s, bool &done) {
+  PlanDev &d = plan->d;
+  if (!degrid_h3_fits<NPOL, NB>(d.tile_u, d.tile_v, d.support)) return AV_OK;
+  const DegridH3Lds L = degrid_h3_layout<NPOL, NB, NBW>(d.ov, d.support, d.tile_u, d.tile_v);
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
+  if ((size_t)L.total > prop.lds_per_block) return AV_OK;
+  done = true;
+  {
+    const int rc_lds =
+        ensure_lds_limit<k_degrid_h3<NPOL, NB, NBW>>(L.total, "av_degrid: hipFuncSetAttribute");
+    if (rc_lds) return rc_lds;
+  }
+  int rc = prepare_tables(d, cgk, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_degrid_h3<NPOL, NB, NBW>),
+                     dim3((unsigned)xcd_grouped_grid(d.max_dunits, kDegridXcdGroup)),
+                     dim3(kDegridThreads), L.total, s, d, (const float *)grid, npol_g, pol_map,
+                     flag, (float *)vis_out, mode);
+  AV_LAUNCH_CHECK("k_degrid_h3");
+  return launch_degrid_finish<float, NPOL>(d, vis_out, mode, s);
+}
+
 template <typename R, int NPOL>
 static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
                            const uint8_t *flag, const double *cgk, void *vis_out, int mode,
                            hipStream_t s) {
   // regions of at most 16 column blocks run on the matrix cores
   const int nb = degrid_mfma_blocks_needed<NPOL>(plan->d.tile_v, plan->d.support);
+  if constexpr (sizeof(R) == 4) {
+    // FP32 16 x 16 tiles: split-FP16 products unless FP32 products are asked for
+    if (!(mode & AV_DEGRID_F32_PRODUCTS)) {
+      bool done = false;
+      int rc = AV_OK;
+      if (NPOL == 4 && nb == 12 && degrid_halves_fit<R, NPOL, 12, 8>(plan->d.tile_u, plan->d.tile_v,
+                                                                     plan->d.support))
+        rc = try_degrid_h3<4, 12, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
+                                     done);
+      else if (nb <= 4)
+        rc = try_degrid_h3<NPOL, 4, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
+                                       done);
+      else if (nb <= 8)
+        rc = try_degrid_h3<NPOL, 8, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
+                                       done);
+      else if (nb <= 12)
+        rc = try_degrid_h3<NPOL, 12, 12>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode,
+                                         s, done);
+      if (done || rc) return rc;
+    }
+  }
   if (nb <= 4)
     return launch_degrid_mfma<R, NPOL, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s);
   if (nb <= 8)

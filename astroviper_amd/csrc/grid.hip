@@ -878,9 +878,6 @@ static int launch_grid_mfma(av_plan *plan, const void *vis, const void *weight,
 constexpr int kH3TabExp = 14;  // weight table scale 2^14
 constexpr int kH3NoScale = -100000;
 
-typedef _Float16 H3Half8 __attribute__((ext_vector_type(8)));
-typedef float H3Float4 __attribute__((ext_vector_type(4)));
-typedef uint32_t H3U4 __attribute__((ext_vector_type(4)));
 
 struct H3Lds {
   int tab, nrm, st_d, st_gu, st_gv, wave_bytes, total, nw, pad, rows, nnrm;
@@ -905,58 +902,6 @@ __host__ __device__ inline H3Lds h3_lds_layout(int ov, int support) {
   L.wave_bytes = L.st_gv + 64 * 4;
   L.total = L.st_d + kGridMaxWaves * L.wave_bytes;
   return L;
-}
-
-// Eight FP32 values a_j as FP16 halves, two per register: hi_j = fp16(a_j)
-// (v_cvt_pk_f16_f32, round to nearest even) and lo_j = fp16(a_j - hi_j).
-// v_fma_mix forms its FMA in FP32 and rounds the result to FP16; mixlo
-// writes bits 0-15 of its register and mixhi bits 16-31, keeping the other
-// half.  The closing s_nop gives the wait states a matrix-core read of a
-// VALU result needs (the compiler cannot see which instruction of the block
-// writes which register).
-__device__ inline void h3_split8(const float *a, H3U4 &hi, H3U4 &lo) {
-  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
-  asm("v_cvt_pk_f16_f32 %0, %8, %9\n\t"
-      "v_cvt_pk_f16_f32 %1, %10, %11\n\t"
-      "v_cvt_pk_f16_f32 %2, %12, %13\n\t"
-      "v_cvt_pk_f16_f32 %3, %14, %15\n\t"
-      "v_fma_mixlo_f16 %4, %8, 1.0, -%0 op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixhi_f16 %4, %9, 1.0, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixlo_f16 %5, %10, 1.0, -%1 op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixhi_f16 %5, %11, 1.0, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixlo_f16 %6, %12, 1.0, -%2 op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixhi_f16 %6, %13, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixlo_f16 %7, %14, 1.0, -%3 op_sel_hi:[0,0,1]\n\t"
-      "v_fma_mixhi_f16 %7, %15, 1.0, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-      "s_nop 1"
-      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
-      : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]),
-        "v"(a[7]));
-  hi = H3U4{h0, h1, h2, h3};
-  lo = H3U4{l0, l1, l2, l3};
-}
-
-// Eight FP32 products x_j = w_j * d_j as FP16 halves, two per register:
-// hi_j = fp16(x_j) and lo_j = fp16(x_j - hi_j), two v_fma_mix each.
-__device__ inline void h3_split_products8(const float *w, const float *d, H3U4 &hi, H3U4 &lo) {
-  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
-#define AV_H3_PAIR(H, L, W0, D0, W1, D1)                                      \
-  "v_fma_mixlo_f16 " H ", " W0 ", " D0 ", 0\n\t"                            \
-  "v_fma_mixhi_f16 " H ", " W1 ", " D1 ", 0\n\t"                            \
-  "v_fma_mixlo_f16 " L ", " W0 ", " D0 ", -" H " op_sel_hi:[0,0,1]\n\t"     \
-  "v_fma_mixhi_f16 " L ", " W1 ", " D1 ", -" H " op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-  asm(AV_H3_PAIR("%0", "%4", "%8", "%16", "%9", "%17")
-      AV_H3_PAIR("%1", "%5", "%10", "%18", "%11", "%19")
-      AV_H3_PAIR("%2", "%6", "%12", "%20", "%13", "%21")
-      AV_H3_PAIR("%3", "%7", "%14", "%22", "%15", "%23")
-      "s_nop 1"
-      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
-      : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "v"(w[6]),
-        "v"(w[7]), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]), "v"(d[4]), "v"(d[5]),
-        "v"(d[6]), "v"(d[7]));
-#undef AV_H3_PAIR
-  hi = H3U4{h0, h1, h2, h3};
-  lo = H3U4{l0, l1, l2, l3};
 }
 
 template <int NPOL, bool PSF>

@@ -13,7 +13,7 @@ from typing import Optional, Tuple
 import torch
 
 from . import _lib
-from ._lib import (AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_CONJUGATE, AV_GEOM_DEGRID,
+from ._lib import (AV_DEGRID_F32_PRODUCTS, AV_DEGRID_INCREMENTAL, AV_F32, AV_F64, AV_GEOM_CONJUGATE, AV_GEOM_DEGRID,
                    AV_PLAN_ROW_CHANNELS,
                    AV_GEOM_REFERENCE, AV_GRID_ACCUMULATE, AV_GRID_ADJOINT,
                    AV_GRID_COMPENSATED, AV_GRID_F32_PRODUCTS, AV_GRID_PSF, AV_GRID_SUBTILE,
@@ -260,8 +260,11 @@ class GridPlan:
 
     def degrid(self, grid: torch.Tensor, flag: Optional[torch.Tensor] = None,
                vis_out: Optional[torch.Tensor] = None, *, n_pol: Optional[int] = None,
-               pol_map: Optional[torch.Tensor] = None, incremental: bool = False):
-        """Predict samples [samples, n_pol] from grid [n_img_chan, npol_g, n_u, n_v]."""
+               pol_map: Optional[torch.Tensor] = None, incremental: bool = False,
+               f32_products: bool = False):
+        """Predict samples [samples, n_pol] from grid [n_img_chan, npol_g, n_u, n_v].
+        FP32 16 x 16 tiles run the split-FP16 matrix-core degridder unless
+        ``f32_products`` asks for FP32 products."""
         code = _real_code(grid.dtype)
         npol_g = grid.shape[1]
         if n_pol is None:
@@ -280,7 +283,8 @@ class GridPlan:
             pol_map = pol_map.to(self.device, torch.int32).contiguous()
         if ns == 0:
             return vis_out
-        mode = AV_DEGRID_INCREMENTAL if incremental else 0
+        mode = ((AV_DEGRID_INCREMENTAL if incremental else 0)
+                | (AV_DEGRID_F32_PRODUCTS if f32_products else 0))
         call("av_degrid", self._h, _ptr(grid.contiguous()), int(npol_g), _ptr(pol_map),
              _ptr(flag), int(n_pol), _ptr(self.cgk), _ptr(vis_out), code, mode, _stream())
         return vis_out
@@ -324,11 +328,18 @@ def grid_kernel_name(tile, n_pol: int, fp64: bool = False, compensated: bool = F
     return "k_grid_tiles"
 
 
-def degrid_kernel_name(tile, n_pol: int, support: int = 7) -> str:
-    """Name of the kernel av_degrid runs: the matrix-core kernel when the
-    tile plus its halo of support//2 + 1 pixels spans at most 16 blocks of
-    16 components per row, else the LDS kernel."""
-    blocks = ((int(tile[1]) + 2 * (support // 2 + 1)) * 2 * n_pol + 15) // 16
+def degrid_kernel_name(tile, n_pol: int, support: int = 7, fp64: bool = False,
+                       f32_products: bool = False) -> str:
+    """Name of the kernel av_degrid runs: for FP32 16 x 16 tiles whose
+    rows plus the halo of support//2 + 1 pixels fit 24 region rows the
+    split-FP16 matrix-core kernel (unless ``f32_products``), else the FP32 or
+    FP64 matrix-core kernel when the tile plus its halo spans at most 16
+    blocks of 16 components per row, else the LDS kernel."""
+    halo = support // 2 + 1
+    blocks = ((int(tile[1]) + 2 * halo) * 2 * n_pol + 15) // 16
+    if (not fp64 and not f32_products and (int(tile[0]), int(tile[1])) == (16, 16)
+            and support >= 2 and 16 + 2 * halo <= 24 and blocks <= 12):
+        return "k_degrid_h3"
     return "k_degrid_mfma" if blocks <= 16 else "k_degrid_tiles"
 
 

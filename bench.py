@@ -266,8 +266,8 @@ def main():
     ap.add_argument("--subtile", action="store_true",
                     help="grid FP32 16x16 tiles with the sub-tile scheduled 4x4x1 kernel")
     ap.add_argument("--f32-products", action="store_true",
-                    help="grid FP32 16x16 tiles with FP32 products (16x16x4 kernel) instead "
-                         "of split-FP16 operands")
+                    help="grid and degrid FP32 16x16 tiles with FP32 products (16x16x4 "
+                         "kernels) instead of split-FP16 operands")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -324,7 +324,7 @@ def main():
                   subtile=args.subtile, f32_products=args.f32_products)
         if ev is not None:
             ev[2].record()
-        plan.degrid(grid, fl2, model)
+        plan.degrid(grid, fl2, model, f32_products=args.f32_products)
         if ev is not None:
             ev[3].record()
 
@@ -359,7 +359,7 @@ def main():
     achieved = bdom / (tdom / 1e3) / 1e9
     names = {"grid": grid_kernel_name(args.tile, npol, compensated=args.compensated,
                                       subtile=args.subtile, f32_products=args.f32_products),
-             "degrid": degrid_kernel_name(args.tile, npol, 7)}
+             "degrid": degrid_kernel_name(args.tile, npol, 7, f32_products=args.f32_products)}
     roofline = {"bound": "hbm", "kernel": names[dom],
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved * 1e9 / HBM_PEAK, 4),
@@ -470,7 +470,7 @@ def main():
                                f"10% flagged",
                    "mode": args.mode, "tile": list(args.tile), "visibilities": V,
                    "compensated": bool(args.compensated), "subtile": bool(args.subtile),
-                   "grid_kernel": names["grid"]},
+                   "grid_kernel": names["grid"], "degrid_kernel": names["degrid"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

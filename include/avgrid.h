@@ -43,6 +43,8 @@ extern "C" {
                                    instead of from split FP16 operands */
 /* av_degrid mode bits */
 #define AV_DEGRID_INCREMENTAL 1 /* add into vis_out instead of overwriting */
+#define AV_DEGRID_F32_PRODUCTS 2 /* FP32 16 x 16 tiles: FP32 products (16x16x4
+                                    kernel) instead of split FP16 operands */
 
 /* av_plan_create geometry */
 #define AV_GEOM_REFERENCE 0 /* grid uses standard_grid_jit rounding and

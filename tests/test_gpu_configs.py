@@ -7,8 +7,9 @@ instantiations the benchmark and the precision sweep time.
   (1.0 and 1.4 GHz), cube and continuum, FP32 and FP64.  These plans run
   the split-FP16 kernel k_grid_h3<4> (the benchmark's FP32 gridder), the
   FP32-products kernel k_grid_mfma<float,4,1,1>, the sub-tile kernel,
-  k_grid_mfma<double,4,1,1> and the v-half degridder k_degrid_mfma
-  (4 polarisations, 12 column blocks, 8 per half).
+  k_grid_mfma<double,4,1,1>, and the v-half degridders (4 polarisations,
+  12 column blocks, 8 per half): split-FP16 k_degrid_h3<4,12,8> (the
+  benchmark's), and k_degrid_mfma for FP32 products and FP64.
 * The same generator at 4096 x 4096 (the precision sweep's grid), 40 time
   samples x 2 channels, cube: FP32, FP32 compensated and FP64.
 
@@ -97,7 +98,8 @@ def run_gpu(gpu, d, mode, prec, compensated=False, tile=(16, 16), subtile=False,
               "k_grid_mfma" if not plain32 or f32_products else "k_grid_h3")
     assert grid_kernel_name(tile, npol, fp64=prec == 64, compensated=compensated,
                             subtile=subtile, f32_products=f32_products) == expect
-    assert degrid_kernel_name(tile, npol, 7) == "k_degrid_mfma"
+    assert degrid_kernel_name(tile, npol, 7, fp64=prec == 64, f32_products=f32_products) == (
+        "k_degrid_h3" if prec == 32 and not f32_products else "k_degrid_mfma")
     plan = GridPlan(rows, nchan, nchan if cube else 1, d["n_uv"], d["cell"], 7, 100,
                     tile=tile, device=gpu)
     plan.set_uvw(torch.from_numpy(np.ascontiguousarray(d["uvw"].reshape(rows, 3))).to(gpu),
@@ -163,7 +165,8 @@ def test_benchmark_workload_subset(gpu, oracle, c2_subset, mode, prec, variant):
     cdt = torch.complex64 if prec == 32 else torch.complex128
     gin = ref["grid"].astype(np.complex64).astype(np.complex128) if prec == 32 else ref["grid"]
     mref = oracle.degrid(d["uvw"], d["flag"], d["freq"], gin, d["cell"])
-    m = got["plan"].degrid(torch.from_numpy(gin).to(gpu, cdt), got["flag"]).cpu().numpy()
+    m = got["plan"].degrid(torch.from_numpy(gin).to(gpu, cdt), got["flag"],
+                           f32_products=variant == "f32_products").cpu().numpy()
     m = m.reshape(mref.shape)
     e_m = rel(m, mref)
     print(f"2048^2 {mode} FP{prec}: degrid {e_m:.2e}", flush=True)

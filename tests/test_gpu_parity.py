@@ -315,14 +315,18 @@ def test_accumulate_mode(gpu, oracle):
 # ------------------------------------------------------------- degridding
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("cube", [False, True])
-def test_degrid_c1_matches_oracle(gpu, oracle, prec, cube):
+@pytest.mark.parametrize("tile", [(32, 32), (16, 16)])
+def test_degrid_c1_matches_oracle(gpu, oracle, prec, cube, tile):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True, flag_frac=0.1)
     rng = np.random.default_rng(5)
     nchan_g = freq.size if cube else 1
     grid = rng.standard_normal((nchan_g, 2) + n_uv) + 1j * rng.standard_normal((nchan_g, 2) + n_uv)
     grid = as_prec(prec, grid)
     mref = oracle.degrid(uvw, flag, freq, grid, cell)
-    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, cube, 64)
+    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, cube, 64, tile=tile)
+    from astroviper_amd.device import degrid_kernel_name
+    assert degrid_kernel_name(tile, 2, fp64=prec == 64) == (
+        "k_degrid_h3" if (prec, tile) == (32, (16, 16)) else "k_degrid_mfma")
     cdt = torch.complex64 if prec == 32 else torch.complex128
     gd = torch.from_numpy(grid).to(gpu, cdt)
     fl = torch.from_numpy(flag.reshape(-1, 2).astype(np.uint8)).to(gpu)
@@ -335,9 +339,10 @@ def test_degrid_c1_matches_oracle(gpu, oracle, prec, cube):
     assert rel(m2, mref + 1.0) < (TOL32 if prec == 32 else TOL64)
 
 
-def test_degrid_pol_map_and_constant_grid(gpu, oracle):
+@pytest.mark.parametrize("tile", [(32, 32), (16, 16)])
+def test_degrid_pol_map_and_constant_grid(gpu, oracle, tile):
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
-    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 64)
+    _, _, plan = device_grid(gpu, vis, uvw, w, None, freq, n_uv, cell, False, 64, tile=tile)
     g = torch.full((1, 1) + n_uv, 2.0 + 0j, dtype=torch.complex128, device=gpu)
     m = plan.degrid(g, n_pol=2, pol_map=torch.zeros(2, dtype=torch.int32)).cpu().numpy()
     on = np.abs(m) > 0
@@ -519,8 +524,9 @@ def test_degrid_skips_out_of_range_pol_and_zero_norm(gpu, oracle, prec):
     grid = rng.standard_normal((1, 1) + n_uv) + 1j * rng.standard_normal((1, 1) + n_uv)
     grid = as_prec(prec, grid)
     from astroviper_amd.device import degrid_kernel_name
-    assert degrid_kernel_name((16, 16), 2) == "k_degrid_mfma"
-    assert degrid_kernel_name((8, 64), 2) == "k_degrid_tiles"
+    assert degrid_kernel_name((16, 16), 2, fp64=prec == 64) == (
+        "k_degrid_h3" if prec == 32 else "k_degrid_mfma")
+    assert degrid_kernel_name((8, 64), 2, fp64=prec == 64) == "k_degrid_tiles"
     for tile in ((16, 16), (8, 64)):  # the matrix-core and the LDS degridder
         plan = GridPlan(rows, nchan, 1, n_uv, cell, tile=tile, device=gpu)
         plan.set_uvw(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu),
