@@ -18,8 +18,13 @@ import torch
 
 from . import device as _dev
 from .msv4 import datasets, is_datatree
+from .staging import StagedGridder, array_partition
 
-# Plans are limited to 2^32/5 samples; larger calls are cut into row blocks.
+# Samples per staged block (rows x channels): bounds the device buffers and
+# the plan of one call.
+_BLOCK_SAMPLES = 1 << 24
+# Plans are limited to 2^32/5 samples; larger degrid calls are cut into row
+# blocks.
 _MAX_PLAN_SAMPLES = 1 << 28
 
 
@@ -42,43 +47,23 @@ def grid_to_device(vis_data, uvw, weight, flag, freq_chan, chan_map, n_img_chan,
                    delta_lm, support, oversampling, do_psf, prec, cgk_dev,
                    grid_dev=None, sumwt_dev=None, tile=(16, 16), geometry="reference"):
     """Grid host arrays [nt, nb, nchan, npol] on the GPU, accumulating into
-    grid_dev [n_img_chan, npol, n_u, n_v] / sumwt_dev when given.
+    grid_dev [n_img_chan, npol, n_u, n_v] / sumwt_dev when given.  The rows
+    stream to the device in blocks through pinned buffers, with one plan
+    reused for every block (staging.StagedGridder).
     ``geometry="conjugate"`` places every sample at its conjugate centre."""
-    dev = _dev.require_gpu()
     weight = np.asarray(weight)
     nt, nb, nchan, npol = weight.shape
-    rows = nt * nb
-    rdt = torch.float32 if prec == _dev.AV_F32 else torch.float64
-    cdt = torch.complex64 if prec == _dev.AV_F32 else torch.complex128
-    if grid_dev is None:
-        grid_dev = torch.zeros((n_img_chan, npol, int(n_uv[0]), int(n_uv[1])), dtype=cdt,
-                               device=dev)
-    if sumwt_dev is None:
-        sumwt_dev = torch.zeros((n_img_chan, npol), dtype=torch.float64, device=dev)
-    uvw_h = np.ascontiguousarray(np.asarray(uvw, dtype=np.float64).reshape(rows, 3))
-    w_h = weight.reshape(rows, nchan, npol)
-    v_h = None if do_psf else np.asarray(vis_data).reshape(rows, nchan, -1)[:, :, :npol]
-    f_h = None if flag is None else np.asarray(flag).reshape(rows, nchan, npol)
-    freq_t = torch.from_numpy(np.ascontiguousarray(freq_chan, dtype=np.float64)).to(dev)
-    cmap_t = torch.from_numpy(np.ascontiguousarray(chan_map, dtype=np.int32)).to(dev)
-    step = max(1, _MAX_PLAN_SAMPLES // max(1, nchan))
-    for r0 in range(0, rows, step):
-        r1 = min(rows, r0 + step)
-        plan = _dev.GridPlan(r1 - r0, nchan, n_img_chan, n_uv, delta_lm, support,
-                             oversampling, tile=tile, geometry=geometry, device=dev)
-        plan.cgk = cgk_dev
-        plan.set_uvw(torch.from_numpy(uvw_h[r0:r1]).to(dev), freq_t, cmap_t)
-        w_t = torch.from_numpy(np.ascontiguousarray(w_h[r0:r1])).to(dev, rdt)
-        v_t = None
-        if v_h is not None:
-            v_t = torch.from_numpy(np.ascontiguousarray(v_h[r0:r1])).to(dev).to(cdt)
-        f_t = None
-        if f_h is not None:
-            f_t = torch.from_numpy(np.ascontiguousarray(f_h[r0:r1]).astype(np.uint8)).to(dev)
-        plan.grid(v_t, w_t, f_t, grid=grid_dev, sumwt=sumwt_dev, n_pol=npol, psf=do_psf,
-                  accumulate=True, dtype=rdt)
-        plan.close()
-    return grid_dev, sumwt_dev
+    sg = StagedGridder(n_uv, delta_lm, nchan, npol, support=support, oversampling=oversampling,
+                       fp32=prec == _dev.AV_F32, block_rows=max(nb, _BLOCK_SAMPLES // max(1, nchan)),
+                       tile=tile, psf=do_psf, cgk=cgk_dev, geometry=geometry,
+                       chan_map=chan_map, n_img_chan=n_img_chan, grid=grid_dev,
+                       sumwt=sumwt_dev)
+    try:
+        sg.add(array_partition(None if do_psf else vis_data, np.asarray(uvw, dtype=np.float64),
+                               weight, flag, np.asarray(freq_chan, dtype=np.float64)))
+    finally:
+        sg.close()
+    return sg.grid, sg.sumwt
 
 
 def standard_grid_jit(grid, sum_weight, do_psf, do_imaging_weight, vis_data, uvw, freq_chan,
