@@ -34,7 +34,8 @@ constexpr double kLightSpeed = 299792458.0;  // standard_grid.py:296
 constexpr uint32_t kInvalidGeo = 0xFFFFFFFFu;
 constexpr int kPosBias = 16;   // tile-relative positions are stored + 16
 constexpr int kOffBias = 128;  // kernel offsets are stored + 128
-constexpr int kMaxSupport = 16;
+constexpr int kMaxSupport = 16;      // av_grid / av_degrid kernels
+constexpr int kMaxPlanSupport = 63;  // binning only (the mosaic gridder)
 constexpr int kMaxPol = 4;
 constexpr int kMaxDevices = 64;
 constexpr int kChunkThreads = 1024;       // threads of a chunked-binning workgroup
@@ -201,6 +202,11 @@ int launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n,
                        uint32_t *tmp, hipStream_t s);
 int prepare_tables(const PlanDev &d, const double *cgk, hipStream_t s);
+// grid.hip: overwrite-mode zeroing before a tile kernel (sums of weights and
+// split-bin tiles), and the per-unit sums of weights into sumwt after it
+int zero_before_grid(const PlanDev &d, void *grid, int npol, bool fp64, double *sumwt,
+                     hipStream_t s);
+int reduce_sumwt(const PlanDev &d, int npol, double *sumwt, hipStream_t s);
 
 // properties of the current device, queried once per device
 struct DeviceProps {

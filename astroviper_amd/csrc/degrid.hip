@@ -1384,6 +1384,8 @@ extern "C" int av_degrid(av_plan *plan, const void *grid, int32_t n_grid_pol,
   if (n_grid_pol < 1 || n_grid_pol > kMaxPol)
     return fail(AV_ERR_UNSUPPORTED, "av_degrid: n_grid_pol must be in [1, 4]");
   if (!plan->d.uvw) return fail(AV_ERR_INVALID, "av_degrid: plan has no uvw (av_plan_set_uvw)");
+  if (plan->d.support > kMaxSupport)
+    return fail(AV_ERR_UNSUPPORTED, "av_degrid: the plan's support exceeds 16");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == AV_F32)
     return degrid_by_pol<float>(plan, grid, n_grid_pol, pol_map, flag, n_pol, cgk, vis_out, mode, s);

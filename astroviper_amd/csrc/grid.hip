@@ -1627,18 +1627,38 @@ static int dispatch_grid(av_plan *plan, const void *vis, const void *weight, con
   return launch_grid<R, NPOL, 16, false>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
 }
 
+// Overwrite mode, before a tile kernel: zero the sums of weights and the
+// tiles of split bins (the units of a split bin add into them).
+int zero_before_grid(const PlanDev &d, void *grid, int npol, bool fp64, double *sumwt,
+                     hipStream_t s) {
+  int64_t n = (int64_t)d.n_img_chan * npol;
+  hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sumwt, n);
+  AV_LAUNCH_CHECK("k_zero_f64");
+  const unsigned zb = (unsigned)std::min<int64_t>((d.max_gunits + 255) / 256, 8192);
+  if (fp64)
+    hipLaunchKernelGGL(k_zero_multi<double>, dim3(zb), dim3(256), 0, s, d, (double *)grid, npol);
+  else
+    hipLaunchKernelGGL(k_zero_multi<float>, dim3(zb), dim3(256), 0, s, d, (float *)grid, npol);
+  AV_LAUNCH_CHECK("k_zero_multi");
+  return AV_OK;
+}
+
+// After a tile kernel: the per-unit partial sums of weights into sumwt.
+int reduce_sumwt(const PlanDev &d, int npol, double *sumwt, hipStream_t s) {
+  hipLaunchKernelGGL(k_sumwt_reduce, dim3((unsigned)d.n_img_chan), dim3(256), 0, s, d, npol,
+                     sumwt);
+  AV_LAUNCH_CHECK("k_sumwt_reduce");
+  return AV_OK;
+}
+
 template <typename R>
 static int grid_by_pol(av_plan *plan, const void *vis, const void *weight, const uint8_t *flag,
                        int npol, const double *cgk, void *grid, double *sumwt, int mode,
                        hipStream_t s) {
   PlanDev &d = plan->d;
   if (!(mode & AV_GRID_ACCUMULATE)) {
-    int64_t n = (int64_t)d.n_img_chan * npol;
-    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sumwt, n);
-    AV_LAUNCH_CHECK("k_zero_f64");
-    const unsigned zb = (unsigned)std::min<int64_t>((d.max_gunits + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_zero_multi<R>, dim3(zb), dim3(256), 0, s, d, (R *)grid, npol);
-    AV_LAUNCH_CHECK("k_zero_multi");
+    const int rc = zero_before_grid(d, grid, npol, sizeof(R) == 8, sumwt, s);
+    if (rc) return rc;
   }
   switch (npol) {
     case 1: return dispatch_grid<R, 1>(plan, vis, weight, flag, cgk, grid, sumwt, mode, s);
@@ -1661,6 +1681,8 @@ extern "C" int av_grid(av_plan *plan, const void *vis, const void *weight, const
   if (!adjoint && !weight) return fail(AV_ERR_INVALID, "av_grid: weight is NULL");
   if (psf && adjoint) return fail(AV_ERR_INVALID, "av_grid: PSF and ADJOINT are exclusive");
   if (!plan->d.uvw) return fail(AV_ERR_INVALID, "av_grid: plan has no uvw (av_plan_set_uvw)");
+  if (plan->d.support > kMaxSupport)
+    return fail(AV_ERR_UNSUPPORTED, "av_grid: the plan's support exceeds 16");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == AV_F32)
     return grid_by_pol<float>(plan, vis, weight, flag, n_pol, cgk, grid, sumwt, mode, s);

@@ -1,0 +1,285 @@
+// mosaic.hip -- the mosaic / A-projection gridder (replaces mosaic_grid_jit,
+// core/imaging/imaging_utils/mosaic_grid.py:306-486).
+//
+// Each sample is convolved with a non-separable, oversampled complex
+// convolution function (already multiplied by the field's phase gradient)
+// chosen by its baseline, channel and polarisation, over a support that the
+// same three indices choose.  The samples come binned by a plan whose
+// support is 2 * max_support + 1: its bounds test is then the reference's
+// (u_centre +- max_support inside the grid) and every footprint lies inside
+// the tiles the plan records the sample in.
+//
+// One workgroup (one wavefront) per work unit.  The unit's tile, all its
+// polarisations, is accumulated in LDS.  Records are staged 64 at a time
+// (one per lane: its FP64 position, kernel offsets, convolution-function
+// bases and supports, weighted data); the wavefront then takes the staged
+// records one after another, its lanes over the record's taps.  A tap reads
+// its complex kernel value from the convolution function in global memory
+// (read-only, shared by every unit of the same baseline group, so it stays
+// in L2) and adds kernel * data into its tile cell: the taps of one record
+// are distinct cells and only this wavefront touches the tile, so no
+// atomics.  The kernel sum of a record (the reference's `norm`, over all its
+// taps) is reduced across the wavefront for primary records only and gives
+// the sum of weights: weight * Re(norm) for the PSF, weight * Re(norm^2)
+// for data (mosaic_grid.py:470-484).  The tile then goes to HBM once, split
+// bins adding theirs with atomics.
+#include "tile_common.h"
+
+namespace av {
+
+constexpr int kMosaicStage = 64;
+
+template <typename R>
+struct MosaicArgs {
+  const R *vis, *weight;
+  const int32_t *cf_baseline_map, *cf_chan_map, *cf_pol_map;
+  const R *cfpg;  // complex [ncb][ncc][ncp][cu][cv]
+  const int32_t *weight_support;  // [ncb][ncc][ncp][2]
+  int n_baseline, ncc, ncp, cu_n, cv_n, ov_u, ov_v;
+  R *grid;
+};
+
+// per staged record: centre in tile coordinates, cf row/column of tap 0,
+// and per polarisation the cf base, supports and weighted data
+template <typename R, int NPOL>
+struct MosaicStaged {
+  int cu, cv, cfu0, cfv0;
+  int base[NPOL], su[NPOL], sv[NPOL];
+  R dre[NPOL], dim[NPOL], w[NPOL];
+  int prim;
+};
+
+template <typename R, int NPOL>
+__global__ __launch_bounds__(64) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, int psf,
+                                                    int accumulate) {
+  extern __shared__ __align__(16) char smem[];
+  using St = MosaicStaged<R, NPOL>;
+  const int T = P.tile_u, TV = P.tile_v;
+  Cplx2<R> *tile = (Cplx2<R> *)smem;  // [cell][pol]
+  St *st = (St *)(smem + (size_t)T * TV * NPOL * sizeof(Cplx2<R>));
+  const int lane = threadIdx.x;
+  const uint32_t u = blockIdx.x;
+  const uint32_t n_units = P.gunit_start[P.n_bins];
+  if (u >= n_units) return;
+  const uint4 unit = P.gunits[u];
+  const uint32_t b = unit.x, first = unit.y, cnt = unit.z, fl = unit.w;
+  const uint32_t tile_i = b / (uint32_t)P.n_img_chan;
+  const int ic = (int)(b - tile_i * (uint32_t)P.n_img_chan);
+  const uint32_t tu_i = tile_i / (uint32_t)P.ntv;
+  const int u0 = (int)tu_i * T, v0 = (int)(tile_i - tu_i * (uint32_t)P.ntv) * TV;
+  const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
+  R *gplane = A.grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
+  const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+
+  for (int i = lane; i < T * TV * NPOL; i += 64) tile[i] = Cplx2<R>{(R)0, (R)0};
+  double sw[NPOL];
+#pragma unroll
+  for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+  const int ccu = A.cu_n / 2, ccv = A.cv_n / 2;
+  const size_t cfsize = (size_t)A.cu_n * A.cv_n;
+
+  for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
+    __syncthreads();  // the previous batch's staged records are consumed
+    // ---- stage this lane's record
+    {
+      St s;
+      s.prim = 0;
+      s.cu = s.cv = s.cfu0 = s.cfv0 = 0;
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        s.base[p] = s.su[p] = s.sv[p] = 0;
+        s.dre[p] = s.dim[p] = s.w[p] = (R)0;
+      }
+      const uint32_t r = base + lane;
+      if (r < first + cnt) {
+        const uint32_t item = P.rec[3 * (size_t)r];
+        const uint32_t geo = P.rec[3 * (size_t)r + 1];
+        int au, ou, avv, ovv;
+        unpack_geo(geo, au, ou, avv, ovv);
+        const int64_t row = (int64_t)(item / (uint32_t)P.n_chan);
+        const int ch = (int)(item - (uint32_t)row * (uint32_t)P.n_chan);
+        // the reference's offsets at this call's oversampling, from the
+        // same FP64 positions the plan binned (mosaic_grid.py:376-386)
+        const double uu = P.uvw[row * 3] * P.gscale_u[ch] + (double)(P.n_u / 2);
+        const double vv = P.uvw[row * 3 + 1] * P.gscale_v[ch] + (double)(P.n_v / 2);
+        const int gcu = u0 + au, gcv = v0 + avv;
+        s.cu = au;
+        s.cv = avv;
+        s.cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
+        s.cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
+        s.prim = r < prim_end;
+        const int cfb = A.cf_baseline_map[(int)(row % A.n_baseline)];
+        const int cfc = A.cf_chan_map[ch];
+#pragma unroll
+        for (int p = 0; p < NPOL; ++p) {
+          const int cfp = A.cf_pol_map[p];
+          const int idx = (cfb * A.ncc + cfc) * A.ncp + cfp;
+          s.base[p] = idx;
+          s.su[p] = A.weight_support[2 * idx];
+          s.sv[p] = A.weight_support[2 * idx + 1];
+          const R w = A.weight[(size_t)item * NPOL + p];
+          R dre, dim;
+          if (psf) {
+            dre = w;
+            dim = (R)0;
+          } else {
+            dre = A.vis[((size_t)item * NPOL + p) * 2] * w;
+            dim = A.vis[((size_t)item * NPOL + p) * 2 + 1] * w;
+          }
+          // mosaic_grid.py:412: skip NaN or exactly-zero data
+          if (!(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
+            s.dre[p] = dre;
+            s.dim[p] = dim;
+            s.w[p] = w;
+          }
+        }
+      }
+      st[lane] = s;
+    }
+    __syncthreads();
+    // ---- the staged records, one after another; lanes over the taps
+    const int nst = (int)min((uint32_t)kMosaicStage, first + cnt - base);
+    for (int k = 0; k < nst; ++k) {
+      const St &s = st[k];
+#pragma unroll
+      for (int p = 0; p < NPOL; ++p) {
+        const R dre = s.dre[p], dim = s.dim[p];
+        if (dre == (R)0 && dim == (R)0) continue;  // uniform: every lane reads the same record
+        const int su = s.su[p], sv = s.sv[p], scu = su / 2, scv = sv / 2;
+        const Cplx2<R> *cf = (const Cplx2<R> *)A.cfpg + (size_t)s.base[p] * cfsize;
+        double nre = 0.0, nim = 0.0;
+        for (int t = lane; t < su * sv; t += 64) {
+          const int iv = t / su - scv, iu = t % su - scu;
+          const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
+          if (cfu < 0 || cfu >= A.cu_n || cfv < 0 || cfv >= A.cv_n) continue;
+          const Cplx2<R> c = cf[(size_t)cfu * A.cv_n + cfv];
+          nre += (double)c.re;
+          nim += (double)c.im;
+          const int lu = s.cu + iu, lv = s.cv + iv;
+          if (lu >= 0 && lu < T && lv >= 0 && lv < TV) {
+            Cplx2<R> &cell = tile[(lu * TV + lv) * NPOL + p];
+            cell.re += c.re * dre - c.im * dim;
+            cell.im += c.re * dim + c.im * dre;
+          }
+        }
+        if (s.prim) {
+          for (int off = 32; off >= 1; off >>= 1) {
+            nre += __shfl_xor(nre, off);
+            nim += __shfl_xor(nim, off);
+          }
+          const double w = (double)s.w[p];
+          sw[p] += psf ? w * nre : w * (nre * nre - nim * nim);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- flush the tile (cells outside the grid dropped)
+  const bool multi = (fl & kUnitMulti) != 0;
+  for (int i = lane; i < T * TV * NPOL; i += 64) {
+    const int p = i % NPOL, cell = i / NPOL;
+    const int gu = u0 + cell / TV, gv = v0 + cell % TV;
+    if (gu >= P.n_u || gv >= P.n_v) continue;
+    const Cplx2<R> val = tile[i];
+    R *dst = gplane + p * pstride + ((size_t)gu * P.n_v + gv) * 2;
+    if (multi) {
+      if (val.re != (R)0) atomicAdd(dst, val.re);
+      if (val.im != (R)0) atomicAdd(dst + 1, val.im);
+    } else if (accumulate) {
+      dst[0] += val.re;
+      dst[1] += val.im;
+    } else {
+      dst[0] = val.re;
+      dst[1] = val.im;
+    }
+  }
+  if (lane < NPOL) {
+    double v = 0.0;
+#pragma unroll
+    for (int p = 0; p < NPOL; ++p)
+      if (p == lane) v = sw[p];
+    P.upart[(size_t)u * 4 + lane] = v;
+  }
+}
+
+template <typename R, int NPOL>
+static int launch_mosaic(av_plan *plan, const MosaicArgs<R> &A, double *sumwt, int mode,
+                         hipStream_t s) {
+  PlanDev &d = plan->d;
+  const size_t lds = (size_t)d.tile_u * d.tile_v * NPOL * sizeof(Cplx2<R>) +
+                     (size_t)kMosaicStage * sizeof(MosaicStaged<R, NPOL>);
+  const DeviceProps &prop = device_props();
+  if (prop.error) return hip_fail(prop.error, "av_mosaic_grid: device properties");
+  if (lds > prop.lds_per_block)
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: tile too large for LDS (" +
+                                        std::to_string(lds) + " bytes)");
+  {
+    const int rc = ensure_lds_limit<k_mosaic_grid<R, NPOL>>((int)lds,
+                                                            "av_mosaic_grid: hipFuncSetAttribute");
+    if (rc) return rc;
+  }
+  const bool accumulate = (mode & AV_GRID_ACCUMULATE) != 0;
+  if (!accumulate) {
+    const int rc = zero_before_grid(d, A.grid, NPOL, sizeof(R) == 8, sumwt, s);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64), lds, s,
+                     d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
+  AV_LAUNCH_CHECK("k_mosaic_grid");
+  return reduce_sumwt(d, NPOL, sumwt, s);
+}
+
+template <typename R>
+static int mosaic_by_pol(av_plan *plan, MosaicArgs<R> A, int npol, double *sumwt, int mode,
+                         hipStream_t s) {
+  switch (npol) {
+    case 1: return launch_mosaic<R, 1>(plan, A, sumwt, mode, s);
+    case 2: return launch_mosaic<R, 2>(plan, A, sumwt, mode, s);
+    case 4: return launch_mosaic<R, 4>(plan, A, sumwt, mode, s);
+    default: return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: npol must be 1, 2 or 4");
+  }
+}
+
+}  // namespace av
+
+using namespace av;
+
+extern "C" int av_mosaic_grid(av_plan *plan, const void *vis, const void *weight, int32_t n_pol,
+                              int32_t n_baseline, const int32_t *cf_baseline_map,
+                              const int32_t *cf_chan_map, const int32_t *cf_pol_map,
+                              const void *cfpg, int32_t ncc, int32_t ncp, int32_t cu_n,
+                              int32_t cv_n, const int32_t *weight_support, int32_t ov_u,
+                              int32_t ov_v, void *grid, double *sumwt, int32_t dtype,
+                              int32_t mode, void *stream) {
+  if (!plan || !weight || !cf_baseline_map || !cf_chan_map || !cf_pol_map || !cfpg ||
+      !weight_support || !grid || !sumwt)
+    return fail(AV_ERR_INVALID, "av_mosaic_grid: NULL argument");
+  const bool psf = (mode & AV_GRID_PSF) != 0;
+  if (!psf && !vis) return fail(AV_ERR_INVALID, "av_mosaic_grid: vis is NULL");
+  if (mode & ~(AV_GRID_PSF | AV_GRID_ACCUMULATE))
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: mode takes AV_GRID_PSF and "
+                                    "AV_GRID_ACCUMULATE only");
+  const PlanDev &d = plan->d;
+  if (!d.uvw) return fail(AV_ERR_INVALID, "av_mosaic_grid: plan has no uvw (av_plan_set_uvw)");
+  if (d.row_channels || d.geometry != AV_GEOM_REFERENCE || d.conjugate)
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: needs an AV_GEOM_REFERENCE plan "
+                                    "without row channels");
+  if (n_baseline < 1 || d.n_rows % n_baseline != 0)
+    return fail(AV_ERR_INVALID, "av_mosaic_grid: n_baseline must divide the plan's rows");
+  if (ncc < 1 || ncp < 1 || cu_n < 1 || cv_n < 1 || ov_u < 1 || ov_v < 1)
+    return fail(AV_ERR_INVALID, "av_mosaic_grid: bad convolution-function shape");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == AV_F32) {
+    MosaicArgs<float> A{(const float *)vis, (const float *)weight, cf_baseline_map, cf_chan_map,
+                        cf_pol_map, (const float *)cfpg, weight_support, n_baseline, ncc, ncp,
+                        cu_n, cv_n, ov_u, ov_v, (float *)grid};
+    return mosaic_by_pol<float>(plan, A, n_pol, sumwt, mode, s);
+  }
+  if (dtype == AV_F64) {
+    MosaicArgs<double> A{(const double *)vis, (const double *)weight, cf_baseline_map,
+                         cf_chan_map, cf_pol_map, (const double *)cfpg, weight_support,
+                         n_baseline, ncc, ncp, cu_n, cv_n, ov_u, ov_v, (double *)grid};
+    return mosaic_by_pol<double>(plan, A, n_pol, sumwt, mode, s);
+  }
+  return fail(AV_ERR_INVALID, "av_mosaic_grid: dtype must be AV_F32 or AV_F64");
+}

@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 6
+#define AV_ABI_VERSION 7
 
 const char *av_last_error(void);
 int av_version(void);
@@ -113,7 +113,9 @@ int av_ps_outer(const double *k1, int32_t nrow, int32_t nfill,
  *             is one sample at one of them
  * n_img_chan  grid channels; n_u, n_v grid axes 2 and 3
  * cell_u/v    pixelincr in radians (signed)
- * tile_u/v    tile shape in pixels (0 = default 32)
+ * support     kernel support in [2, 63]; av_grid and av_degrid take plans of
+ *             support <= 16, wider ones bin for av_mosaic_grid
+ * tile_u/v    tile shape in pixels (0 = default 32), in [support, 128]
  * geometry    AV_GEOM_REFERENCE or AV_GEOM_DEGRID, optionally or'ed with
  *             AV_PLAN_ROW_CHANNELS (then av_plan_set_uvw needs row_chan);
  *             AV_GEOM_REFERENCE may also be or'ed with AV_GEOM_CONJUGATE
@@ -197,6 +199,28 @@ int av_grid_noalloc(const double *uvw, int64_t n_rows, const double *freq,
                     const uint8_t *flag, int32_t n_pol, const double *cgk,
                     void *grid, double *sumwt, int32_t dtype, int32_t mode,
                     void *stream);
+
+/* mosaic_grid_jit, core/imaging/imaging_utils/mosaic_grid.py:306-486: the
+ * mosaic / A-projection gridder.  Every sample of the plan (created with
+ * AV_GEOM_REFERENCE, support = 2 * max(weight_support) + 1, so that its
+ * bounds test is the reference's, and tiles at least that wide) is
+ * convolved with the complex convolution function cfpg[cfb][cfc][cfp]
+ * [cu_n][cv_n] (conv_kernel * phase_gradient[0], mosaic_grid.py:359) of
+ * cfb = cf_baseline_map[row % n_baseline], cfc = cf_chan_map[chan],
+ * cfp = cf_pol_map[pol], over weight_support[cfb][cfc][cfp][2] taps,
+ * oversampled by ov_u, ov_v.  vis [rows][n_chan][n_pol] complex (ignored
+ * with AV_GRID_PSF), weight [rows][n_chan][n_pol] (imaging weights), grid
+ * [n_img_chan][n_pol][n_u][n_v] complex, sumwt [n_img_chan][n_pol] FP64:
+ * weight * Re(norm) with AV_GRID_PSF, weight * Re(norm^2) without.  mode:
+ * AV_GRID_PSF, AV_GRID_ACCUMULATE.  cf_*_map, weight_support int32 on the
+ * device; cfpg complex64 (AV_F32) or complex128 (AV_F64). */
+int av_mosaic_grid(av_plan *plan, const void *vis, const void *weight, int32_t n_pol,
+                   int32_t n_baseline, const int32_t *cf_baseline_map,
+                   const int32_t *cf_chan_map, const int32_t *cf_pol_map,
+                   const void *cfpg, int32_t ncc, int32_t ncp, int32_t cu_n,
+                   int32_t cv_n, const int32_t *weight_support, int32_t ov_u,
+                   int32_t ov_v, void *grid, double *sumwt, int32_t dtype,
+                   int32_t mode, void *stream);
 
 /* ------------------------------------------------- plain GPU baseline --
  * One thread per sample of ONE visibility channel, global atomics, no

@@ -580,3 +580,95 @@ int ora_degrid_imaging_weights(double *imaging_weight, const double *grid, const
       }
   return 0;
 }
+
+/* mosaic_grid_jit, core/imaging/imaging_utils/mosaic_grid.py:306-486: the
+ * A-projection / mosaic gridder with a non-separable oversampled complex
+ * convolution function per (cf baseline, cf channel, cf polarisation),
+ * already multiplied by phase_gradient[0] (mosaic_grid.py:359, done by the
+ * caller, as conv_kernel * phase_gradient[0, :, :]).  Complex grids only
+ * (numba cannot store the complex product into a real grid).  cfpg is
+ * [ncb][ncc][ncp][cu_n][cv_n] complex, weight_support [ncb][ncc][ncp][2].
+ * The reference does not bound the cf indices; a tap outside the cf array is
+ * skipped here (no in-range input reaches it). */
+void ora_mosaic_grid(double *grid, double *sum_weight, int do_psf, const double *vis,
+                     int nvpol, const double *uvw, int nt, int nb, const double *freq_chan,
+                     int nchan, const int64_t *chan_map, const int64_t *pol_map, int npol,
+                     const int64_t *cf_baseline_map, const int64_t *cf_chan_map,
+                     const int64_t *cf_pol_map, const double *imaging_weight,
+                     const double *cfpg, int ncc, int ncp, int cu_n, int cv_n, int n_img_pol,
+                     int n_u, int n_v, double dl_u, double dl_v, const int64_t *weight_support,
+                     int64_t max_support, int ov_u, int ov_v) {
+  const double c = 299792458.0;
+  const int64_t uc = n_u / 2, vc = n_v / 2;
+  const int64_t conv_v_center = cv_n / 2, conv_u_center = cu_n / 2;
+  const int64_t plane = (int64_t)n_u * n_v;
+  for (int t = 0; t < nt; ++t)
+    for (int b = 0; b < nb; ++b) {
+      const int64_t cf_baseline = cf_baseline_map[b];
+      for (int ch = 0; ch < nchan; ++ch) {
+        const int64_t cf_chan = cf_chan_map[ch];
+        const int64_t a_chan = chan_map[ch];
+        const double su = -(freq_chan[ch] * dl_u * (double)n_u) / c;  /* :336-338 */
+        const double sv = -(freq_chan[ch] * dl_v * (double)n_v) / c;
+        const int64_t row = (int64_t)t * nb + b;
+        const double u = uvw[row * 3 + 0] * su, v = uvw[row * 3 + 1] * sv;
+        if (isnan(u) || isnan(v)) continue;
+        const double u_pos = u + (double)uc, v_pos = v + (double)vc;
+        const double xu = u_pos + 0.5, xv = v_pos + 0.5;
+        if (!(xu > -9.0e18 && xu < 9.0e18 && xv > -9.0e18 && xv < 9.0e18)) continue;
+        const int64_t u_ci = (int64_t)xu, v_ci = (int64_t)xv;
+        if (!((u_ci + max_support < n_u) && (v_ci + max_support < n_v) &&
+              (u_ci - max_support >= 0) && (v_ci - max_support >= 0)))
+          continue;
+        const double u_offset = (double)u_ci - u_pos;
+        const int64_t u_coi = (int64_t)floor(u_offset * (double)ov_u + 0.5) + conv_u_center;
+        const double v_offset = (double)v_ci - v_pos;
+        const int64_t v_coi = (int64_t)floor(v_offset * (double)ov_v + 0.5) + conv_v_center;
+        for (int ip = 0; ip < npol; ++ip) {
+          const int64_t k = (row * nchan + ch) * npol + ip;
+          const double iw = imaging_weight[k];
+          double wd_re, wd_im;
+          if (do_psf) {
+            wd_re = iw;
+            wd_im = 0.0;
+          } else {
+            const int64_t vi = ((row * nchan + ch) * nvpol + ip) * 2;
+            wd_re = vis[vi] * iw;
+            wd_im = vis[vi + 1] * iw;
+          }
+          if (isnan(wd_re) || isnan(wd_im)) continue;
+          if (wd_re == 0.0 && wd_im == 0.0) continue;
+          const int64_t cf_pol = cf_pol_map[ip];
+          const int64_t a_pol = pol_map[ip];
+          const int64_t *sup = weight_support + ((cf_baseline * ncc + cf_chan) * ncp + cf_pol) * 2;
+          const int64_t support_u = sup[0], support_v = sup[1];
+          const int64_t scu = support_u / 2, scv = support_v / 2;
+          const double *cf = cfpg + (((cf_baseline * ncc + cf_chan) * ncp + cf_pol) *
+                                     (int64_t)cu_n * cv_n) * 2;
+          double *g = grid + ((a_chan * n_img_pol) + a_pol) * plane * 2;
+          double n_re = 0.0, n_im = 0.0;
+          for (int64_t i_v = -scv; i_v < support_v - scv; ++i_v) {
+            const int64_t v_indx = v_ci + i_v;
+            const int64_t cf_v = (int64_t)ov_v * i_v + v_coi;
+            for (int64_t i_u = -scu; i_u < support_u - scu; ++i_u) {
+              const int64_t u_indx = u_ci + i_u;
+              const int64_t cf_u = (int64_t)ov_u * i_u + u_coi;
+              if (cf_u < 0 || cf_u >= cu_n || cf_v < 0 || cf_v >= cv_n) continue;
+              const double c_re = cf[(cf_u * cv_n + cf_v) * 2];
+              const double c_im = cf[(cf_u * cv_n + cf_v) * 2 + 1];
+              const int64_t cell = u_indx * n_v + v_indx;
+              g[2 * cell] = g[2 * cell] + (c_re * wd_re - c_im * wd_im);
+              g[2 * cell + 1] = g[2 * cell + 1] + (c_re * wd_im + c_im * wd_re);
+              n_re = n_re + c_re;
+              n_im = n_im + c_im;
+            }
+          }
+          double *sw = sum_weight + a_chan * n_img_pol + a_pol;
+          if (do_psf)
+            *sw = *sw + iw * n_re;
+          else
+            *sw = *sw + iw * (n_re * n_re - n_im * n_im);
+        }
+      }
+    }
+}

@@ -58,8 +58,11 @@ def lib():
         L.ora_grid_adjoint.argtypes = [dp, dp, u8p, I, I, I, I, dp, dp, dp, I, I, I, I, dp, D,
                                        I, I, dp, i64p, i64p]
         L.ora_grid_adjoint.restype = None
+        L.ora_mosaic_grid.argtypes = [dp, dp, I, dp, I, dp, I, I, dp, I, i64p, i64p, I, i64p,
+                                      i64p, i64p, dp, dp, I, I, I, I, I, I, I, D, D, i64p,
+                                      I64, I, I]
         for f in (L.ora_ps_function, L.ora_ps_kernel_1d, L.ora_ps_taper,
-                  L.ora_standard_grid, L.ora_dgrid, L.ora_normalise):
+                  L.ora_standard_grid, L.ora_dgrid, L.ora_normalise, L.ora_mosaic_grid):
             f.restype = None
         _lib = L
     return _lib
@@ -474,3 +477,41 @@ def calculate_briggs_params(grid_of_imaging_weights, sum_weight, imaging_weights
         bf = np.zeros((2, 1, 1) + sum_weight.shape)
         bf[0, :, :] = np.ones((1, 1, 1) + sum_weight.shape)
     return bf
+
+
+# --------------------------------------------------------------------------
+# Mosaic / A-projection gridder  (core/imaging/imaging_utils/mosaic_grid.py)
+# --------------------------------------------------------------------------
+def mosaic_grid(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map, pol_map,
+                cf_baseline_map, cf_chan_map, cf_pol_map, imaging_weight, conv_kernel, n_uv,
+                delta_lm, weight_support, oversampling, phase_gradient):
+    """mosaic_grid_jit (mosaic_grid.py:306-486), accumulating into the
+    complex `grid` [n_chan, n_pol, n_u, n_v] and `sum_weight` in place.
+    conv_kernel [ncb, ncc, ncp, cu, cv] complex, phase_gradient [nfield, cu, cv];
+    the product conv_kernel * phase_gradient[0] (mosaic_grid.py:359) is
+    formed here with NumPy, as the reference does."""
+    assert np.iscomplexobj(grid) and grid.flags.c_contiguous
+    cfpg = np.ascontiguousarray(np.asarray(conv_kernel) * np.asarray(phase_gradient)[0, :, :],
+                                dtype=np.complex128)
+    ncb, ncc, ncp, cu_n, cv_n = cfpg.shape
+    iw = np.ascontiguousarray(imaging_weight, dtype=np.float64)
+    nt, nb, nchan, npol = iw.shape
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    if do_psf:
+        vis_c = np.zeros((1, 1, 1, 1), dtype=np.complex128)
+        nvpol = 1
+    else:
+        vis_c = np.ascontiguousarray(vis_data, dtype=np.complex128)
+        nvpol = vis_c.shape[3]
+    ws = np.ascontiguousarray(weight_support, dtype=np.int64)
+    i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+    cm, pm, cbm, ccm, cpm = (i64(chan_map), i64(pol_map), i64(cf_baseline_map),
+                             i64(cf_chan_map), i64(cf_pol_map))
+    lib().ora_mosaic_grid(
+        _dp(grid.view(np.float64)), _dp(sum_weight), int(do_psf), _dp(vis_c.view(np.float64)),
+        nvpol, _dp(uvw), nt, nb, _dp(np.ascontiguousarray(freq_chan, dtype=np.float64)), nchan,
+        _i64p(cm), _i64p(pm), npol, _i64p(cbm), _i64p(ccm), _i64p(cpm), _dp(iw),
+        _dp(cfpg.view(np.float64)), ncc, ncp, cu_n, cv_n, grid.shape[1], int(n_uv[0]),
+        int(n_uv[1]), float(delta_lm[0]), float(delta_lm[1]), _i64p(ws), int(np.max(ws)),
+        int(oversampling[0]), int(oversampling[1]))
+    return grid, sum_weight
