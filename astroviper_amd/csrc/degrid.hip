@@ -759,11 +759,22 @@ void k_degrid_mfma(
 // Region layout in LDS, for the A operands (lane l holds A[l % 16][8 (l/16)
 // + j], j = 0..7): hi and lo planes of [row group 0..2][region entry n =
 // column * 2npol + component][8 rows] FP16, so the eight rows of a lane are
-// one 16-byte read; lanes of row group 3 read a zero block.  The records of a
-// unit are split by v-half only (keys: half 0, half 1, no geometry).
+// one 16-byte read.  Lanes of row group 3 (region rows 24..31) read row group
+// 2 again: their u-weights are exactly zero (taps past the support read the
+// table's zero row) and the region values are finite, so those products add
+// zeros.  Entry n is stored at h3_swz(n), which keeps every 8-entry half of a
+// 16-entry block in place, so the 16 lanes of a block read still cover 16
+// distinct entries, and sends the stores of consecutive tile columns (8
+// entries apart) to distinct banks.  The records of a unit are split by
+// v-half only (keys: half 0, half 1, no geometry).
 struct DegridH3Lds {
-  int tab, ahi, alo, zero, sgeo, sitem, sflag, scnt, red, total, ru, rv, rvp, nbn, halo;
+  int tab, ahi, alo, sgeo, sitem, sflag, scnt, red, total, ru, rv, rvp, nbn, halo;
 };
+
+__host__ __device__ constexpr int h3_swz(int n) { return n ^ ((n >> 3) & 7); }
+
+// per-(wave, key) counts of a sort pass, then the three pass totals
+constexpr int kH3CntWords = 16;
 
 template <int NPOL, int NB, int NBW>
 __host__ __device__ inline DegridH3Lds degrid_h3_layout(int ov, int support, int tu, int tv) {
@@ -779,14 +790,13 @@ __host__ __device__ inline DegridH3Lds degrid_h3_layout(int ov, int support, int
     o += (bytes + 15) & ~15;
     return r;
   };
-  L.tab = take(WTab(ov, support).size() * 4);
   L.ahi = take(3 * L.nbn * 16);
   L.alo = take(3 * L.nbn * 16);
-  L.zero = take(16);
+  L.tab = take(WTab(ov, support).size() * 4);
   L.sgeo = take((kSortCap + 16) * 4);
   L.sitem = take((kSortCap + 16) * 4);
   L.sflag = take((kSortCap + 16) * 4);
-  L.scnt = take(8 * 4);
+  L.scnt = take(kH3CntWords * 4);
   L.red = take(kDegridWaves * 4);
   L.total = o;
   return L;
@@ -812,6 +822,12 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
   constexpr bool kHalves = NBW < NB;
   constexpr int KB1 = NB - NBW;  // first block of the second half's window
   constexpr int HK = kSortCap / kDegridThreads;
+  // the second window starts on a multiple of 4 blocks, so a block's swizzle
+  // depends only on its index within the window (mod 4)
+  static_assert(!kHalves || KB1 % 4 == 0, "window start");
+  constexpr int nkeys = kHalves ? 3 : 2;
+  constexpr int NVW = HK * kDegridWaves;  // (record slot, wave) pairs of a pass
+  static_assert(NVW * nkeys + 3 <= kH3CntWords, "sort counts");
   extern __shared__ __align__(16) char smem[];
   const uint32_t u = xcd_grouped_block(blockIdx.x, kDegridXcdGroup);
   if (u >= P.dunit_start[P.n_bins]) return;
@@ -831,7 +847,6 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
   const uint32_t polbad = pol_map_skips<NPOL>(pol_map, npol_g);
   const int Hh = L.halo;
   const int li = lane & 15, lk = lane >> 4;
-  constexpr int nkeys = kHalves ? 3 : 2;
   const int half_v = Tv / 2;
 
   const uint4 unit = P.dunits[u];
@@ -868,7 +883,6 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
     const int n = W.size();
     for (int i = tid; i < n; i += kDegridThreads) tab[i] = P.wtab32[i] * 16384.0f;
   }
-  if (tid < 4) ((uint32_t *)(smem + L.zero))[tid] = 0u;
   const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
   const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
   const uint32_t tu_i = tile / (uint32_t)P.ntv;
@@ -931,13 +945,19 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
       H3U4 hr, lr, hm, lm;
       h3_split8(sre, hr, lr);
       h3_split8(sim, hm, lm);
-      const int o = (kg * L.nbn + n) * 8;
-      *(H3U4 *)(ahi + o) = hr;
-      *(H3U4 *)(alo + o) = lr;
-      *(H3U4 *)(ahi + o + 8) = hm;
-      *(H3U4 *)(alo + o + 8) = lm;
+      const int ore = (kg * L.nbn + h3_swz(n)) * 8, oim = (kg * L.nbn + h3_swz(n + 1)) * 8;
+      *(H3U4 *)(ahi + ore) = hr;
+      *(H3U4 *)(alo + ore) = lr;
+      *(H3U4 *)(ahi + oim) = hm;
+      *(H3U4 *)(alo + oim) = lm;
     }
   }
+
+  // this lane's A entries (in halves) for blocks b = 0..3 mod 4: entry 16 b +
+  // li of row group min(lk, 2), stored at 16 b + h3_swz(16 b + li) - 16 b
+  int aswz[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) aswz[j] = (min(lk, 2) * L.nbn + h3_swz(16 * j + li) - 16 * j) * 8;
 
   for (uint32_t c0 = first; c0 < end; c0 += kSortCap) {
     const int n = (int)min((uint32_t)kSortCap, end - c0);
@@ -951,32 +971,52 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
         pf[k] = load_flags(rw.x);
       }
     }
-    // ---- counting sort of the pass's records by v-half, in LDS
-    if (tid < nkeys) s_cnt[tid] = 0u;
-    __syncthreads();
+    // ---- stable counting sort of the pass's records by v-half, in LDS: each
+    // wave counts its keys by ballot, one thread turns the (slot, wave, key)
+    // counts into bases, and a record's slot is its base plus the number of
+    // lower lanes of its wave with the same key (no LDS atomics)
+    int key[HK];
+    uint64_t kmask[HK][nkeys];
 #pragma unroll
-    for (int k = 0; k < HK; ++k)
-      if (tid + k * kDegridThreads < n) atomicAdd(&s_cnt[key_of(pg[k])], 1u);
-    __syncthreads();
-    if (tid == 0) {
-      // the second half starts on a group boundary b1; slots [h0, b1) hold
-      // no record
-      const uint32_t h0 = s_cnt[0];
-      const uint32_t b1 = kHalves ? ((h0 + 15u) & ~15u) : h0;
-      uint32_t run = 0;
-      for (int k = 0; k < nkeys; ++k) {
-        const uint32_t cnt = s_cnt[k];
-        s_cnt[k] = run + (k >= 1 ? b1 - h0 : 0u);
-        run += cnt;
+    for (int k = 0; k < HK; ++k) {
+      const bool valid = tid + k * kDegridThreads < n;
+      key[k] = key_of(pg[k]);
+#pragma unroll
+      for (int kk = 0; kk < nkeys; ++kk) {
+        kmask[k][kk] = __ballot(valid && key[k] == kk);
+        if (lane == 0)
+          s_cnt[(k * kDegridWaves + wave) * nkeys + kk] = (uint32_t)__popcll(kmask[k][kk]);
       }
-      s_cnt[nkeys] = b1;
-      s_cnt[nkeys + 1] = (uint32_t)n + (b1 - h0);
-      s_cnt[nkeys + 2] = h0;
     }
     __syncthreads();
-    const int b1 = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys]);
-    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys + 1]);
-    const int h0 = __builtin_amdgcn_readfirstlane((int)s_cnt[nkeys + 2]);
+    if (tid == 0) {
+      // key 0's records, then key 1's from the group boundary b1 (slots [h0,
+      // b1) hold no record), then key 2's; within a key in pass order
+      uint32_t cnt[NVW * nkeys];
+#pragma unroll
+      for (int i = 0; i < NVW * nkeys; ++i) cnt[i] = s_cnt[i];
+      uint32_t h0 = 0;
+#pragma unroll
+      for (int w = 0; w < NVW; ++w) h0 += cnt[w * nkeys];
+      const uint32_t b1 = kHalves ? ((h0 + 15u) & ~15u) : h0;
+      uint32_t run = 0;
+#pragma unroll
+      for (int kk = 0; kk < nkeys; ++kk) {
+        if (kk == 1) run += b1 - h0;
+#pragma unroll
+        for (int w = 0; w < NVW; ++w) {
+          s_cnt[w * nkeys + kk] = run;
+          run += cnt[w * nkeys + kk];
+        }
+      }
+      s_cnt[NVW * nkeys] = b1;
+      s_cnt[NVW * nkeys + 1] = (uint32_t)n + (b1 - h0);
+      s_cnt[NVW * nkeys + 2] = h0;
+    }
+    __syncthreads();
+    const int b1 = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys]);
+    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys + 1]);
+    const int h0 = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys + 2]);
     if (tid < b1 - h0) {
       s_geo[h0 + tid] = kInvalidGeo;
       s_item[h0 + tid] = 0u;
@@ -985,7 +1025,12 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
 #pragma unroll
     for (int k = 0; k < HK; ++k)
       if (tid + k * kDegridThreads < n) {
-        const uint32_t pos = atomicAdd(&s_cnt[key_of(pg[k])], 1u);
+        uint64_t m = kmask[k][0];
+#pragma unroll
+        for (int kk = 1; kk < nkeys; ++kk) m = key[k] == kk ? kmask[k][kk] : m;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t pos = s_cnt[(k * kDegridWaves + wave) * nkeys + key[k]] + below;
         s_geo[pos] = pg[k];
         s_item[pos] = pi[k];
         s_flag[pos] = pf[k];
@@ -1018,17 +1063,13 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
       H3U4 bhu, blu;
       h3_split8(wu, bhu, blu);
       const H3Half8 bh = __builtin_bit_cast(H3Half8, bhu), bl = __builtin_bit_cast(H3Half8, blu);
-      // ---- A: the window's blocks of the region (row group 3 reads zeros)
-      const _Float16 *ah_p = lk < 3 ? ahi + (lk * L.nbn + 16 * kb0 + li) * 8
-                                    : (const _Float16 *)(smem + L.zero);
-      const _Float16 *al_p = lk < 3 ? alo + (lk * L.nbn + 16 * kb0 + li) * 8
-                                    : (const _Float16 *)(smem + L.zero);
-      const int astep = lk < 3 ? 128 : 0;
+      // ---- A: the window's blocks of the region
+      const _Float16 *ah_p = ahi + 128 * kb0, *al_p = alo + 128 * kb0;
       MV T[NBW];
 #pragma unroll
       for (int k = 0; k < NBW; ++k) {
-        const H3Half8 ah = *(const H3Half8 *)(ah_p + k * astep);
-        const H3Half8 al = *(const H3Half8 *)(al_p + k * astep);
+        const H3Half8 ah = *(const H3Half8 *)(ah_p + aswz[k & 3] + 128 * k);
+        const H3Half8 al = *(const H3Half8 *)(al_p + aswz[k & 3] + 128 * k);
         const MV z = {0.f, 0.f, 0.f, 0.f};
         MV t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, z, 0, 0, 0);
         t = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t, 0, 0, 0);
