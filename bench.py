@@ -254,6 +254,521 @@ def small_workload_leg(dev, reps=10):
                 "degrid_from_own_grid": rel(mm, mref)}}
 
 
+# --------------------------------------------------------------------------
+# Rows beside the hot path: Hogbom CLEAN, imaging weights, the mosaic gridder
+# and resident major cycles, each on one GPU with a bounded CPU leg and the
+# GPU result compared with the CPU one on the same sample.
+#
+#     python bench.py --rows hogbom,weights,mosaic,major_cycle
+#
+# prints one JSON line per row (the headline line is not printed then).
+# --------------------------------------------------------------------------
+def _events_ms(fn, reps=3, warm=1):
+    """Median HIP-event time (ms, torch's current stream, on which the
+    library launches) of fn()."""
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts))
+
+
+def _roof(nbytes, ms, kernel):
+    gbs = nbytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 1),
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(gbs * 1e9 / HBM_PEAK, 4),
+            "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
+
+
+def _ref_hogbom():
+    """The reference's own Hogbom extension built from its sources into
+    oracle/_ref (None when that build is absent)."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if ref not in sys.path:
+        sys.path.insert(0, ref)
+    try:
+        import _hogbom_ext
+        return _hogbom_ext
+    except ImportError:
+        return None
+
+
+def row_hogbom(dev, n=2048, niter=2000, half_box=32, cpu_s=10.0):
+    """Hogbom CLEAN of one 2048 x 2048 float32 plane (hclean.cpp:106-210):
+    a PSF with a Gaussian main lobe and ring sidelobes over the whole image,
+    40 point sources within 32 pixels of the centre, a clean box of the
+    central 64 x 64 pixels, gain 0.1, threshold 0, 2000 iterations.
+
+    An iteration subtracts gain * peak * PSF over the window where the
+    shifted PSF overlaps the image, W = (n - |dx|)(n - |dy|) pixels for a peak
+    (dx, dy) from the centre, and finds the next peak in the box (inside the
+    window): 12 bytes per window pixel (residual read and written, PSF
+    read).  The peaks lie in the box, so W >= (n - 32)^2 = 0.969 n^2; the
+    algorithmic bytes use that bound, so `achieved` is a lower bound."""
+    import torch
+
+    from astroviper_amd.deconvolvers import hogbom as H
+
+    rng = np.random.default_rng(11)
+    c = n // 2
+    yy, xx = np.mgrid[0:n, 0:n]
+    r = np.hypot(xx - c, yy - c)
+    psf = np.exp(-r ** 2 / 18.0) + 0.05 * np.cos(2 * np.pi * r / 12.0) * np.exp(-r / 200.0)
+    psf /= psf[c, c]
+    sky = np.zeros((n, n))
+    k = 40
+    np.add.at(sky, (rng.integers(c - half_box + 1, c + half_box, k),
+                    rng.integers(c - half_box + 1, c + half_box, k)), rng.uniform(0.1, 1.0, k))
+    dirty = np.real(np.fft.ifft2(np.fft.fft2(sky) * np.fft.fft2(np.fft.ifftshift(psf))))
+    dirty = dirty.astype(np.float32)
+    psf = psf.astype(np.float32)
+    del yy, xx, r
+    box = (c - half_box, c + half_box, c - half_box, c + half_box)  # xbeg, xend, ybeg, yend
+    d_t = torch.from_numpy(dirty).to(dev)
+    p_t = torch.from_numpy(psf).to(dev)
+    work = H._work(dev)
+    res = {}
+
+    def run(iters):
+        rr = d_t.clone()
+        mm = torch.zeros_like(rr)
+        res["it"] = H.clean_device(rr, p_t, mm, None, box, iters, 0, 0.1, 0.0, work=work)
+        res["r"], res["m"] = rr, mm
+
+    ms = _events_ms(lambda: run(niter), reps=3)
+    it = res["it"]
+    log(f"bench: hogbom {it} iterations in {ms:.2f} ms on the GPU")
+    wmin = (n - half_box) ** 2
+    out = {"row": "hogbom", "workload": f"Hogbom CLEAN, one {n}x{n} float32 plane, {k} point "
+                                         f"sources, clean box {2 * half_box}x{2 * half_box} at "
+                                         f"the centre, gain 0.1, {niter} iterations",
+           "value": round(it / (ms / 1e3), 1), "unit": "iterations/sec",
+           "gpu_ms": round(ms, 3), "iterations": it,
+           "us_per_iteration": round(ms * 1e3 / max(it, 1), 3),
+           "roofline": _roof(12.0 * wmin * it, ms, "k_hogbom")}
+    out["roofline"]["note"] = ("the residual and the PSF (32 MB) fit in the 256 MB "
+                               "Infinity Cache, so iterations may run above HBM speed")
+    # CPU leg: the reference's own extension (else the oracle's restatement),
+    # one core, on the same images, iterations sized to about cpu_s seconds;
+    # the GPU run of the same iterations is compared with it
+    ext = _ref_hogbom()
+    kind = "reference" if ext is not None else "port"
+    if ext is None:
+        from oracle import oracle as O
+
+        def cpu_clean(iters):
+            return O.hogbom_clean(dirty, psf, clean_box=box, max_iter=iters, gain=0.1,
+                                  threshold=0.0)
+    else:
+        def cpu_clean(iters):
+            return ext.clean(dirty, psf, clean_box=box, max_iter=iters, gain=0.1, threshold=0.0)
+    t0 = time.perf_counter()
+    cpu_clean(20)
+    per = (time.perf_counter() - t0) / 20
+    ci = int(max(20, min(niter, cpu_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    ref = cpu_clean(ci)
+    tc = time.perf_counter() - t0
+    run(ci)
+    dm = float(np.max(np.abs(res["m"].cpu().numpy() - ref["model_image"])))
+    dr = float(np.max(np.abs(res["r"].cpu().numpy() - ref["residual_image"])))
+    out["cpu_baseline"] = {"value": round(ref["iterations_performed"] / tc, 2),
+                           "unit": "iterations/sec", "cores": 1, "kind": kind,
+                           "sample": f"the same images, {ref['iterations_performed']} "
+                                     f"iterations: {tc:.2f} s"}
+    out["parity_vs_cpu"] = {"iterations": ci, "same_iterations": res["it"] == ref[
+        "iterations_performed"], "model_max_abs_diff": dm, "residual_max_abs_diff": dr}
+    return out
+
+
+def row_weights(dev, obs, uvw, w, flag, n_uv, cell, robust=0.5, cpu_s=5.0):
+    """Briggs imaging weights of the benchmark workload (64 antennas x 1000
+    times x 64 channels, 2048 x 2048 cells per channel; calculate_imaging_
+    weights.py:28-275): the data weights of polarisations 0 and 3 averaged,
+    NaN where flagged; gridded at the nearest pixel and its conjugate
+    (grid_imaging_weights_jit, grid_imaging_weights.py:105-206), the Briggs
+    factors from the density grid (briggs_weighting.py:1-31), and the
+    per-sample imaging weights (degrid_imaging_weights_jit :274-443).
+
+    Algorithmic bytes: grid pass uvw (24 B per row) + weight (8 B per
+    sample) + density grid once (8 B per cell); degrid pass the same plus
+    the imaging weight written (8 B per sample)."""
+    import torch
+
+    from astroviper_amd import _lib
+    from astroviper_amd.device import _ptr, _stream
+
+    rows, nchan, npol = w.shape
+    S = rows * nchan
+    w1 = ((w[..., 0].double() + w[..., npol - 1].double()) / 2)
+    w1 = torch.where(flag[..., 0].bool(), torch.full_like(w1, float("nan")), w1).contiguous()
+    freq = torch.from_numpy(obs.freq).to(dev)
+    n_u, n_v = n_uv
+    grid = torch.empty((nchan, n_u, n_v), dtype=torch.float64, device=dev)
+    sw = torch.empty(nchan, dtype=torch.float64, device=dev)
+    iw = torch.empty((rows, nchan), dtype=torch.float64, device=dev)
+    nanf = torch.zeros(1, dtype=torch.int32, device=dev)
+    bf = torch.empty((2, nchan), dtype=torch.float64, device=dev)
+
+    def grid_w():
+        grid.zero_()
+        sw.zero_()
+        _lib.call("av_grid_imaging_weights", _ptr(uvw), rows, _ptr(freq), nchan, _ptr(w1), 1,
+                  n_u, n_v, float(cell[0]), float(cell[1]), _ptr(grid), _ptr(sw), _stream())
+
+    def factors():
+        sq = torch.sum(grid * grid, dim=(1, 2))
+        bf[0] = (5.0 * 10.0 ** (-robust)) ** 2 / (sq / sw)
+        bf[1] = 1.0
+
+    def degrid_w():
+        _lib.call("av_degrid_imaging_weights", _ptr(grid), _ptr(bf), _ptr(uvw), rows,
+                  _ptr(freq), nchan, _ptr(w1), n_u, n_v, float(cell[0]), float(cell[1]),
+                  _ptr(iw), _ptr(nanf), _stream())
+
+    t_g = _events_ms(grid_w)
+    t_f = _events_ms(factors)
+    t_d = _events_ms(degrid_w)
+    log(f"bench: weights grid {t_g:.2f} ms, factors {t_f:.2f} ms, degrid {t_d:.2f} ms")
+    cells = nchan * n_u * n_v
+    b_g = rows * 24 + S * 8 + cells * 8
+    b_d = rows * 24 + S * 16 + cells * 8
+    tot = t_g + t_f + t_d
+    out = {"row": "weights", "workload": f"Briggs (robust {robust}) imaging weights, {rows} rows "
+                                          f"x {nchan} channels = {S} samples, {n_u}x{n_v} "
+                                          f"density grid per channel",
+           "value": round(S / (tot / 1e3) / 1e6, 1), "unit": "M samples/sec",
+           "ms": {"grid_imaging_weights": round(t_g, 3), "briggs_factors": round(t_f, 3),
+                  "degrid_imaging_weights": round(t_d, 3)},
+           "roofline": _roof(b_g, t_g, "k_grid_imaging_weights"),
+           "roofline_degrid": _roof(b_d, t_d, "k_degrid_imaging_weights")}
+    # CPU leg: the oracle on the first rows x all channels, one core
+    from oracle import oracle as O
+    nb = obs.uvw.shape[1]
+    nt = 4
+    while True:
+        r_s = nt * nb
+        u_h = uvw[:r_s].cpu().numpy().reshape(nt, nb, 3)
+        w_h = w1[:r_s].cpu().numpy().reshape(nt, nb, nchan, 1)
+        g_h = np.zeros((nchan, 1, n_u, n_v))
+        s_h = np.zeros((nchan, 1))
+        t0 = time.perf_counter()
+        O.grid_imaging_weights(g_h, s_h, u_h, w_h, obs.freq, n_uv, cell)
+        sq = np.sum(g_h[:, 0] ** 2, axis=(1, 2))
+        bf_h = np.stack([(5.0 * 10.0 ** (-robust)) ** 2 / (sq / s_h[:, 0]), np.ones(nchan)])
+        iw_h = O.degrid_imaging_weights(g_h, u_h, w_h, bf_h, obs.freq, n_uv, cell)
+        tc = time.perf_counter() - t0
+        log(f"bench: weights CPU leg {r_s} rows: {tc:.2f} s")
+        if tc > cpu_s / 4 or r_s * 4 > rows:
+            break
+        nt *= 4
+    # the GPU on the same sample
+    uvw_s, w1_s = uvw[:r_s].contiguous(), w1[:r_s].contiguous()
+    g_s = torch.zeros((nchan, n_u, n_v), dtype=torch.float64, device=dev)
+    s_s = torch.zeros(nchan, dtype=torch.float64, device=dev)
+    _lib.call("av_grid_imaging_weights", _ptr(uvw_s), r_s, _ptr(freq), nchan, _ptr(w1_s), 1,
+              n_u, n_v, float(cell[0]), float(cell[1]), _ptr(g_s), _ptr(s_s), _stream())
+    bf_s = torch.from_numpy(bf_h).to(dev).contiguous()
+    iw_s = torch.empty((r_s, nchan), dtype=torch.float64, device=dev)
+    _lib.call("av_degrid_imaging_weights", _ptr(g_s), _ptr(bf_s), _ptr(uvw_s), r_s, _ptr(freq),
+              nchan, _ptr(w1_s), n_u, n_v, float(cell[0]), float(cell[1]), _ptr(iw_s),
+              _ptr(nanf), _stream())
+
+    def rel(a, b):
+        return float(np.nanmax(np.abs(a - b)) / np.nanmax(np.abs(b)))
+
+    iw_ref = np.asarray(iw_h)[..., 0].reshape(r_s, nchan)
+    out["cpu_baseline"] = {"value": round(r_s * nchan / tc / 1e6, 3), "unit": "M samples/sec",
+                           "cores": 1, "kind": "port",
+                           "sample": f"first {r_s} rows x {nchan} channels = {r_s * nchan} "
+                                     f"samples, grid + factors + degrid: {tc:.2f} s"}
+    iw_g = iw_s.cpu().numpy()
+    out["parity_vs_cpu"] = {"samples": r_s * nchan,
+                            "density_grid_max_rel": rel(g_s.cpu().numpy(), g_h[:, 0]),
+                            "imaging_weight_max_rel": rel(iw_g, iw_ref),
+                            "nan_positions_differ": int(np.sum(np.isnan(iw_g) != np.isnan(iw_ref)))}
+    return out
+
+
+def _mosaic_cfs(rng, ws_choices=(7, 9, 11), ov=(8, 8), ncb=3, ncc=4):
+    """Complex convolution functions of A-projection shape: a Gaussian
+    aperture product with a phase ramp, per (baseline group, channel group),
+    one polarisation group; supports per axis from ws_choices."""
+    ws = rng.choice(np.asarray(ws_choices), size=(ncb, ncc, 1, 2))
+    smax = int(ws.max())
+    cu_n, cv_n = ov[0] * (smax + 1) + 1, ov[1] * (smax + 1) + 1
+    x = (np.arange(cu_n) - cu_n // 2) / cu_n
+    y = (np.arange(cv_n) - cv_n // 2) / cv_n
+    cf = np.zeros((ncb, ncc, 1, cu_n, cv_n), dtype=np.complex128)
+    for idx in np.ndindex(ncb, ncc, 1):
+        a, b_, ph = rng.uniform(4, 8), rng.uniform(4, 8), rng.uniform(-3, 3)
+        cf[idx] = (np.exp(-a * x[:, None] ** 2 - b_ * y[None, :] ** 2)
+                   * np.exp(1j * ph * (x[:, None] + 0.5 * y[None, :])))
+    pg = np.exp(1j * rng.uniform(-0.3, 0.3, size=(1, cu_n, cv_n)))
+    return ws, cf, pg, ov
+
+
+def row_mosaic(dev, n_time=100, cpu_s=5.0):
+    """The mosaic / A-projection gridder (mosaic_grid_jit, mosaic_grid.py:
+    306-486) on the benchmark's array (64 antennas, 2016 baselines) over
+    `n_time` time samples x 64 channels x 4 polarisations, 2048 x 2048
+    cube, FP32: 3 baseline groups x 4 channel groups of complex convolution
+    functions, 7, 9 or 11 taps per axis, oversampling 8 x 8.
+
+    Algorithmic bytes: visibility 8 + weight 4 bytes per visibility, the
+    convolution functions once, the grid written once (8 bytes per cell)."""
+    import torch
+
+    from astroviper_amd import _lib, synth
+    from astroviper_amd import device as D
+    from astroviper_amd.mosaic_grid import _tile_for
+
+    cfg = synth.CONFIG2
+    arr = synth.ArrayConfig(**{**cfg["array"].__dict__, "n_time": n_time})
+    sky = cfg["sky"]
+    n_uv = cfg["n_uv"]
+    cv = cfg["cell_arcsec"] * synth.ARCSEC
+    cell = (-cv, cv)
+    obs = synth.make_observation(arr, sky)
+    uvw, vis, w, flag = synth.device_visibilities(obs, sky, cfg["flag_frac"], cfg["weight_range"],
+                                                  device=dev)
+    rows, nchan, npol = vis.shape
+    nb = obs.uvw.shape[1]
+    V = rows * nchan * npol
+    rng = np.random.default_rng(12)
+    ws, cf, pg, ov = _mosaic_cfs(rng)
+    ncb, ncc, ncp, cu_n, cv_n = cf.shape
+    cfbm = (np.arange(nb) % ncb).astype(np.int32)
+    cfcm = (np.arange(nchan) * ncc // nchan).astype(np.int32)
+    cfpm = np.zeros(npol, dtype=np.int32)
+    smax = int(ws.max())
+    support = 2 * smax + 1
+    tile = _tile_for(support)
+    w_eff = torch.where(flag.bool(), torch.zeros_like(w), w)  # flagged samples weigh 0
+    freq = torch.from_numpy(obs.freq).to(dev)
+
+    def grid_on(rows_, vis_, w_, nch, chan_ids):
+        plan = D.GridPlan(rows_, nch, nch, n_uv, cell, support, ov[0], tile=(tile, tile),
+                          device=dev)
+        plan.set_uvw(uvw[:rows_].contiguous(), freq[chan_ids].contiguous(),
+                     torch.arange(nch, dtype=torch.int32, device=dev))
+        i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
+        keep = (i32(cfbm), i32(cfcm[chan_ids.cpu().numpy()]), i32(cfpm), i32(ws))
+        cfpg = (torch.from_numpy(cf).to(dev) * torch.from_numpy(pg[0]).to(dev)).to(
+            torch.complex64).contiguous()
+        g = torch.zeros((nch, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
+        s = torch.zeros((nch, npol), dtype=torch.float64, device=dev)
+
+        def call():
+            _lib.call("av_mosaic_grid", plan._h, D._ptr(vis_), D._ptr(w_), npol, nb,
+                      D._ptr(keep[0]), D._ptr(keep[1]), D._ptr(keep[2]), D._ptr(cfpg), ncc, ncp,
+                      cu_n, cv_n, D._ptr(keep[3]), ov[0], ov[1], D._ptr(g), D._ptr(s),
+                      _lib.AV_F32, 0, D._stream())
+        return plan, g, s, call
+
+    allc = torch.arange(nchan, device=dev)
+    plan, g, s, call = grid_on(rows, vis.reshape(rows * nchan, npol),
+                               w_eff.reshape(rows * nchan, npol), nchan, allc)
+    ms = _events_ms(call)
+    log(f"bench: mosaic {V} vis in {ms:.2f} ms on the GPU")
+    st = plan.stats()
+    plan.close()
+    cells = nchan * npol * n_uv[0] * n_uv[1]
+    nbytes = V * 12 + cf.size * 8 + cells * 8
+    out = {"row": "mosaic", "workload": f"mosaic gridder, 64 antennas x {n_time} times x "
+                                         f"{nchan} chan x {npol} pol = {V} vis, "
+                                         f"{n_uv[0]}x{n_uv[1]} cube, FP32, {ncb}x{ncc} "
+                                         f"convolution functions of 7-11 taps per axis, "
+                                         f"oversampling {ov[0]}x{ov[1]}, {tile}x{tile} tiles",
+           "value": round(V / (ms / 1e3) / 1e6, 1), "unit": "M visibilities/sec",
+           "gpu_ms": round(ms, 3), "records": st.get("records"),
+           "roofline": _roof(nbytes, ms, "k_mosaic_grid")}
+    del g
+    # CPU leg: the FP64 oracle on the first rows x 4 channels, one core; the
+    # GPU grids the same sample in FP32 for the comparison
+    from oracle import oracle as O
+    nch_s = 4
+    cs = torch.arange(nch_s, device=dev)
+    nt_s = 2
+    while True:
+        r_s = nt_s * nb
+        u_h = uvw[:r_s].cpu().numpy().reshape(nt_s, nb, 3)
+        v_h = vis[:r_s, :nch_s].cpu().numpy().astype(np.complex128).reshape(nt_s, nb, nch_s, npol)
+        w_h = w_eff[:r_s, :nch_s].cpu().numpy().astype(np.float64).reshape(nt_s, nb, nch_s, npol)
+        gref = np.zeros((nch_s, npol) + tuple(n_uv), dtype=np.complex128)
+        sref = np.zeros((nch_s, npol))
+        t0 = time.perf_counter()
+        O.mosaic_grid(gref, sref, False, v_h, u_h, obs.freq[:nch_s], np.arange(nch_s),
+                      np.arange(npol), cfbm, cfcm[:nch_s], cfpm, w_h, cf, np.array(n_uv), cell,
+                      ws, np.array(ov), pg)
+        tc = time.perf_counter() - t0
+        log(f"bench: mosaic CPU leg {r_s} rows: {tc:.2f} s")
+        if tc > cpu_s / 4 or r_s * 4 > rows:
+            break
+        nt_s *= 4
+    vs = vis[:r_s, :nch_s].contiguous().reshape(r_s * nch_s, npol)
+    wsm = w_eff[:r_s, :nch_s].contiguous().reshape(r_s * nch_s, npol)
+    plan, g, s, call = grid_on(r_s, vs, wsm, nch_s, cs)
+    call()
+    gg = g.cpu().numpy()
+    plan.close()
+    out["cpu_baseline"] = {"value": round(r_s * nch_s * npol / tc / 1e6, 3),
+                           "unit": "M visibilities/sec", "cores": 1, "kind": "port",
+                           "sample": f"first {r_s} rows x {nch_s} channels x {npol} pols, "
+                                     f"FP64: {tc:.2f} s"}
+    out["parity_vs_cpu"] = {"visibilities": r_s * nch_s * npol,
+                            "grid_max_rel": float(np.max(np.abs(gg - gref)) / np.max(np.abs(gref))),
+                            "sumwt_max_rel": float(np.max(np.abs(s.cpu().numpy() - sref))
+                                                   / np.max(np.abs(sref)))}
+    return out
+
+
+def row_major_cycle(dev, reps=3):
+    """One major cycle of imager.py:108-342 on the benchmark workload (64
+    antennas x 1000 times x 64 channels x 4 pols, 2048 x 2048 cube, FP32
+    gridding, FP64 FFTs) with every visibility-sized array and the binning
+    plan resident on the GPU: grid the residual visibilities and form the
+    dirty image (grid2image_spheroid_ms4), FFT a model image to the uv plane
+    and degrid it (predict_model_visibilities, imager.py:88-105), and form
+    RESIDUAL = VISIBILITY - VISIBILITY_MODEL (imager.py:82-85).  CLEAN is
+    the Hogbom row.  Beside it: the same cycle when the visibilities,
+    weights and flags are staged from pinned host memory and the model and
+    residual visibilities copied back each cycle, as a host-resident
+    caller must."""
+    import torch
+
+    from astroviper_amd import synth
+    from astroviper_amd.device import GridPlan, dirty_image
+
+    cfg = synth.CONFIG2
+    arr = cfg["array"]
+    sky = cfg["sky"]
+    n_uv = cfg["n_uv"]
+    cv = cfg["cell_arcsec"] * synth.ARCSEC
+    cell = (-cv, cv)
+    obs = synth.make_observation(arr, sky)
+    uvw, vis, w, flag = synth.device_visibilities(obs, sky, cfg["flag_frac"], cfg["weight_range"],
+                                                  device=dev)
+    rows, nchan, npol = vis.shape
+    V = rows * nchan * npol
+    freq = torch.from_numpy(obs.freq).to(dev)
+    vis2, w2, fl2 = (t.reshape(rows * nchan, npol) for t in (vis, w, flag))
+    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev)
+    plan.set_uvw(uvw, freq)
+    grid = torch.empty((nchan, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
+    sumwt = torch.empty((nchan, npol), dtype=torch.float64, device=dev)
+    model_vis = torch.zeros_like(vis2)
+    resid = vis2.clone()
+    # a model image: 0.5 of each plane's two brightest dirty pixels
+    plan.grid(vis2, w2, fl2, grid=grid, sumwt=sumwt)
+    d0 = dirty_image(grid, sumwt)
+    model = torch.zeros_like(d0)
+    flat = d0.reshape(nchan * npol, -1)
+    top = torch.topk(flat, 2, dim=1)
+    model.reshape(nchan * npol, -1).scatter_(1, top.indices, 0.5 * top.values)
+    del d0, flat
+    t = {}
+
+    def cycle(ev=None):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        e[0].record()
+        plan.grid(resid, w2, fl2, grid=grid, sumwt=sumwt)
+        e[1].record()
+        img = dirty_image(grid, sumwt)
+        e[2].record()
+        ax = (-2, -1)
+        uvm = torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(model, dim=ax), dim=ax),
+                                 dim=ax).to(torch.complex64)
+        plan.degrid(uvm, fl2, model_vis)
+        e[3].record()
+        torch.sub(vis2, model_vis, out=resid)
+        e[4].record()
+        e[4].synchronize()
+        for name, a, b in (("grid", 0, 1), ("image", 1, 2), ("predict", 2, 3),
+                           ("residual", 3, 4)):
+            t.setdefault(name, []).append(e[a].elapsed_time(e[b]))
+        del img, uvm
+
+    cycle()
+    t.clear()
+    for _ in range(reps):
+        cycle()
+        log(f"bench: major cycle {sum(v[-1] for v in t.values()):.2f} ms")
+    parts = {k: round(float(np.median(v)), 3) for k, v in t.items()}
+    tot = sum(parts.values())
+    # the same cycle with host-resident data: H2D of visibilities, weights
+    # and flags, D2H of the model and residual visibilities (pinned buffers)
+    h_vis = torch.empty(vis2.shape, dtype=vis2.dtype, pin_memory=True)
+    h_w = torch.empty(w2.shape, dtype=w2.dtype, pin_memory=True)
+    h_fl = torch.empty(fl2.shape, dtype=fl2.dtype, pin_memory=True)
+    h_mod = torch.empty(vis2.shape, dtype=vis2.dtype, pin_memory=True)
+    h_res = torch.empty(vis2.shape, dtype=vis2.dtype, pin_memory=True)
+    h_vis.copy_(vis2)
+    h_w.copy_(w2)
+    h_fl.copy_(fl2)
+
+    def transfers():
+        vis2.copy_(h_vis, non_blocking=True)
+        w2.copy_(h_w, non_blocking=True)
+        fl2.copy_(h_fl, non_blocking=True)
+        h_mod.copy_(model_vis, non_blocking=True)
+        h_res.copy_(resid, non_blocking=True)
+
+    t_x = _events_ms(transfers, reps=reps)
+    xbytes = (vis2.numel() * vis2.element_size() * 3 + w2.numel() * w2.element_size()
+              + fl2.numel())
+    plan.close()
+    b_cycle = V * 22 + rows * 48 + nchan * npol * n_uv[0] * n_uv[1] * 16
+    return {"row": "major_cycle",
+            "workload": f"one major cycle without CLEAN, {V} vis resident (64 antennas x 1000 "
+                        f"times x {nchan} chan x {npol} pol), {n_uv[0]}x{n_uv[1]} cube, FP32 "
+                        f"gridding, FP64 FFTs",
+            "value": round(V / (tot / 1e3) / 1e6, 1), "unit": "M visibilities/sec",
+            "ms": {**parts, "cycle": round(tot, 3)},
+            "roofline": _roof(b_cycle, parts["grid"] + parts["predict"], "grid + degrid"),
+            "host_resident_cycle": {"transfer_ms": round(t_x, 3),
+                                    "transfer_GB": round(xbytes / 1e9, 2),
+                                    "host_GB_per_s": round(xbytes / (t_x / 1e3) / 1e9, 2),
+                                    "cycle_ms": round(tot + t_x, 3),
+                                    "M_vis_per_s": round(V / ((tot + t_x) / 1e3) / 1e6, 1)}}
+
+
+def run_rows(names, dev):
+    rows_fn = {"hogbom": lambda: row_hogbom(dev),
+               "mosaic": lambda: row_mosaic(dev),
+               "major_cycle": lambda: row_major_cycle(dev)}
+    for name in names:
+        if name == "weights":
+            import torch
+
+            from astroviper_amd import synth
+            cfg = synth.CONFIG2
+            obs = synth.make_observation(cfg["array"], cfg["sky"])
+            uvw, vis, w, flag = synth.device_visibilities(obs, cfg["sky"], cfg["flag_frac"],
+                                                          cfg["weight_range"], device=dev)
+            del vis
+            cv = cfg["cell_arcsec"] * synth.ARCSEC
+            res = row_weights(dev, obs, uvw, w, flag, cfg["n_uv"], (-cv, cv))
+            del uvw, w, flag
+            torch.cuda.empty_cache()
+        elif name in rows_fn:
+            res = rows_fn[name]()
+        else:
+            raise SystemExit(f"unknown row {name!r}")
+        log(f"bench: row {name} done")
+        print(json.dumps(res), flush=True)
+        import torch
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +787,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--time-samples", type=int, default=1000)
+    ap.add_argument("--rows", default="",
+                    help="comma list of rows beside the hot path to measure instead of the "
+                         "headline: hogbom, weights, mosaic, major_cycle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the timed steps, copy a BYTES-sized device buffer once "
                          "(the reference kernel for scaling HBM byte counters)")
@@ -281,6 +799,10 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.rows:
+        run_rows([r for r in args.rows.split(",") if r], torch.device("cuda", 0))
+        return
 
     from astroviper_amd import synth
     from astroviper_amd.device import (GridPlan, degrid_kernel_name, degrid_plain,
