@@ -18,12 +18,13 @@
 // iteration is enough.  All arithmetic is in the image's type, with separate
 // roundings (no contraction), as the reference's C++ does.
 #include <algorithm>
+#include <type_traits>
 
 #include "avgrid_internal.h"
 
 namespace av {
 
-constexpr int kHogThreads = 256;
+constexpr int kHogThreads = 1024;
 constexpr int kHogMaxGroups = 1024;
 constexpr uint32_t kHogSpinLimit = 1u << 25;  // s_sleep rounds before a barrier gives up
 
@@ -131,26 +132,78 @@ __global__ __launch_bounds__(kHogThreads) void k_hogbom(
   const int tid = threadIdx.x;
   const bool domask = mask != nullptr;
 
-  // candidate of this thread's pixels
+  // The band's pixels q = tid + j * B, j = 0, 1, ... (row r0 + q / nx,
+  // column q % nx) are walked U at a time (a 2048-row image on 256 bands of
+  // 1024 threads gives each thread 16 pixels), the row and column
+  // advanced incrementally (one division per thread, not per pixel), and the
+  // loads of each group of U pixels issued before their use.
+  constexpr int U = sizeof(T) == 4 ? 16 : 8;
+  const int B = (int)blockDim.x;
+  const int drow = B / nx, dcol = B - drow * nx;
+  const int iy_t = r0 + tid / nx, ix_t = tid - (tid / nx) * nx;
   auto in_box = [&](int ix, int iy) {
     return ix >= xbeg && ix < xend && iy >= ybeg && iy < yend &&
            (!domask || mask[iy * nx + ix] > (T)0.5);
   };
-  T ba = (T)0, bv = (T)0;
-  int bi = 0x7FFFFFFF;
-  for (int k = tid; k < npix; k += blockDim.x) {
-    const int iy = r0 + k / nx, ix = k - (k / nx) * nx;
-    if (in_box(ix, iy)) {
-      const T r = resid[iy * nx + ix];
-      const T a = fabs(r);
-      const int idx = iy * nx + ix;
-      if (hog_better(a, idx, ba, bi)) {
-        ba = a;
-        bv = r;
-        bi = idx;
+  // one pass over the band: subtract pv * PSF shifted to (px, py) inside the
+  // window [x1, x2] x [y1, y2] (when SUB), then this thread's candidate for
+  // the peak of the clean box
+  auto band_pass = [&](auto sub_tag, T pv, int px, int py, int x1, int x2, int y1, int y2,
+                       T &ba, T &bv, int &bi) {
+    constexpr bool SUB = decltype(sub_tag)::value;
+    ba = (T)0;
+    bv = (T)0;
+    bi = 0x7FFFFFFF;
+    int ix = ix_t, iy = iy_t;
+    for (int q0 = tid; q0 < npix; q0 += U * B) {
+      int cx[U], cy[U];
+      bool ok[U], win[U];
+      T r[U], pp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cx[u] = ix;
+        cy[u] = iy;
+        ok[u] = q0 + u * B < npix;
+        win[u] = false;
+        pp[u] = (T)0;
+        r[u] = ok[u] ? resid[iy * nx + ix] : (T)0;
+        if constexpr (SUB) {
+          const int qx = nx / 2 + ix - px, qy = ny / 2 + iy - py;
+          win[u] = ok[u] && iy >= y1 && iy <= y2 && ix >= x1 && ix <= x2 && qx >= 0 &&
+                   qx < nx && qy >= 0 && qy < ny;
+          pp[u] = win[u] ? psf[qy * nx + qx] : (T)0;
+        }
+        ix += dcol;
+        iy += drow;
+        if (ix >= nx) {
+          ix -= nx;
+          ++iy;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = cy[u] * nx + cx[u];
+        if constexpr (SUB) {
+          if (win[u]) {
+            const T prod = pv * pp[u];
+            r[u] = r[u] - prod;
+            resid[idx] = r[u];
+          }
+        }
+        if (ok[u] && in_box(cx[u], cy[u])) {
+          const T a = fabs(r[u]);
+          if (hog_better(a, idx, ba, bi)) {
+            ba = a;
+            bv = r[u];
+            bi = idx;
+          }
+        }
       }
     }
-  }
+  };
+  T ba, bv;
+  int bi;
+  band_pass(std::false_type{}, (T)0, 0, 0, 0, -1, 0, -1, ba, bv, bi);
 
   int px = px0, py = py0;
   T maxval = (T)0;
@@ -197,30 +250,7 @@ __global__ __launch_bounds__(kHogThreads) void k_hogbom(
     if (tid == 0 && py >= r0 && py < r1) model[py * nx + px] += pv;
     const int x1 = max(0, px - nx / 2), y1 = max(0, py - ny / 2);
     const int x2 = min(nx - 1, px + nx / 2 - 1), y2 = min(ny - 1, py + ny / 2 - 1);
-    ba = (T)0;
-    bv = (T)0;
-    bi = 0x7FFFFFFF;
-    for (int q = tid; q < npix; q += blockDim.x) {
-      const int iy = r0 + q / nx, ix = q - (q / nx) * nx;
-      const int idx = iy * nx + ix;
-      T r = resid[idx];
-      if (iy >= y1 && iy <= y2 && ix >= x1 && ix <= x2) {
-        const int qx = nx / 2 + ix - px, qy = ny / 2 + iy - py;
-        if (qx >= 0 && qx < nx && qy >= 0 && qy < ny) {
-          const T prod = pv * psf[qy * nx + qx];
-          r = r - prod;
-          resid[idx] = r;
-        }
-      }
-      if (in_box(ix, iy)) {
-        const T a = fabs(r);
-        if (hog_better(a, idx, ba, bi)) {
-          ba = a;
-          bv = r;
-          bi = idx;
-        }
-      }
-    }
+    band_pass(std::true_type{}, pv, px, py, x1, x2, y1, y2, ba, bv, bi);
   }
   if (g == 0 && tid == 0) {
     h->iter = iter > niter ? niter : iter;

@@ -9,11 +9,13 @@
 // (u_centre +- max_support inside the grid) and every footprint lies inside
 // the tiles the plan records the sample in.
 //
-// One workgroup (one wavefront) per work unit.  The unit's tile, all its
-// polarisations, is accumulated in LDS.  Records are staged 64 at a time
-// (one per lane: its FP64 position, kernel offsets, convolution-function
-// bases and supports, weighted data); the wavefront then takes the staged
-// records one after another, its lanes over the record's taps.  A tap reads
+// One workgroup per work unit, one wavefront per polarisation.  The unit's
+// tile, all its polarisations (polarisation-major, so the lanes of a
+// wavefront, which run along v, touch consecutive cells), is accumulated in
+// LDS.  Records are staged 64 at a time by the first wavefront (one per
+// lane: its FP64 position, kernel offsets, convolution-function bases and
+// supports, weighted data); every wavefront then takes the staged records
+// one after another, its lanes over the taps of its own polarisation.  A tap reads
 // its complex kernel value from the convolution function in global memory
 // (read-only, shared by every unit of the same baseline group, so it stays
 // in L2) and adds kernel * data into its tile cell: the taps of one record
@@ -50,14 +52,15 @@ struct MosaicStaged {
 };
 
 template <typename R, int NPOL>
-__global__ __launch_bounds__(64) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, int psf,
-                                                    int accumulate) {
+__global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, int psf,
+                                                           int accumulate) {
   extern __shared__ __align__(16) char smem[];
   using St = MosaicStaged<R, NPOL>;
   const int T = P.tile_u, TV = P.tile_v;
-  Cplx2<R> *tile = (Cplx2<R> *)smem;  // [cell][pol]
+  Cplx2<R> *tile = (Cplx2<R> *)smem;  // [pol][u][v]
   St *st = (St *)(smem + (size_t)T * TV * NPOL * sizeof(Cplx2<R>));
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wp = __builtin_amdgcn_readfirstlane(tid >> 6);  // this wavefront's polarisation
   const uint32_t u = blockIdx.x;
   const uint32_t n_units = P.gunit_start[P.n_bins];
   if (u >= n_units) return;
@@ -71,17 +74,15 @@ __global__ __launch_bounds__(64) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, 
   R *gplane = A.grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
 
-  for (int i = lane; i < T * TV * NPOL; i += 64) tile[i] = Cplx2<R>{(R)0, (R)0};
-  double sw[NPOL];
-#pragma unroll
-  for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
+  for (int i = tid; i < T * TV * NPOL; i += 64 * NPOL) tile[i] = Cplx2<R>{(R)0, (R)0};
+  double sw = 0.0;  // this wavefront's polarisation
   const int ccu = A.cu_n / 2, ccv = A.cv_n / 2;
   const size_t cfsize = (size_t)A.cu_n * A.cv_n;
 
   for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
     __syncthreads();  // the previous batch's staged records are consumed
-    // ---- stage this lane's record
-    {
+    // ---- stage this lane's record (first wavefront)
+    if (wp == 0) {
       St s;
       s.prim = 0;
       s.cu = s.cv = s.cfu0 = s.cfv0 = 0;
@@ -137,47 +138,51 @@ __global__ __launch_bounds__(64) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, 
       st[lane] = s;
     }
     __syncthreads();
-    // ---- the staged records, one after another; lanes over the taps
+    // ---- the staged records, one after another; lanes over the taps of
+    // this wavefront's polarisation, v fastest (t = iu' * sv + iv')
     const int nst = (int)min((uint32_t)kMosaicStage, first + cnt - base);
+    Cplx2<R> *ptile = tile + (size_t)wp * T * TV;
     for (int k = 0; k < nst; ++k) {
       const St &s = st[k];
-#pragma unroll
-      for (int p = 0; p < NPOL; ++p) {
-        const R dre = s.dre[p], dim = s.dim[p];
-        if (dre == (R)0 && dim == (R)0) continue;  // uniform: every lane reads the same record
-        const int su = s.su[p], sv = s.sv[p], scu = su / 2, scv = sv / 2;
-        const Cplx2<R> *cf = (const Cplx2<R> *)A.cfpg + (size_t)s.base[p] * cfsize;
-        double nre = 0.0, nim = 0.0;
-        for (int t = lane; t < su * sv; t += 64) {
-          const int iv = t / su - scv, iu = t % su - scu;
-          const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
-          if (cfu < 0 || cfu >= A.cu_n || cfv < 0 || cfv >= A.cv_n) continue;
-          const Cplx2<R> c = cf[(size_t)cfu * A.cv_n + cfv];
-          nre += (double)c.re;
-          nim += (double)c.im;
-          const int lu = s.cu + iu, lv = s.cv + iv;
-          if (lu >= 0 && lu < T && lv >= 0 && lv < TV) {
-            Cplx2<R> &cell = tile[(lu * TV + lv) * NPOL + p];
-            cell.re += c.re * dre - c.im * dim;
-            cell.im += c.re * dim + c.im * dre;
-          }
+      const R dre = s.dre[wp], dim = s.dim[wp];
+      if (dre == (R)0 && dim == (R)0) continue;  // uniform: every lane reads the same record
+      const int su = s.su[wp], sv = s.sv[wp], scu = su / 2, scv = sv / 2;
+      const Cplx2<R> *cf = (const Cplx2<R> *)A.cfpg + (size_t)s.base[wp] * cfsize;
+      const float inv_sv = 1.0f / (float)sv;
+      double nre = 0.0, nim = 0.0;
+      for (int t = lane; t < su * sv; t += 64) {
+        // t / sv for t < 64 * 63 by a rounded-down float quotient and one
+        // correction
+        int q = (int)((float)t * inv_sv);
+        q = q * sv > t ? q - 1 : ((q + 1) * sv <= t ? q + 1 : q);
+        const int iu = q - scu, iv = t - q * sv - scv;
+        const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
+        if (cfu < 0 || cfu >= A.cu_n || cfv < 0 || cfv >= A.cv_n) continue;
+        const Cplx2<R> c = cf[(size_t)cfu * A.cv_n + cfv];
+        nre += (double)c.re;
+        nim += (double)c.im;
+        const int lu = s.cu + iu, lv = s.cv + iv;
+        if (lu >= 0 && lu < T && lv >= 0 && lv < TV) {
+          Cplx2<R> &cell = ptile[lu * TV + lv];
+          cell.re += c.re * dre - c.im * dim;
+          cell.im += c.re * dim + c.im * dre;
         }
-        if (s.prim) {
-          for (int off = 32; off >= 1; off >>= 1) {
-            nre += __shfl_xor(nre, off);
-            nim += __shfl_xor(nim, off);
-          }
-          const double w = (double)s.w[p];
-          sw[p] += psf ? w * nre : w * (nre * nre - nim * nim);
+      }
+      if (s.prim) {
+        for (int off = 32; off >= 1; off >>= 1) {
+          nre += __shfl_xor(nre, off);
+          nim += __shfl_xor(nim, off);
         }
+        const double w = (double)s.w[wp];
+        sw += psf ? w * nre : w * (nre * nre - nim * nim);
       }
     }
   }
   __syncthreads();
   // ---- flush the tile (cells outside the grid dropped)
   const bool multi = (fl & kUnitMulti) != 0;
-  for (int i = lane; i < T * TV * NPOL; i += 64) {
-    const int p = i % NPOL, cell = i / NPOL;
+  for (int i = tid; i < T * TV * NPOL; i += 64 * NPOL) {
+    const int p = i / (T * TV), cell = i - p * (T * TV);
     const int gu = u0 + cell / TV, gv = v0 + cell % TV;
     if (gu >= P.n_u || gv >= P.n_v) continue;
     const Cplx2<R> val = tile[i];
@@ -193,13 +198,7 @@ __global__ __launch_bounds__(64) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, 
       dst[1] = val.im;
     }
   }
-  if (lane < NPOL) {
-    double v = 0.0;
-#pragma unroll
-    for (int p = 0; p < NPOL; ++p)
-      if (p == lane) v = sw[p];
-    P.upart[(size_t)u * 4 + lane] = v;
-  }
+  if (lane == 0) P.upart[(size_t)u * 4 + wp] = sw;
 }
 
 template <typename R, int NPOL>
@@ -223,7 +222,7 @@ static int launch_mosaic(av_plan *plan, const MosaicArgs<R> &A, double *sumwt, i
     const int rc = zero_before_grid(d, A.grid, NPOL, sizeof(R) == 8, sumwt, s);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64), lds, s,
+  hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64 * NPOL), lds, s,
                      d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
   AV_LAUNCH_CHECK("k_mosaic_grid");
   return reduce_sumwt(d, NPOL, sumwt, s);
