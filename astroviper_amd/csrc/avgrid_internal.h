@@ -39,8 +39,8 @@ constexpr int kMaxPlanSupport = 63;  // binning only (the mosaic gridder)
 constexpr int kMaxPol = 4;
 constexpr int kMaxDevices = 64;
 constexpr int kChunkThreads = 1024;       // threads of a chunked-binning workgroup
-constexpr int kMaxChunkTiles = 16384;     // tiles whose counters fit in 64 KB of LDS
-constexpr int kMaxRowsPerChunk = 31 * kChunkThreads;
+constexpr int kMaxChunkTiles = 16384;     // tiles whose two counters fit in 128 KB of LDS
+constexpr int kMaxRowsPerChunk = 124 * kChunkThreads;  // rows of one chunked-binning workgroup
 constexpr int kRowBlockRows = 8 * kChunkThreads;  // rows of a row-block binning workgroup
 
 // Packed geometry of one sample relative to a tile origin:
@@ -172,9 +172,9 @@ struct PlanDev {
   // chunked binning: a chunk is rows_per_chunk rows of one channel
   int32_t chunked;                  // chunked binning possible for this plan
   int32_t rows_per_chunk, chunks_per_chan;
+  // per chunk and tile, the chunk's primary (chunk_hist) and duplicate
+  // (chunk_dup) counts, each then replaced by the first slot of that run
   uint32_t *chunk_hist;             // [n_chan * chunks_per_chan][ntu * ntv]
-  // per chunk and tile, the first slot of the chunk's duplicates (the first
-  // slot of its primaries replaces the counts in chunk_hist)
   uint32_t *chunk_dup;              // [n_chan * chunks_per_chan][ntu * ntv]
   int32_t chunk_scatter;            // records written through per-chunk LDS cursors
   // row-block binning (rows carry their own channel): kRowBlockRows rows per

@@ -265,18 +265,17 @@ __global__ __launch_bounds__(256) void k_scatter(PlanDev P) {
 
 // ---- chunked binning ------------------------------------------------------
 // A chunk is rows [r0, r0 + rows_per_chunk) of one channel.  Per tile it
-// holds (primaries) | (duplicates << 16): a row adds at most two primaries
-// and one duplicate to a tile, and rows_per_chunk <= 31744, so neither half
-// overflows.
+// counts primaries (chunk_hist) and duplicates (chunk_dup), 32 bits each.
 
 // Chunked pass 1: per-tile counts of one chunk, gathered in LDS.
 __global__ __launch_bounds__(kChunkThreads) void k_count_chunks(PlanDev P) {
-  extern __shared__ uint32_t h[];  // [n_tiles]
+  extern __shared__ uint32_t h[];  // [n_tiles] primaries, then [n_tiles] duplicates
   const int64_t w = blockIdx.x;
   const int chan = (int)(w / P.chunks_per_chan);
   const int64_t r0 = (w - (int64_t)chan * P.chunks_per_chan) * P.rows_per_chunk;
   const int ntiles = P.ntu * P.ntv;
-  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) h[i] = 0u;
+  uint32_t *hd = h + ntiles;
+  for (int i = threadIdx.x; i < 2 * ntiles; i += blockDim.x) h[i] = 0u;
   __syncthreads();
   for (int64_t row = r0 + threadIdx.x; row < r0 + P.rows_per_chunk; row += blockDim.x) {
     ItemGeo g;
@@ -289,13 +288,16 @@ __global__ __launch_bounds__(kChunkThreads) void k_count_chunks(PlanDev P) {
       atomicAdd(&h[rp.tu * P.ntv + rp.tv], 1u);
       for (int a = rp.tu0; a <= rp.tu1; ++a)
         for (int b = rp.tv0; b <= rp.tv1; ++b)
-          if (a != rp.tu || b != rp.tv) atomicAdd(&h[a * P.ntv + b], 1u << 16);
+          if (a != rp.tu || b != rp.tv) atomicAdd(&hd[a * P.ntv + b], 1u);
     }
     if (rp.dsep) atomicAdd(&h[rp.du_t * P.ntv + rp.dv_t], 1u);
   }
   __syncthreads();
-  uint32_t *dst = P.chunk_hist + w * ntiles;
-  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) dst[i] = h[i];
+  uint32_t *dst = P.chunk_hist + w * ntiles, *dstd = P.chunk_dup + w * ntiles;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+    dst[i] = h[i];
+    dstd[i] = hd[i];
+  }
 }
 
 // Chunked pass 2: the counts of every bin, summed over the chunks of the
@@ -310,9 +312,9 @@ __global__ void k_chunk_totals(PlanDev P) {
   for (int c = 0; c < P.n_chan; ++c) {
     if (P.chan_map[c] != ic) continue;
     for (int k = 0; k < P.chunks_per_chan; ++k) {
-      const uint32_t hv = P.chunk_hist[((int64_t)c * P.chunks_per_chan + k) * ntiles + t];
-      sp += hv & 0xFFFFu;
-      sd += hv >> 16;
+      const int64_t i = ((int64_t)c * P.chunks_per_chan + k) * ntiles + t;
+      sp += P.chunk_hist[i];
+      sd += P.chunk_dup[i];
     }
   }
   const int64_t b = (int64_t)t * P.n_img_chan + ic;
@@ -323,8 +325,8 @@ __global__ void k_chunk_totals(PlanDev P) {
 // Chunked pass 3 (after the bin offsets): the first slot of every chunk's
 // run in every bin.  The chunks of a bin take their runs in chunk order, so
 // the records of a bin are in sample order from chunk to chunk.  The first
-// primary slot overwrites the counts; the first duplicate slot goes to
-// chunk_dup.
+// primary slot overwrites the primary counts (chunk_hist), the first
+// duplicate slot the duplicate counts (chunk_dup).
 __global__ void k_chunk_bases(PlanDev P) {
   const int ntiles = P.ntu * P.ntv;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,11 +340,11 @@ __global__ void k_chunk_bases(PlanDev P) {
     if (P.chan_map[c] != ic) continue;
     for (int k = 0; k < P.chunks_per_chan; ++k) {
       const int64_t i = ((int64_t)c * P.chunks_per_chan + k) * ntiles + t;
-      const uint32_t hv = P.chunk_hist[i];
+      const uint32_t np = P.chunk_hist[i], nd = P.chunk_dup[i];
       P.chunk_hist[i] = sp;
       P.chunk_dup[i] = sd;
-      sp += hv & 0xFFFFu;
-      sd += hv >> 16;
+      sp += np;
+      sd += nd;
     }
   }
 }
@@ -817,7 +819,7 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
     hipLaunchKernelGGL(k_count_rows, dim3(row_grid), dim3(kChunkThreads), lds, s, d);
     AV_LAUNCH_CHECK("k_count_rows");
   } else if (d.chunked && !row_chan) {
-    const int lds = (int)(sizeof(uint32_t) * d.ntu * d.ntv);
+    const int lds = (int)(2 * sizeof(uint32_t) * d.ntu * d.ntv);
     if ((rc = ensure_lds_limit<k_count_chunks>(lds, "av_plan_set_uvw: hipFuncSetAttribute")))
       return rc;
     hipLaunchKernelGGL(k_count_chunks, dim3((unsigned)nchunks), dim3(kChunkThreads), lds, s, d);
