@@ -928,15 +928,45 @@ void k_grid_h3(PlanDev P, const float *__restrict__ vis, const float *__restrict
   const uint32_t n_units = P.gunit_start[P.n_bins];
   const uint32_t blk = xcd_grouped_block(blockIdx.x, kGridXcdGroup);
   if (blk * kGridMaxWaves >= n_units) return;
-  copy_wtab_padded(tab, mfma_lds_layout<float, NPOL>(ov, S, 16), P, (float)(1 << kH3TabExp));
-  for (int i = tid; i < L.nnrm; i += blockDim.x) nrm[i] = P.nrm[i];
-  __syncthreads();
+  // The tables (the padded weight table times 2^14, as copy_wtab_padded
+  // builds it, and the kernel-sum table) are copied into LDS while the unit
+  // and its first two batches of records load: the table rows and the unit
+  // are requested first, the records as soon as the unit arrives, and the
+  // LDS stores wait only for the table rows.
+  constexpr int kTabK = 4;  // table loads in flight per thread
+  const MfmaLds M = mfma_lds_layout<float, NPOL>(ov, S, 16);
+  const int nsrc = (S + 2) * M.nw, tlo = (M.pad - 1) * M.nw, ntab = M.rows * M.nw;
+  float tv[kTabK];
+#pragma unroll
+  for (int k = 0; k < kTabK; ++k) tv[k] = P.wtab32[min(tid + k * kGridThreads, nsrc - 1)];
+  const double nv = P.nrm[min(tid, L.nnrm - 1)];
   const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blk * kGridMaxWaves + wave));
-  if (u >= n_units) return;
-
-  const uint4 unit = P.gunits[u];
-  const int64_t b = unit.x;
+  const bool live = u < n_units;
+  uint4 unit = make_uint4(0u, 0u, 0u, 0u);
+  if (live) unit = P.gunits[u];
   const uint32_t first = unit.y, cnt = unit.z, fl = unit.w;
+  const uint32_t end = first + cnt;
+  // unconditional loads (a branch here makes the compiler wait for them, and
+  // for the table rows before them, at its join): an empty unit reads record
+  // 0, which every plan holds
+  const uint32_t lfirst = cnt ? first : 0u, lend = cnt ? end : 1u;
+  uint2 r0 = load_record(P, lfirst + lane, lend), r1 = load_record(P, lfirst + 64 + lane, lend);
+  {
+    const float scale = (float)(1 << kH3TabExp);
+    for (int i = tid; i < tlo; i += kGridThreads) tab[i] = 0.f;
+    for (int i = tlo + nsrc + tid; i < ntab; i += kGridThreads) tab[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < kTabK; ++k)
+      if (tid + k * kGridThreads < nsrc) tab[tlo + tid + k * kGridThreads] = tv[k] * scale;
+    for (int i = tid + kTabK * kGridThreads; i < nsrc; i += kGridThreads)
+      tab[tlo + i] = P.wtab32[i] * scale;  // tables of more than kTabK rows of threads
+    if (tid < L.nnrm) nrm[tid] = nv;
+    for (int i = tid + kGridThreads; i < L.nnrm; i += kGridThreads) nrm[i] = P.nrm[i];
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int64_t b = unit.x;
   const uint32_t tile = (uint32_t)b / (uint32_t)P.n_img_chan;
   const int ic = (int)((uint32_t)b - tile * (uint32_t)P.n_img_chan);
   const uint32_t tu_i = tile / (uint32_t)P.ntv;
@@ -968,8 +998,6 @@ void k_grid_h3(PlanDev P, const float *__restrict__ vis, const float *__restrict
 #pragma unroll
   for (int p = 0; p < NPOL; ++p) sw[p] = 0.0;
 
-  const uint32_t end = first + cnt;
-  uint2 r0 = load_record(P, first + lane, end), r1 = load_record(P, first + 64 + lane, end);
   RawSample<float, NPOL> raw0;
   fetch_raw<float, NPOL>(raw0, r0.x, vis, weight, flag, PSF, adjoint);
   for (uint32_t base = first; base < end; base += 64) {

@@ -692,7 +692,7 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
          o_call = take(4 * nb), o_cprim = take(4 * nb), o_bstart = take(4 * (nb + 1)),
          o_curp = take(4 * nb), o_curd = take(4 * nb), o_gus = take(4 * (nb + 1)),
          o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
-         o_rec = take(12 * d.max_records), o_gun = take(sizeof(uint4) * d.max_gunits),
+         o_rec = take(12 * std::max<int64_t>(d.max_records, 1)), o_gun = take(sizeof(uint4) * d.max_gunits),
          o_dun = take(sizeof(uint4) * d.max_dunits),
          o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
          o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells), o_rh = take(4 * row_cells),
