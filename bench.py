@@ -259,7 +259,7 @@ def small_workload_leg(dev, reps=10):
 # and resident major cycles, each on one GPU with a bounded CPU leg and the
 # GPU result compared with the CPU one on the same sample.
 #
-#     python bench.py --rows hogbom,weights,mosaic,major_cycle
+#     python bench.py --rows hogbom,weights,mosaic,gcf,major_cycle
 #
 # prints one JSON line per row (the headline line is not printed then).
 # --------------------------------------------------------------------------
@@ -741,9 +741,97 @@ def row_major_cycle(dev, reps=3):
                                     "M_vis_per_s": round(V / ((tot + t_x) / 1e3) / 1e6, 1)}}
 
 
+def row_gcf(dev, n=1024, n_chan=64, cpu_s=5.0):
+    """The mosaic gridder's convolution functions
+    (make_gridding_convolution_function.py:24-193) for a 2-type array (3
+    antenna-type pairs), 64 channels over 1.0-1.4 GHz (57 primary-beam
+    channels at the default tolerance), a 1024 x 1024 padded grid,
+    oversampling 10: aperture patterns (voltage and power), baseline
+    products, FFTs, supports, crops and the phase gradient.
+
+    Algorithmic bytes: each of the two pattern sets is written once by the
+    pattern kernel and read and written by each FFT pass (counted as one
+    read and one write of the complex plane)."""
+    import torch
+    from types import SimpleNamespace
+
+    from astroviper_amd.make_gridding_convolution_function import \
+        make_gridding_convolution_function
+
+    n_ant = 12
+    names = np.array([f"A{i:02d}" for i in range(n_ant)])
+    bl = np.array([(i, j) for i in range(n_ant) for j in range(i + 1, n_ant)])
+    freq = np.linspace(1.0e9, 1.4e9, n_chan)
+    field = SimpleNamespace(FIELD_PHASE_CENTER_DIRECTION=np.array([1.0 + 1e-3, 0.5]))
+    ms = SimpleNamespace(xr_ms=SimpleNamespace(get_field_and_source_xds=lambda name: field),
+                         antenna_xds=SimpleNamespace(antenna_name=names),
+                         baseline_antenna1_name=names[bl[:, 0]],
+                         baseline_antenna2_name=names[bl[:, 1]], frequency=freq,
+                         polarization=np.array(["XX", "YY"]))
+    gcf = {"function": "casa_airy", "list_dish_diameters": np.array([10.7, 6.0]),
+           "list_blockage_diameters": np.array([0.75, 0.0]),
+           "unique_ant_indx": (np.arange(n_ant) % 3 == 2).astype(int),
+           "phase_direction": np.array([1.0, 0.5])}
+    gp = {"image_size": np.array([n, n]), "image_size_padded": np.array([n, n]),
+          "cell_size": np.array([-3.5e-5, 3.5e-5])}
+    run = lambda: make_gridding_convolution_function(None, ms, gcf, gp, {})
+    out = run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    npair, npb = out["CONV_KERNEL"].data.shape[:2]
+    planes = npair * npb
+    nbytes = 2 * planes * n * n * (8 + 2 * 16)
+    res = {"row": "gcf", "workload": f"convolution functions, {npair} antenna-type pairs x {npb} "
+                                      f"primary-beam channels, {n}x{n} padded grid, "
+                                      f"oversampling 10, casa_airy",
+           "value": round(planes / t, 1), "unit": "kernel planes/sec (voltage + power)",
+           "ms": round(t * 1e3, 2), "supports": sorted(set(
+               int(v) for v in np.asarray(out["SUPPORT"].data).ravel())),
+           "roofline": _roof(nbytes, t * 1e3, "patterns + FFT + crop (whole call)")}
+    # CPU leg: the oracle's restatement (NumPy/SciPy, one thread for the
+    # FFT) on the first antenna-type pair and the first primary-beam
+    # channels, and the GPU planes compared with it
+    from oracle import oracle as O
+    g = dict(gcf, oversampling=np.array([10, 10]), max_support=np.array([15, 15]),
+             support_cut_level=2.5e-2)
+    g["resize_conv_size"] = (g["max_support"] + 1) * g["oversampling"]
+    cmap, pbf = O.gcf_cf_chan_map(freq, 0.005)
+    k = 1
+    while True:
+        t0 = time.perf_counter()
+        ks = []
+        for ipower in (1, 2):
+            g["ipower"] = ipower
+            bp = O.gcf_baseline_patterns(pbf[:k], [0], np.array([[0, 0]]), True, g, gp)
+            ks.append(np.real(np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(
+                bp, axes=(3, 4)), axes=(3, 4)), axes=(3, 4))))
+        rk, rkc, sup = O.gcf_resize_and_calc_support(ks[0], ks[1], g, gp)
+        tc = time.perf_counter() - t0
+        if tc > cpu_s / 4 or 2 * k > len(pbf):
+            break
+        k *= 2
+    got = np.asarray(out["CONV_KERNEL"].data)[0, :k]
+    res["cpu_baseline"] = {"value": round(k / tc, 3), "unit": "kernel planes/sec (voltage + power)",
+                           "cores": 1, "kind": "port",
+                           "sample": f"pair (0, 0), first {k} primary-beam channels: {tc:.2f} s"}
+    res["parity_vs_cpu"] = {"planes": k,
+                            "support_equal": bool(np.array_equal(
+                                np.asarray(out["SUPPORT"].data)[0, :k], sup[0])),
+                            "conv_kernel_max_rel": float(np.max(np.abs(got - rk[0]))
+                                                         / np.max(np.abs(rk[0])))}
+    return res
+
+
 def run_rows(names, dev):
     rows_fn = {"hogbom": lambda: row_hogbom(dev),
                "mosaic": lambda: row_mosaic(dev),
+               "gcf": lambda: row_gcf(dev),
                "major_cycle": lambda: row_major_cycle(dev)}
     for name in names:
         if name == "weights":
@@ -789,7 +877,7 @@ def main():
     ap.add_argument("--time-samples", type=int, default=1000)
     ap.add_argument("--rows", default="",
                     help="comma list of rows beside the hot path to measure instead of the "
-                         "headline: hogbom, weights, mosaic, major_cycle")
+                         "headline: hogbom, weights, mosaic, gcf, major_cycle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the timed steps, copy a BYTES-sized device buffer once "
                          "(the reference kernel for scaling HBM byte counters)")

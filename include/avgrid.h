@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 7
+#define AV_ABI_VERSION 8
 
 const char *av_last_error(void);
 int av_version(void);
@@ -339,6 +339,30 @@ int av_degrid_imaging_weights(const double *grid, const double *briggs,
                               double cell_u, double cell_v,
                               double *imaging_weight, int32_t *nan_flag,
                               void *stream);
+
+/* --------------------------------------- mosaic convolution functions --
+ * The per-pixel parts of make_gridding_convolution_function
+ * (core/imaging/make_gridding_convolution_function.py:24-193). */
+
+/* airy_disk_rorder (ipower, casa = 0) and casa_airy_disk_rorder (casa = 1),
+ * core/imaging/imaging_utils/make_pb_symmetric.py:148-280, as
+ * make_baseline_patterns (make_gridding_convolution_function.py:196-229)
+ * evaluates them: out [n_ant][n_freq][n0][n1] float64, the pattern of
+ * antenna type a (dish[a], blockage[a] in metres) at wavenumber kf[f] =
+ * 2 pi freq / c over pixels ((i - c0) cell0, (j - c1) cell1) in radians,
+ * raised to ipower (1: voltage, 2: power); 1 at the centre pixel. */
+int av_gcf_aperture_patterns(const double *kf, int32_t n_freq, const double *dish,
+                             const double *blockage, int32_t n_ant, int32_t n0,
+                             int32_t n1, int32_t c0, int32_t c1, double cell0,
+                             double cell1, int32_t ipower, int32_t casa,
+                             double *out, void *stream);
+
+/* make_phase_gradient (make_gridding_convolution_function.py:426-485) from
+ * its per-field phase steps pix [n_field][2] (radians per pixel): out
+ * [n_field][nx][ny] complex128 = exp(1j ((x - cx) pix0 + (y - cy) pix1)). */
+int av_gcf_phase_gradient(const double *pix, int32_t n_field, int32_t nx,
+                          int32_t ny, int32_t cx, int32_t cy, double *out,
+                          void *stream);
 
 #ifdef __cplusplus
 }

@@ -515,3 +515,155 @@ def mosaic_grid(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map, po
         int(n_uv[1]), float(delta_lm[0]), float(delta_lm[1]), _i64p(ws), int(np.max(ws)),
         int(oversampling[0]), int(oversampling[1]))
     return grid, sum_weight
+
+
+# --------------------------------------------------------------------------
+# Convolution functions of the mosaic gridder
+# (core/imaging/make_gridding_convolution_function.py and
+# core/imaging/imaging_utils/make_pb_symmetric.py), in NumPy/SciPy as the
+# reference computes them.  The reference's make_phase_gradient projects the
+# field direction with astropy's RA---SIN/DEC--SIN WCS, which is absent here;
+# gcf_sin_offset restates that projection (zenithal SIN, native longitude of
+# the pole 180 degrees, unit PC matrix).
+# --------------------------------------------------------------------------
+def gcf_aperture_patterns(freq_chan, pol, pb_params, grid_params, casa):
+    """airy_disk_rorder (casa False) / casa_airy_disk_rorder (casa True),
+    make_pb_symmetric.py:148-280: [n_ant_type, n_freq, n_pol, n0, n1]."""
+    from scipy.special import jn
+    cell = np.asarray(grid_params["cell_size"], dtype=np.float64)
+    size = np.asarray(grid_params["image_size"])
+    ctr = np.asarray(grid_params["image_center"])
+    dishes = np.asarray(pb_params["list_dish_diameters"], dtype=np.float64)
+    blocks = np.asarray(pb_params["list_blockage_diameters"], dtype=np.float64)
+    ipower = pb_params["ipower"]
+    kf = (2 * np.pi * np.asarray(freq_chan, dtype=np.float64)) / C_LIGHT
+    x = np.arange(-ctr[0], size[0] - ctr[0]) * cell[0]
+    y = np.arange(-ctr[1], size[1] - ctr[1]) * cell[1]
+    xg, yg = np.meshgrid(x, y, indexing="ij")
+    rad = np.sqrt(xg ** 2 + yg ** 2)
+    out = np.zeros((len(blocks), len(kf), 1, size[0], size[1]))
+    for a, (d, b) in enumerate(zip(dishes, blocks)):
+        r = np.moveaxis(rad[:, :, None] * kf * (d / 2), 2, 0)
+        r[:, ctr[0], ctr[1]] = 1.0
+        if b == 0.0:
+            v = 2.0 * jn(1, r) / r
+        elif not casa:
+            e = b / d
+            v = (2.0 * jn(1, r) / r - 2.0 * e * jn(1, r * e) / r) / (1.0 - e ** 2)
+        else:
+            ar, lr = (d / b) ** 2, d / b
+            v = (ar * 2.0 * jn(1, r) / r - 2.0 * jn(1, r * lr) / (r * lr)) / (ar - 1.0)
+        out[a, :, 0] = v ** ipower
+    out[:, :, 0, ctr[0], ctr[1]] = 1.0
+    return np.tile(out, (1, 1, len(pol), 1, 1))
+
+
+def gcf_baseline_patterns(pb_freq, pb_pol, pb_ant_pairs, casa, gcf_params, grid_params):
+    """make_baseline_patterns (make_gridding_convolution_function.py:196-229)."""
+    pgp = dict(grid_params)
+    pgp["cell_size"] = np.asarray(grid_params["cell_size"]) * np.asarray(gcf_params["oversampling"])
+    pgp["image_size"] = np.asarray(grid_params["image_size_padded"])
+    pgp["image_center"] = pgp["image_size"] // 2
+    pat = gcf_aperture_patterns(pb_freq, pb_pol, gcf_params, pgp, casa)
+    n = pgp["image_size"]
+    out = np.zeros((len(pb_ant_pairs), len(pb_freq), len(pb_pol), n[0], n[1]))
+    for k, (i, j) in enumerate(pb_ant_pairs):
+        out[k, :, 0] = pat[i, :, 0] * pat[j, :, 0]
+    return out
+
+
+def gcf_cf_baseline_map(unique_ant_indx, basline_ant, n_unique_ant):
+    """create_cf_baseline_map (make_gridding_convolution_function.py:232-257)."""
+    pairs = np.array([[i, j] for i in range(n_unique_ant) for j in range(i, n_unique_ant)],
+                     dtype=int).reshape(-1, 2)
+    u = np.asarray(unique_ant_indx)
+    ba = np.asarray(basline_ant)
+    t0, t1 = u[ba[:, 0]], u[ba[:, 1]]
+    cbm = np.zeros(ba.shape[0], dtype=int)
+    for k, (i, j) in enumerate(pairs):
+        cbm[(t0 == i) & (t1 == j)] = k
+    return cbm, pairs
+
+
+def gcf_cf_chan_map(freq_chan, chan_tolerance_factor):
+    """create_cf_chan_map (make_gridding_convolution_function.py:260-288)."""
+    f = np.asarray(freq_chan, dtype=np.float64)
+    n = len(f)
+    tol = np.max(f) * chan_tolerance_factor
+    npb = int(np.floor((np.max(f) - np.min(f)) / tol) + 0.5)
+    npb = max(npb, 1)
+    if npb >= n:
+        return np.arange(n), f
+    step = (np.max(f) - np.min(f)) / npb
+    pb_freq = np.arange(npb) * step + np.min(f) + step / 2
+    return np.array([int(np.argmin(np.abs(pb_freq - v))) for v in f]), pb_freq
+
+
+def gcf_calc_conv_size(sub, imsize, support_cut_level, oversampling, max_support):
+    """calc_conv_size (make_gridding_convolution_function.py:374-423)."""
+    a = np.abs(sub)
+    cut = support_cut_level * a.max()
+    assert a.min() < cut
+    out = []
+    for ax in (0, 1):
+        idx = [imsize[0] // 2, imsize[1] // 2]
+        while sub[idx[0], idx[1]] > cut:
+            idx[ax] += 1
+            assert idx[ax] < imsize[ax]
+        out.append((int(0.5 + (idx[ax] - imsize[ax] // 2) / oversampling[ax]) + 1) * 2 + 1)
+    assert out[0] < max_support[0] and out[1] < max_support[1]
+    s = max(out)
+    return [s, s]
+
+
+def gcf_resize_and_calc_support(ck, ckc, gcf_params, grid_params):
+    """resize_and_calc_support (make_gridding_convolution_function.py:291-371)."""
+    shp = ck.shape[:3]
+    rcs = np.asarray(gcf_params["resize_conv_size"])
+    ov = np.asarray(gcf_params["oversampling"])
+    npad = np.asarray(grid_params["image_size_padded"])
+    s0 = npad // 2 - rcs // 2
+    e0 = s0 + rcs
+    sup = np.zeros(shp + (2,), dtype=int)
+    rk = np.zeros(shp + tuple(rcs))
+    rkc = np.zeros(shp + tuple(rcs))
+    for idx in np.ndindex(*shp):
+        sup[idx] = gcf_calc_conv_size(ckc[idx], npad, gcf_params["support_cut_level"], ov,
+                                      gcf_params["max_support"])
+        emb = (sup[idx] + 1) * ov
+        es = rcs // 2 - emb // 2
+        ee = es + emb
+        for src, dst in ((ck, rk), (ckc, rkc)):
+            dst[idx] = src[idx][s0[0]:e0[0], s0[1]:e0[1]]
+            dst[idx] = dst[idx] / np.real(np.sum(dst[idx][es[0]:ee[0], es[1]:ee[1]])
+                                          / (ov[0] * ov[1]))
+    return rk, rkc, sup
+
+
+def gcf_sin_offset(field_dir, phase_center, cell):
+    """Pixel offset of field_dir [n, 2] (RA, Dec radians) from the reference
+    pixel of a RA---SIN/DEC--SIN image centred on phase_center with pixel
+    steps cell (radians): (l / cell0, m / cell1) with l = cos d sin(a - a0),
+    m = sin d cos d0 - cos d sin d0 cos(a - a0)."""
+    fd = np.atleast_2d(np.asarray(field_dir, dtype=np.float64))
+    a, d = fd[:, 0], fd[:, 1]
+    a0, d0 = float(phase_center[0]), float(phase_center[1])
+    l = np.cos(d) * np.sin(a - a0)
+    m = np.sin(d) * np.cos(d0) - np.cos(d) * np.sin(d0) * np.cos(a - a0)
+    return np.stack([l / cell[0], m / cell[1]], axis=1)
+
+
+def gcf_phase_gradient(field_phase_dir, gcf_params, grid_params):
+    """make_phase_gradient (make_gridding_convolution_function.py:426-485):
+    [n_field, resize0, resize1] complex."""
+    npad = np.asarray(grid_params["image_size_padded"])
+    pd = gcf_sin_offset(field_phase_dir, np.asarray(gcf_params["phase_direction"]),
+                        np.asarray(grid_params["cell_size"], dtype=np.float64))
+    pix = -pd * 2 * np.pi / (npad * np.asarray(gcf_params["oversampling"]))
+    size = np.asarray(gcf_params["resize_conv_size"])
+    c = size // 2
+    x = np.arange(-c[0], size[0] - c[0])
+    y = np.arange(-c[1], size[1] - c[1])
+    xg, yg = np.meshgrid(x, y, indexing="ij")
+    return np.moveaxis(np.exp(1j * (xg[:, :, None] * pix[:, 0] + yg[:, :, None] * pix[:, 1])),
+                       2, 0)
