@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 8  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 9  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -62,8 +62,8 @@ SIGNATURES = {
     "av_degrid": (_c.c_int, [_P, _P, _I32, _P, _P, _I32, _P, _P, _I32, _I32, _P]),
     "av_grid_noalloc": (_c.c_int, [_P, _I64, _P, _I32, _P, _I32, _I32, _I32, _D, _D, _I32, _I32,
                                    _I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _I32, _I32, _P]),
-    "av_mosaic_grid": (_c.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P, _I32, _I32, _I32, _I32,
-                                  _P, _I32, _I32, _P, _P, _I32, _I32, _P]),
+    "av_mosaic_grid": (_c.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P, _I32, _I32, _I32,
+                                  _I32, _I32, _P, _I32, _I32, _P, _P, _I32, _I32, _P]),
     "av_grid_plain": (_c.c_int, [_P, _I64, _I32, _I32, _D, _I32, _P, _P, _P, _I32, _P,
                                  _I32, _I32, _D, _D, _I32, _I32, _P, _P, _I32, _I32, _P]),
     "av_degrid_plain": (_c.c_int, [_P, _I64, _I32, _I32, _D, _I32, _P, _I32, _P, _I32,

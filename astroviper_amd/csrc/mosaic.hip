@@ -12,19 +12,24 @@
 // One workgroup per work unit, one wavefront per polarisation.  The unit's
 // tile, all its polarisations (polarisation-major, so the lanes of a
 // wavefront, which run along v, touch consecutive cells), is accumulated in
-// LDS.  Records are staged 64 at a time by the first wavefront (one per
-// lane: its FP64 position, kernel offsets, convolution-function bases and
-// supports, weighted data); every wavefront then takes the staged records
-// one after another, its lanes over the taps of its own polarisation.  A tap reads
-// its complex kernel value from the convolution function in global memory
-// (read-only, shared by every unit of the same baseline group, so it stays
-// in L2) and adds kernel * data into its tile cell: the taps of one record
-// are distinct cells and only this wavefront touches the tile, so no
-// atomics.  The kernel sum of a record (the reference's `norm`, over all its
-// taps) is reduced across the wavefront for primary records only and gives
-// the sum of weights: weight * Re(norm) for the PSF, weight * Re(norm^2)
-// for data (mosaic_grid.py:470-484).  The tile then goes to HBM once, split
-// bins adding theirs with atomics.
+// LDS.  Records are staged 64 at a time, one per lane, each wavefront
+// staging its own polarisation (FP64 position, kernel offsets,
+// convolution-function base and supports, weighted data); every wavefront
+// then takes the staged records
+// one after another, its lanes over the taps of its own polarisation.  A
+// tap reads its complex kernel value from the convolution function in
+// global memory (read-only, shared by every unit of the same baseline
+// group, so it stays in L2) and adds kernel * data into its tile cell: the
+// taps of one record are distinct cells and only this wavefront touches the
+// tile, so no atomics.  The tile then goes to HBM once, split bins adding
+// theirs with atomics.
+//
+// The kernel sum of a record (the reference's `norm`, over all its taps)
+// depends only on its convolution function and the kernel offsets of its
+// first tap, so it is tabulated per call, summed in the reference's order
+// (v outer, u inner, in FP64), and each wavefront adds its primary
+// records' share of the sum of weights as it stages them: weight * Re(norm)
+// for the PSF, weight * Re(norm^2) for data (mosaic_grid.py:470-484).
 #include "tile_common.h"
 
 namespace av {
@@ -37,8 +42,12 @@ struct MosaicArgs {
   const int32_t *cf_baseline_map, *cf_chan_map, *cf_pol_map;
   const R *cfpg;  // complex [ncb][ncc][ncp][cu][cv]
   const int32_t *weight_support;  // [ncb][ncc][ncp][2]
-  int n_baseline, ncc, ncp, cu_n, cv_n, ov_u, ov_v;
+  int n_baseline, ncb, ncc, ncp, cu_n, cv_n, ov_u, ov_v;
   R *grid;
+  // kernel sums [ncb * ncc * ncp][ndu][ndv] (re, im), for first-tap kernel
+  // offsets cu_n / 2 + du_lo + du and cv_n / 2 + dv_lo + dv
+  const double *norms;
+  int du_lo, ndu, dv_lo, ndv;
 };
 
 // per staged record: centre in tile coordinates, cf row/column of tap 0,
@@ -47,9 +56,35 @@ template <typename R, int NPOL>
 struct MosaicStaged {
   int cu, cv, cfu0, cfv0;
   int base[NPOL], su[NPOL], sv[NPOL];
-  R dre[NPOL], dim[NPOL], w[NPOL];
-  int prim;
+  R dre[NPOL], dim[NPOL];
 };
+
+// The kernel-sum table: one thread per (convolution function, first-tap
+// offsets), summing the in-range taps as the reference does.
+template <typename R>
+__global__ __launch_bounds__(256) void k_mosaic_norms(MosaicArgs<R> A, int ncf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)A.ndu * A.ndv;
+  if (i >= per * ncf) return;
+  const int b = (int)(i / per), rem = (int)(i - (int64_t)b * per);
+  const int du = rem / A.ndv, dv = rem - du * A.ndv;
+  const int cfu0 = A.cu_n / 2 + A.du_lo + du, cfv0 = A.cv_n / 2 + A.dv_lo + dv;
+  const int su = A.weight_support[2 * b], sv = A.weight_support[2 * b + 1];
+  const int scu = su / 2, scv = sv / 2;
+  const Cplx2<R> *cf = (const Cplx2<R> *)A.cfpg + (size_t)b * A.cu_n * A.cv_n;
+  double nre = 0.0, nim = 0.0;
+  for (int iv = -scv; iv < sv - scv; ++iv) {
+    const int cfv = A.ov_v * iv + cfv0;
+    for (int iu = -scu; iu < su - scu; ++iu) {
+      const int cfu = A.ov_u * iu + cfu0;
+      if (cfu < 0 || cfu >= A.cu_n || cfv < 0 || cfv >= A.cv_n) continue;
+      const Cplx2<R> c = cf[(size_t)cfu * A.cv_n + cfv];
+      nre = nre + (double)c.re;
+      nim = nim + (double)c.im;
+    }
+  }
+  ((double2 *)A.norms)[i] = make_double2(nre, nim);
+}
 
 template <typename R, int NPOL>
 __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs<R> A, int psf,
@@ -73,24 +108,38 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
   const uint32_t prim_end = P.bin_start[b] + P.cnt_prim[b];
   R *gplane = A.grid + (size_t)ic * NPOL * P.n_u * P.n_v * 2;
   const size_t pstride = (size_t)P.n_u * P.n_v * 2;
+  const int ncf = A.ncb * A.ncc * A.ncp;
 
   for (int i = tid; i < T * TV * NPOL; i += 64 * NPOL) tile[i] = Cplx2<R>{(R)0, (R)0};
-  double sw = 0.0;  // this wavefront's polarisation
+  double swl = 0.0;  // this lane's records' sum of weights, this wavefront's polarisation
   const int ccu = A.cu_n / 2, ccv = A.cv_n / 2;
   const size_t cfsize = (size_t)A.cu_n * A.cv_n;
+  const Cplx2<R> *cfall = (const Cplx2<R> *)A.cfpg;
+
+  // tap t of staged record s for this wavefront's polarisation: its kernel
+  // value's index (always in bounds) and whether the tap counts
+  auto tap = [&](const St &s, int t, size_t &idx, bool &ok, int &lu, int &lv) {
+    const int su = s.su[wp], sv = max(s.sv[wp], 1);
+    int q = (int)((float)t * (1.0f / (float)sv));  // t / sv, then one correction
+    q = q * sv > t ? q - 1 : ((q + 1) * sv <= t ? q + 1 : q);
+    const int iu = q - su / 2, iv = t - q * sv - sv / 2;
+    const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
+    ok = t < su * s.sv[wp] && cfu >= 0 && cfu < A.cu_n && cfv >= 0 && cfv < A.cv_n;
+    idx = (size_t)s.base[wp] * cfsize +
+          (size_t)min(max(cfu, 0), A.cu_n - 1) * A.cv_n + min(max(cfv, 0), A.cv_n - 1);
+    lu = s.cu + iu;
+    lv = s.cv + iv;
+  };
 
   for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
     __syncthreads();  // the previous batch's staged records are consumed
-    // ---- stage this lane's record (first wavefront)
-    if (wp == 0) {
-      St s;
-      s.prim = 0;
-      s.cu = s.cv = s.cfu0 = s.cfv0 = 0;
-#pragma unroll
-      for (int p = 0; p < NPOL; ++p) {
-        s.base[p] = s.su[p] = s.sv[p] = 0;
-        s.dre[p] = s.dim[p] = s.w[p] = (R)0;
-      }
+    // ---- stage this lane's record: every wavefront forms the record's
+    // geometry and its own polarisation's data (the first also stores the
+    // geometry), so the four chains of loads run side by side
+    {
+      St &s = st[lane];
+      int cu = 0, cv = 0, cfu0 = 0, cfv0 = 0, pbase = 0, psu = 0, psv = 0;
+      R pdre = (R)0, pdim = (R)0;
       const uint32_t r = base + lane;
       if (r < first + cnt) {
         const uint32_t item = P.rec[3 * (size_t)r];
@@ -99,43 +148,64 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
         unpack_geo(geo, au, ou, avv, ovv);
         const int64_t row = (int64_t)(item / (uint32_t)P.n_chan);
         const int ch = (int)(item - (uint32_t)row * (uint32_t)P.n_chan);
+        const R w = A.weight[(size_t)item * NPOL + wp];
+        R vre = (R)0, vim = (R)0;
+        if (!psf) {
+          vre = A.vis[((size_t)item * NPOL + wp) * 2];
+          vim = A.vis[((size_t)item * NPOL + wp) * 2 + 1];
+        }
+        const int cfb = A.cf_baseline_map[(int)(row % A.n_baseline)];
+        const int cfc = A.cf_chan_map[ch];
+        const int idx = (cfb * A.ncc + cfc) * A.ncp + A.cf_pol_map[wp];
         // the reference's offsets at this call's oversampling, from the
         // same FP64 positions the plan binned (mosaic_grid.py:376-386)
         const double uu = P.uvw[row * 3] * P.gscale_u[ch] + (double)(P.n_u / 2);
         const double vv = P.uvw[row * 3 + 1] * P.gscale_v[ch] + (double)(P.n_v / 2);
         const int gcu = u0 + au, gcv = v0 + avv;
-        s.cu = au;
-        s.cv = avv;
-        s.cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
-        s.cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
-        s.prim = r < prim_end;
-        const int cfb = A.cf_baseline_map[(int)(row % A.n_baseline)];
-        const int cfc = A.cf_chan_map[ch];
-#pragma unroll
-        for (int p = 0; p < NPOL; ++p) {
-          const int cfp = A.cf_pol_map[p];
-          const int idx = (cfb * A.ncc + cfc) * A.ncp + cfp;
-          s.base[p] = idx;
-          s.su[p] = A.weight_support[2 * idx];
-          s.sv[p] = A.weight_support[2 * idx + 1];
-          const R w = A.weight[(size_t)item * NPOL + p];
-          R dre, dim;
-          if (psf) {
-            dre = w;
-            dim = (R)0;
-          } else {
-            dre = A.vis[((size_t)item * NPOL + p) * 2] * w;
-            dim = A.vis[((size_t)item * NPOL + p) * 2 + 1] * w;
-          }
-          // mosaic_grid.py:412: skip NaN or exactly-zero data
-          if (!(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
-            s.dre[p] = dre;
-            s.dim[p] = dim;
-            s.w[p] = w;
+        cu = au;
+        cv = avv;
+        cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
+        cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
+        const bool prim = r < prim_end;
+        const int du = min(max(cfu0 - ccu - A.du_lo, 0), A.ndu - 1);
+        const int dv = min(max(cfv0 - ccv - A.dv_lo, 0), A.ndv - 1);
+        const bool cf_ok = idx >= 0 && idx < ncf;
+        const int idc = cf_ok ? idx : 0;  // in-bounds reads whatever the maps hold
+        const int wsu = A.weight_support[2 * idc], wsv = A.weight_support[2 * idc + 1];
+        const double2 nm = ((const double2 *)A.norms)[((size_t)idc * A.ndu + du) * A.ndv + dv];
+        R dre, dim;
+        if (psf) {
+          dre = w;
+          dim = (R)0;
+        } else {
+          dre = vre * w;
+          dim = vim * w;
+        }
+        // mosaic_grid.py:412: skip NaN or exactly-zero data (and, here,
+        // convolution functions outside the table)
+        if (cf_ok && !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
+          pbase = idx;
+          psu = wsu;
+          psv = wsv;
+          pdre = dre;
+          pdim = dim;
+          if (prim) {
+            const double wd = (double)w;
+            swl += psf ? wd * nm.x : wd * (nm.x * nm.x - nm.y * nm.y);
           }
         }
       }
-      st[lane] = s;
+      if (wp == 0) {
+        s.cu = cu;
+        s.cv = cv;
+        s.cfu0 = cfu0;
+        s.cfv0 = cfv0;
+      }
+      s.base[wp] = pbase;
+      s.su[wp] = psu;
+      s.sv[wp] = psv;
+      s.dre[wp] = pdre;
+      s.dim[wp] = pdim;
     }
     __syncthreads();
     // ---- the staged records, one after another; lanes over the taps of
@@ -146,35 +216,18 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
       const St &s = st[k];
       const R dre = s.dre[wp], dim = s.dim[wp];
       if (dre == (R)0 && dim == (R)0) continue;  // uniform: every lane reads the same record
-      const int su = s.su[wp], sv = s.sv[wp], scu = su / 2, scv = sv / 2;
-      const Cplx2<R> *cf = (const Cplx2<R> *)A.cfpg + (size_t)s.base[wp] * cfsize;
-      const float inv_sv = 1.0f / (float)sv;
-      double nre = 0.0, nim = 0.0;
-      for (int t = lane; t < su * sv; t += 64) {
-        // t / sv for t < 64 * 63 by a rounded-down float quotient and one
-        // correction
-        int q = (int)((float)t * inv_sv);
-        q = q * sv > t ? q - 1 : ((q + 1) * sv <= t ? q + 1 : q);
-        const int iu = q - scu, iv = t - q * sv - scv;
-        const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
-        if (cfu < 0 || cfu >= A.cu_n || cfv < 0 || cfv >= A.cv_n) continue;
-        const Cplx2<R> c = cf[(size_t)cfu * A.cv_n + cfv];
-        nre += (double)c.re;
-        nim += (double)c.im;
-        const int lu = s.cu + iu, lv = s.cv + iv;
-        if (lu >= 0 && lu < T && lv >= 0 && lv < TV) {
+      const int ntaps = s.su[wp] * s.sv[wp];
+      for (int t = lane; t < ntaps; t += 64) {
+        size_t idx;
+        bool ok;
+        int lu, lv;
+        tap(s, t, idx, ok, lu, lv);
+        if (ok && lu >= 0 && lu < T && lv >= 0 && lv < TV) {
+          const Cplx2<R> c = cfall[idx];
           Cplx2<R> &cell = ptile[lu * TV + lv];
           cell.re += c.re * dre - c.im * dim;
           cell.im += c.re * dim + c.im * dre;
         }
-      }
-      if (s.prim) {
-        for (int off = 32; off >= 1; off >>= 1) {
-          nre += __shfl_xor(nre, off);
-          nim += __shfl_xor(nim, off);
-        }
-        const double w = (double)s.w[wp];
-        sw += psf ? w * nre : w * (nre * nre - nim * nim);
       }
     }
   }
@@ -198,11 +251,12 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
       dst[1] = val.im;
     }
   }
-  if (lane == 0) P.upart[(size_t)u * 4 + wp] = sw;
+  for (int off = 32; off >= 1; off >>= 1) swl += __shfl_xor(swl, off);
+  if (lane == 0) P.upart[(size_t)u * 4 + wp] = swl;
 }
 
 template <typename R, int NPOL>
-static int launch_mosaic(av_plan *plan, const MosaicArgs<R> &A, double *sumwt, int mode,
+static int launch_mosaic(av_plan *plan, MosaicArgs<R> A, double *sumwt, int mode,
                          hipStream_t s) {
   PlanDev &d = plan->d;
   const size_t lds = (size_t)d.tile_u * d.tile_v * NPOL * sizeof(Cplx2<R>) +
@@ -222,9 +276,29 @@ static int launch_mosaic(av_plan *plan, const MosaicArgs<R> &A, double *sumwt, i
     const int rc = zero_before_grid(d, A.grid, NPOL, sizeof(R) == 8, sumwt, s);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64 * NPOL), lds, s,
-                     d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
-  AV_LAUNCH_CHECK("k_mosaic_grid");
+  // the kernel-sum table, in stream-ordered memory freed after the gridder
+  // (first-tap offsets: floor(x * ov + 0.5) for x in (-1/2, 1/2], within
+  // [-ov / 2 - 1, ov / 2 + 1])
+  const int64_t ncf = (int64_t)A.ncb * A.ncc * A.ncp;
+  A.du_lo = -(A.ov_u / 2) - 1;
+  A.ndu = A.ov_u + 3;
+  A.dv_lo = -(A.ov_v / 2) - 1;
+  A.ndv = A.ov_v + 3;
+  const int64_t entries = ncf * A.ndu * A.ndv;
+  void *norms = nullptr;
+  AV_HIP(hipMallocAsync(&norms, (size_t)entries * 2 * sizeof(double), s));
+  A.norms = (const double *)norms;
+  hipLaunchKernelGGL((k_mosaic_norms<R>), dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, s,
+                     A, (int)ncf);
+  int rc = hipGetLastError() == hipSuccess ? AV_OK : fail(AV_ERR_HIP, "k_mosaic_norms launch");
+  if (rc == AV_OK) {
+    hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64 * NPOL),
+                       lds, s, d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) rc = fail(AV_ERR_HIP, "k_mosaic_grid launch");
+  }
+  const hipError_t fe = hipFreeAsync(norms, s);
+  if (rc) return rc;
+  if (fe != hipSuccess) return hip_fail(fe, "av_mosaic_grid: hipFreeAsync");
   return reduce_sumwt(d, NPOL, sumwt, s);
 }
 
@@ -246,7 +320,8 @@ using namespace av;
 extern "C" int av_mosaic_grid(av_plan *plan, const void *vis, const void *weight, int32_t n_pol,
                               int32_t n_baseline, const int32_t *cf_baseline_map,
                               const int32_t *cf_chan_map, const int32_t *cf_pol_map,
-                              const void *cfpg, int32_t ncc, int32_t ncp, int32_t cu_n,
+                              const void *cfpg, int32_t ncb, int32_t ncc, int32_t ncp,
+                              int32_t cu_n,
                               int32_t cv_n, const int32_t *weight_support, int32_t ov_u,
                               int32_t ov_v, void *grid, double *sumwt, int32_t dtype,
                               int32_t mode, void *stream) {
@@ -265,19 +340,20 @@ extern "C" int av_mosaic_grid(av_plan *plan, const void *vis, const void *weight
                                     "without row channels");
   if (n_baseline < 1 || d.n_rows % n_baseline != 0)
     return fail(AV_ERR_INVALID, "av_mosaic_grid: n_baseline must divide the plan's rows");
-  if (ncc < 1 || ncp < 1 || cu_n < 1 || cv_n < 1 || ov_u < 1 || ov_v < 1)
+  if (ncb < 1 || ncc < 1 || ncp < 1 || cu_n < 1 || cv_n < 1 || ov_u < 1 || ov_v < 1)
     return fail(AV_ERR_INVALID, "av_mosaic_grid: bad convolution-function shape");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == AV_F32) {
     MosaicArgs<float> A{(const float *)vis, (const float *)weight, cf_baseline_map, cf_chan_map,
-                        cf_pol_map, (const float *)cfpg, weight_support, n_baseline, ncc, ncp,
-                        cu_n, cv_n, ov_u, ov_v, (float *)grid};
+                        cf_pol_map, (const float *)cfpg, weight_support, n_baseline, ncb, ncc,
+                        ncp, cu_n, cv_n, ov_u, ov_v, (float *)grid, nullptr, 0, 0, 0, 0};
     return mosaic_by_pol<float>(plan, A, n_pol, sumwt, mode, s);
   }
   if (dtype == AV_F64) {
     MosaicArgs<double> A{(const double *)vis, (const double *)weight, cf_baseline_map,
                          cf_chan_map, cf_pol_map, (const double *)cfpg, weight_support,
-                         n_baseline, ncc, ncp, cu_n, cv_n, ov_u, ov_v, (double *)grid};
+                         n_baseline, ncb, ncc, ncp, cu_n, cv_n, ov_u, ov_v, (double *)grid,
+                         nullptr, 0, 0, 0, 0};
     return mosaic_by_pol<double>(plan, A, n_pol, sumwt, mode, s);
   }
   return fail(AV_ERR_INVALID, "av_mosaic_grid: dtype must be AV_F32 or AV_F64");

@@ -81,7 +81,7 @@ def mosaic_grid_jit(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map
         mode = AV_GRID_ACCUMULATE | (AV_GRID_PSF if do_psf else 0)
         p = _dev._ptr
         call("av_mosaic_grid", plan._h, p(v_dev), p(w_dev), int(npol), int(nb), p(keep[0]),
-             p(keep[1]), p(keep[2]), p(cfpg), int(ncc), int(ncp), int(cu_n), int(cv_n),
+             p(keep[1]), p(keep[2]), p(cfpg), int(ncb), int(ncc), int(ncp), int(cu_n), int(cv_n),
              p(keep[3]), ov[0], ov[1], p(g_dev), p(s_dev), AV_F32 if fp32 else AV_F64, mode,
              _dev._stream())
         grid[...] = g_dev.cpu().numpy()

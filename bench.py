@@ -569,7 +569,7 @@ def row_mosaic(dev, n_time=100, cpu_s=5.0):
 
         def call():
             _lib.call("av_mosaic_grid", plan._h, D._ptr(vis_), D._ptr(w_), npol, nb,
-                      D._ptr(keep[0]), D._ptr(keep[1]), D._ptr(keep[2]), D._ptr(cfpg), ncc, ncp,
+                      D._ptr(keep[0]), D._ptr(keep[1]), D._ptr(keep[2]), D._ptr(cfpg), ncb, ncc, ncp,
                       cu_n, cv_n, D._ptr(keep[3]), ov[0], ov[1], D._ptr(g), D._ptr(s),
                       _lib.AV_F32, 0, D._stream())
         return plan, g, s, call

@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 8
+#define AV_ABI_VERSION 9
 
 const char *av_last_error(void);
 int av_version(void);
@@ -205,7 +205,7 @@ int av_grid_noalloc(const double *uvw, int64_t n_rows, const double *freq,
  * AV_GEOM_REFERENCE, support = 2 * max(weight_support) + 1, so that its
  * bounds test is the reference's, and tiles at least that wide) is
  * convolved with the complex convolution function cfpg[cfb][cfc][cfp]
- * [cu_n][cv_n] (conv_kernel * phase_gradient[0], mosaic_grid.py:359) of
+ * [cu_n][cv_n] (cfpg holds ncb x ncc x ncp of them) (conv_kernel * phase_gradient[0], mosaic_grid.py:359) of
  * cfb = cf_baseline_map[row % n_baseline], cfc = cf_chan_map[chan],
  * cfp = cf_pol_map[pol], over weight_support[cfb][cfc][cfp][2] taps,
  * oversampled by ov_u, ov_v.  vis [rows][n_chan][n_pol] complex (ignored
@@ -213,12 +213,15 @@ int av_grid_noalloc(const double *uvw, int64_t n_rows, const double *freq,
  * [n_img_chan][n_pol][n_u][n_v] complex, sumwt [n_img_chan][n_pol] FP64:
  * weight * Re(norm) with AV_GRID_PSF, weight * Re(norm^2) without.  mode:
  * AV_GRID_PSF, AV_GRID_ACCUMULATE.  cf_*_map, weight_support int32 on the
- * device; cfpg complex64 (AV_F32) or complex128 (AV_F64). */
+ * device; cfpg complex64 (AV_F32) or complex128 (AV_F64).  A sample
+ * polarisation whose convolution function lies outside cfpg is skipped.  The
+ * call allocates a small stream-ordered table of kernel sums (ncb * ncc *
+ * ncp * (ov_u + 3) * (ov_v + 3) * 16 bytes) and frees it on the stream. */
 int av_mosaic_grid(av_plan *plan, const void *vis, const void *weight, int32_t n_pol,
                    int32_t n_baseline, const int32_t *cf_baseline_map,
                    const int32_t *cf_chan_map, const int32_t *cf_pol_map,
-                   const void *cfpg, int32_t ncc, int32_t ncp, int32_t cu_n,
-                   int32_t cv_n, const int32_t *weight_support, int32_t ov_u,
+                   const void *cfpg, int32_t ncb, int32_t ncc, int32_t ncp,
+                   int32_t cu_n, int32_t cv_n, const int32_t *weight_support, int32_t ov_u,
                    int32_t ov_v, void *grid, double *sumwt, int32_t dtype,
                    int32_t mode, void *stream);
 
