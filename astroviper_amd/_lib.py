@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 9  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 10  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -78,10 +78,10 @@ SIGNATURES = {
     "av_hogbom_clean": (_c.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
                                    _I32, _I32, _I32, _I32, _D, _D, _D, _I32, _I32, _I32, _P,
                                    _P]),
-    "av_grid_imaging_weights": (_c.c_int, [_P, _I64, _P, _I32, _P, _I32, _I32, _I32, _D, _D,
-                                           _P, _P, _P]),
-    "av_degrid_imaging_weights": (_c.c_int, [_P, _P, _P, _I64, _P, _I32, _P, _I32, _I32, _D,
-                                             _D, _P, _P, _P]),
+    "av_grid_imaging_weights": (_c.c_int, [_P, _I64, _I32, _P, _I32, _P, _I32, _I32, _I32, _D,
+                                           _D, _P, _P, _P]),
+    "av_degrid_imaging_weights": (_c.c_int, [_P, _P, _P, _I64, _I32, _P, _I32, _P, _I32, _I32,
+                                             _D, _D, _P, _P, _P]),
     "av_gcf_aperture_patterns": (_c.c_int, [_P, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _D,
                                             _D, _I32, _I32, _P, _P]),
     "av_gcf_phase_gradient": (_c.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _P]),

@@ -37,7 +37,7 @@ def grid_imaging_weights(grid: np.ndarray, sum_weight: np.ndarray, uvw: np.ndarr
     u = _t(np.asarray(uvw).reshape(nt * nb, 3), dev)
     w = _t(data_weight, dev)
     f = _t(freq_chan, dev)
-    _lib.call("av_grid_imaging_weights", _dev._ptr(u), nt * nb, _dev._ptr(f), nchan,
+    _lib.call("av_grid_imaging_weights", _dev._ptr(u), nt * nb, nb, _dev._ptr(f), nchan,
               _dev._ptr(w), npol, n_u, n_v, float(cell[0]), float(cell[1]), _dev._ptr(g),
               _dev._ptr(sw), _dev._stream())
     grid[:, 0] = g.cpu().numpy()
@@ -63,7 +63,7 @@ def degrid_imaging_weights(grid_imaging_weight, uvw, data_weight, briggs_factors
     f = _t(freq_chan, dev)
     out = torch.empty((nt * nb, nchan), dtype=torch.float64, device=dev)
     flag = torch.zeros(1, dtype=torch.int32, device=dev)
-    _lib.call("av_degrid_imaging_weights", _dev._ptr(g), _dev._ptr(b), _dev._ptr(u), nt * nb,
+    _lib.call("av_degrid_imaging_weights", _dev._ptr(g), _dev._ptr(b), _dev._ptr(u), nt * nb, nb,
               _dev._ptr(f), nchan, _dev._ptr(w), n_u, n_v, float(cell[0]), float(cell[1]),
               _dev._ptr(out), _dev._ptr(flag), _dev._stream())
     if int(flag.item()):

@@ -408,6 +408,7 @@ def row_weights(dev, obs, uvw, w, flag, n_uv, cell, robust=0.5, cpu_s=5.0):
 
     rows, nchan, npol = w.shape
     S = rows * nchan
+    nbl = obs.uvw.shape[1]  # rows are time-major, nbl baselines per time
     w1 = ((w[..., 0].double() + w[..., npol - 1].double()) / 2)
     w1 = torch.where(flag[..., 0].bool(), torch.full_like(w1, float("nan")), w1).contiguous()
     freq = torch.from_numpy(obs.freq).to(dev)
@@ -421,7 +422,7 @@ def row_weights(dev, obs, uvw, w, flag, n_uv, cell, robust=0.5, cpu_s=5.0):
     def grid_w():
         grid.zero_()
         sw.zero_()
-        _lib.call("av_grid_imaging_weights", _ptr(uvw), rows, _ptr(freq), nchan, _ptr(w1), 1,
+        _lib.call("av_grid_imaging_weights", _ptr(uvw), rows, nbl, _ptr(freq), nchan, _ptr(w1), 1,
                   n_u, n_v, float(cell[0]), float(cell[1]), _ptr(grid), _ptr(sw), _stream())
 
     def factors():
@@ -430,7 +431,7 @@ def row_weights(dev, obs, uvw, w, flag, n_uv, cell, robust=0.5, cpu_s=5.0):
         bf[1] = 1.0
 
     def degrid_w():
-        _lib.call("av_degrid_imaging_weights", _ptr(grid), _ptr(bf), _ptr(uvw), rows,
+        _lib.call("av_degrid_imaging_weights", _ptr(grid), _ptr(bf), _ptr(uvw), rows, nbl,
                   _ptr(freq), nchan, _ptr(w1), n_u, n_v, float(cell[0]), float(cell[1]),
                   _ptr(iw), _ptr(nanf), _stream())
 
@@ -474,11 +475,11 @@ def row_weights(dev, obs, uvw, w, flag, n_uv, cell, robust=0.5, cpu_s=5.0):
     uvw_s, w1_s = uvw[:r_s].contiguous(), w1[:r_s].contiguous()
     g_s = torch.zeros((nchan, n_u, n_v), dtype=torch.float64, device=dev)
     s_s = torch.zeros(nchan, dtype=torch.float64, device=dev)
-    _lib.call("av_grid_imaging_weights", _ptr(uvw_s), r_s, _ptr(freq), nchan, _ptr(w1_s), 1,
+    _lib.call("av_grid_imaging_weights", _ptr(uvw_s), r_s, nbl, _ptr(freq), nchan, _ptr(w1_s), 1,
               n_u, n_v, float(cell[0]), float(cell[1]), _ptr(g_s), _ptr(s_s), _stream())
     bf_s = torch.from_numpy(bf_h).to(dev).contiguous()
     iw_s = torch.empty((r_s, nchan), dtype=torch.float64, device=dev)
-    _lib.call("av_degrid_imaging_weights", _ptr(g_s), _ptr(bf_s), _ptr(uvw_s), r_s, _ptr(freq),
+    _lib.call("av_degrid_imaging_weights", _ptr(g_s), _ptr(bf_s), _ptr(uvw_s), r_s, nbl, _ptr(freq),
               nchan, _ptr(w1_s), n_u, n_v, float(cell[0]), float(cell[1]), _ptr(iw_s),
               _ptr(nanf), _stream())
 

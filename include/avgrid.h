@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 9
+#define AV_ABI_VERSION 10
 
 const char *av_last_error(void);
 int av_version(void);
@@ -314,7 +314,10 @@ int av_hogbom_clean(void *model, void *residual, const void *psf,
 /* ---------------------------------------------------- imaging weights --
  * core/imaging/imaging_weighting/grid_imaging_weights.py.  uvw [n_rows][3]
  * float64, freq [n_chan], cell in radians; a sample's pixel is int(x + 0.5)
- * of its grid position x, truncated toward zero. */
+ * of its grid position x, truncated toward zero.  Rows are time-major with
+ * n_baseline baselines per time step (n_baseline divides n_rows; 1 serves
+ * any row order): samples are taken baseline by baseline, so that
+ * neighbouring threads hold consecutive times of one baseline. */
 
 /* grid_imaging_weights_jit (:105-206): for every sample whose pixel lies
  * inside the grid and whose weight (weight[(row * n_chan + chan) *
@@ -323,7 +326,7 @@ int av_hogbom_clean(void *model, void *residual, const void *psf,
  * reference writes one past the end there) of grid [n_chan][n_u][n_v]
  * float64, and 2 * weight to sum_weight [n_chan].  Accumulates. */
 int av_grid_imaging_weights(const double *uvw, int64_t n_rows,
-                            const double *freq, int32_t n_chan,
+                            int32_t n_baseline, const double *freq, int32_t n_chan,
                             const double *weight, int32_t weight_stride,
                             int32_t n_u, int32_t n_v, double cell_u,
                             double cell_v, double *grid, double *sum_weight,
@@ -337,7 +340,8 @@ int av_grid_imaging_weights(const double *uvw, int64_t n_rows,
  * non-zero when a Briggs factor used was NaN (the reference raises). */
 int av_degrid_imaging_weights(const double *grid, const double *briggs,
                               const double *uvw, int64_t n_rows,
-                              const double *freq, int32_t n_chan,
+                              int32_t n_baseline, const double *freq,
+                              int32_t n_chan,
                               const double *weight, int32_t n_u, int32_t n_v,
                               double cell_u, double cell_v,
                               double *imaging_weight, int32_t *nan_flag,
