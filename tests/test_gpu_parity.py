@@ -134,12 +134,11 @@ def test_ps_tables_match_oracle(gpu, oracle):
 
 # --------------------------------------------------------------- gridding
 @pytest.mark.parametrize("cube", [False, True])
-@pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("dense", [False, True])
-@pytest.mark.parametrize("tile,f32p", [((32, 32), False), ((16, 16), False), ((16, 16), True)])
+@pytest.mark.parametrize("prec,tile,f32p", [(64, (32, 32), False), (64, (16, 16), False),
+                                            (32, (32, 32), False), (32, (16, 16), False),
+                                            (32, (16, 16), True)])  # FP32 products: FP32 only
 def test_grid_c1_matches_oracle(gpu, oracle, cube, prec, dense, tile, f32p):
-    if f32p and prec == 64:
-        pytest.skip("FP32 products are an FP32 option")
     uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=dense, flag_frac=0.1,
                                                   weights=(0.5, 1.5))
     vis, w = as_prec(prec, vis, w)
