@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 10  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 11  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -57,6 +57,7 @@ SIGNATURES = {
     "av_plan_destroy": (_c.c_int, [_P]),
     "av_plan_set_uvw": (_c.c_int, [_P, _P, _P, _P, _P, _P]),
     "av_plan_set_degrid_transform": (_c.c_int, [_P, _D, _D, _D, _D]),
+    "av_plan_set_row_order": (_c.c_int, [_P, _I32]),
     "av_plan_stats": (_c.c_int, [_P, _P, _P]),
     "av_grid": (_c.c_int, [_P, _P, _P, _P, _I32, _P, _P, _P, _I32, _I32, _P]),
     "av_degrid": (_c.c_int, [_P, _P, _I32, _P, _P, _I32, _P, _P, _I32, _I32, _P]),

@@ -177,6 +177,7 @@ struct PlanDev {
   uint32_t *chunk_hist;             // [n_chan * chunks_per_chan][ntu * ntv]
   uint32_t *chunk_dup;              // [n_chan * chunks_per_chan][ntu * ntv]
   int32_t chunk_scatter;            // records written through per-chunk LDS cursors
+  int32_t order_nb;                 // baselines per time step of time-major rows (1: row order)
   // row-block binning (rows carry their own channel): kRowBlockRows rows per
   // workgroup, per-tile counters and cursors in LDS; the counting pass keeps
   // each block's own-channel counts (prim | dup << 16) for the scatter pass

@@ -349,11 +349,35 @@ __global__ void k_chunk_bases(PlanDev P) {
   }
 }
 
+// Slots for one record per lane from the LDS cursors `cur` (key: the tile,
+// or -1 for no record).  A lane whose key equals its left neighbour's joins
+// that lane's run; the first lane of a run takes the whole run's slots with
+// one LDS atomic, and the run's lanes get consecutive slots in lane order,
+// so their record stores are contiguous.  Called by all 64 lanes together.
+__device__ inline uint32_t run_slots(uint32_t *cur, int key) {
+  if (__ballot(key >= 0) == 0ull) return 0u;
+  const int lane = (int)__lane_id();
+  const int left = __shfl_up(key, 1u);
+  const bool start = lane == 0 || key != left;
+  const unsigned long long starts = __ballot(start);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int first = 63 - __clzll((long long)(starts & upto));
+  const unsigned long long after = starts & ~upto;
+  const int end = after ? __ffsll((long long)after) - 1 : 64;
+  uint32_t base = 0u;
+  if (start && key >= 0) base = atomicAdd(&cur[key], (uint32_t)(end - lane));
+  return (uint32_t)__shfl((int)base, first) + (uint32_t)(lane - first);
+}
+
 // Chunked pass 4: a workgroup per chunk writes the chunk's records.  Its
-// runs' first slots are loaded into LDS cursors, and every record takes its
-// slot from an LDS atomic: no global atomics.  (Numbering the chunks so that
-// 16, 32 or 64 consecutive chunks of a channel share an XCD's L2 measured
-// the same as plain numbering.)
+// runs' first slots are loaded into LDS cursors, and the records take their
+// slots from LDS atomics, one per run of equal tiles in a wavefront: no
+// global atomics.  When the rows are time-major with order_nb baselines per
+// time step, the chunk's rows are visited baseline by baseline: a
+// wavefront then holds consecutive times of one baseline, which mostly fall
+// in the same tile, so its stores to a bin are contiguous.  (Numbering the
+// chunks so that 16, 32 or 64 consecutive chunks of a channel share an
+// XCD's L2 measured the same as plain numbering.)
 __global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P) {
   extern __shared__ uint32_t lds[];
   const int ntiles = P.ntu * P.ntv;
@@ -368,25 +392,45 @@ __global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P) {
     cd[i] = bd[i];
   }
   __syncthreads();
-  for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+  // positions p = b * nt + t over the nt time steps the chunk touches, row
+  // (t_lo + t) * nb + b; rows outside [r0, r1) are skipped (nb = 1: p = row - r0)
+  const int64_t nb = P.order_nb;
+  const int64_t t_lo = r0 / nb;
+  const uint32_t nt = (uint32_t)((r1 - 1) / nb - t_lo + 1);
+  const uint32_t n_pos = nt * (uint32_t)nb;
+  const uint64_t nt_mul = ((1ull << 32) + nt - 1) / nt;  // floor(p * nt_mul / 2^32) - p / nt in {0, 1}
+  const uint32_t lane = __lane_id();
+  for (uint32_t p0 = threadIdx.x - lane; p0 < n_pos; p0 += blockDim.x) {
+    const uint32_t p = p0 + lane;
+    int64_t row = -1;
+    if (p < n_pos) {
+      uint32_t b = (uint32_t)(((uint64_t)p * nt_mul) >> 32);
+      if (b * nt > p) --b;
+      const int64_t r = (t_lo + (int64_t)(p - b * nt)) * nb + b;
+      if (r >= r0 && r < r1) row = r;
+    }
     ItemGeo g;
-    sample_geometry(P, row, chan, g);
-    if (!(g.gvalid || g.dvalid)) continue;
+    g.gvalid = g.dvalid = false;
+    if (row >= 0) sample_geometry(P, row, chan, g);
     const RecPlan rp = record_plan(P, g);
     const int64_t item = row * P.n_chan + chan;
-    if (rp.gprim) {
-      const uint32_t slot = atomicAdd(&cp[rp.tu * P.ntv + rp.tv], 1u);
-      write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
-      for (int a = rp.tu0; a <= rp.tu1; ++a)
-        for (int b = rp.tv0; b <= rp.tv1; ++b)
-          if (a != rp.tu || b != rp.tv) {
-            const uint32_t s2 = atomicAdd(&cd[a * P.ntv + b], 1u);
-            write_records(P, g, rp, item, s2, a, b, false);
-          }
+    const uint32_t slot = run_slots(cp, rp.gprim ? rp.tu * P.ntv + rp.tv : -1);
+    if (rp.gprim) write_records(P, g, rp, item, slot, rp.tu, rp.tv, true);
+    // duplicates: the other tiles of the footprint's (at most) 2 x 2 (a
+    // tile is at least `support` wide), in a fixed order so that
+    // neighbouring lanes' runs line up
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int a = (k & 2) ? rp.tu1 : rp.tu0, b = (k & 1) ? rp.tv1 : rp.tv0;
+      const bool dup = rp.gprim && rp.tu0 <= rp.tu1 && rp.tv0 <= rp.tv1 &&
+                       (!(k & 2) || rp.tu1 != rp.tu0) && (!(k & 1) || rp.tv1 != rp.tv0) &&
+                       (a != rp.tu || b != rp.tv);
+      const uint32_t s2 = run_slots(cd, dup ? a * P.ntv + b : -1);
+      if (dup) write_records(P, g, rp, item, s2, a, b, false);
     }
     if (rp.dsep) {
-      const uint32_t slot = atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
-      write_degrid_record(P, g, rp, item, slot);
+      const uint32_t s3 = atomicAdd(&cp[rp.du_t * P.ntv + rp.dv_t], 1u);
+      write_degrid_record(P, g, rp, item, s3);
     }
   }
 }
@@ -662,6 +706,7 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   // a sample has at most two primary records (grid + separate degrid)
   d.max_dunits = d.n_bins + 2 * n_items / unit_cap + 1;
   d.row_chan = nullptr;
+  d.order_nb = 1;
   d.uvw = nullptr;
 
   const int64_t nb = d.n_bins;
@@ -881,6 +926,17 @@ int av_plan_set_degrid_transform(av_plan *plan, double scale_u, double scale_v,
   plan->d.dscale_v = scale_v;
   plan->d.doff_u = offset_u;
   plan->d.doff_v = offset_v;
+  return AV_OK;
+}
+
+int av_plan_set_row_order(av_plan *plan, int32_t n_baseline) {
+  if (!plan) return fail(AV_ERR_INVALID, "av_plan_set_row_order: NULL plan");
+  PlanDev &d = plan->d;
+  if (n_baseline < 1 || (d.n_rows > 0 && d.n_rows % n_baseline != 0))
+    return fail(AV_ERR_INVALID, "av_plan_set_row_order: n_baseline must be >= 1 and divide n_rows");
+  // a chunk is visited baseline by baseline only when it spans at least 16
+  // time steps; otherwise the runs per bin stay short and row order is kept
+  d.order_nb = (n_baseline > 1 && (int64_t)n_baseline * 16 <= d.rows_per_chunk) ? n_baseline : 1;
   return AV_OK;
 }
 

@@ -120,13 +120,15 @@ class GridPlan:
     ``do_imaging_weight``, standard_grid.py:389-430).  With
     ``row_channels`` every row is one sample at its own channel (``row_chan``
     in :meth:`set_uvw`), and the plan holds n_rows samples instead of
-    n_rows * n_chan.
+    n_rows * n_chan.  ``n_baseline`` > 1 says the rows are time-major with
+    that many baselines per time step (``av_plan_set_row_order``): the
+    binning then writes its records baseline by baseline.
     """
 
     def __init__(self, n_rows: int, n_chan: int, n_img_chan: int, n_uv, cell,
                  support: int = 7, oversampling: int = 100, tile=(16, 16),
                  geometry: str = "reference", unit_cap: int = 0, device=None,
-                 row_channels: bool = False):
+                 row_channels: bool = False, n_baseline: int = 1):
         self.device = require_gpu(device)
         self.n_rows, self.n_chan, self.n_img_chan = int(n_rows), int(n_chan), int(n_img_chan)
         self.n_u, self.n_v = int(n_uv[0]), int(n_uv[1])
@@ -144,6 +146,8 @@ class GridPlan:
                  self.n_v, self.cell[0], self.cell[1], self.support, self.oversampling,
                  self.tile[0], self.tile[1], geo, int(unit_cap), ctypes.byref(h))
         self._h = h
+        if int(n_baseline) > 1:
+            call("av_plan_set_row_order", self._h, int(n_baseline))
         self._keep = ()
         self.row_chan = None
         self.cgk = ps_kernel_1d(self.oversampling, self.support, self.device)

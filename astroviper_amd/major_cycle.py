@@ -66,7 +66,8 @@ class ResidentImager:
         self.freq = np.asarray(ms4.coords["frequency"].values, dtype=np.float64)
         # image arrays are [.., image_size[0], image_size[1]] (imager.py:66-69)
         self.plan = D.GridPlan(rows, nf, self.n_img_chan, (self.ny, self.nx), self.cell,
-                               self.support, self.oversampling, tile=tile, device=self.dev)
+                               self.support, self.oversampling, tile=tile, device=self.dev,
+                               n_baseline=nb)
         self.plan.set_uvw(torch.from_numpy(np.ascontiguousarray(uvw.reshape(rows, 3))),
                           torch.from_numpy(self.freq),
                           torch.from_numpy(cmap.astype(np.int32)))

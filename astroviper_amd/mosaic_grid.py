@@ -63,7 +63,7 @@ def mosaic_grid_jit(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map
         raise ValueError("oversampling must lie in [1, 254]")
     rows = nt * nb
     plan = _dev.GridPlan(rows, nchan, grid.shape[0], n_uv, delta_lm, support,
-                         max(ov[0], 1), tile=(tile, tile), device=dev)
+                         max(ov[0], 1), tile=(tile, tile), device=dev, n_baseline=nb)
     try:
         plan.set_uvw(torch.from_numpy(np.ascontiguousarray(uvw, dtype=np.float64)
                                       .reshape(rows, 3)).to(dev),

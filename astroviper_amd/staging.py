@@ -348,7 +348,8 @@ class StagedGridder:
         self._vdt_h, self._wdt_h = vdt_h, wdt_h
         self._plan = _dev.GridPlan(rows, self.n_chan, self.n_img_chan, self.n_uv, self.cell,
                                    self.support, self.oversampling, tile=self.tile,
-                                   geometry=self.geometry, device=dev)
+                                   geometry=self.geometry, device=dev, n_baseline=nb)
+        self._slot_nb = nb
         self._plan.cgk = self.cgk
         self._freq = None
 
@@ -359,7 +360,7 @@ class StagedGridder:
         nb = part.n_baseline
         t_per = max(1, self.block_rows // max(1, nb))
         rows = t_per * nb
-        if self._slots is None or self._slot_rows != rows:
+        if self._slots is None or self._slot_rows != rows or self._slot_nb != nb:
             self._alloc(rows, nb)
         freq_t = torch.from_numpy(np.ascontiguousarray(part.freq)).to(self.device)
         cur = torch.cuda.current_stream(self.device)
@@ -483,7 +484,7 @@ def main(argv=None) -> None:
         t_staged.append(time.perf_counter() - t0)
     # device-resident: the same samples already in HBM, one plan
     rows = nt * nb
-    plan = _dev.GridPlan(rows, nchan, nchan, n_uv, cell, device=dev)
+    plan = _dev.GridPlan(rows, nchan, nchan, n_uv, cell, device=dev, n_baseline=nb)
     uvw_d = torch.from_numpy(np.ascontiguousarray(obs.uvw.reshape(rows, 3))).to(dev)
     freq_d = torch.from_numpy(obs.freq).to(dev)
     v_d = torch.from_numpy(vis.reshape(-1, npol)).to(dev)

@@ -40,13 +40,14 @@ def _degrid_to_device(uvw, flag, freq, grid, scale, offset, support, sampling, c
     # polarisations mapped outside the grid are skipped by av_degrid itself
     # (standard_degrid.py:419-420)
     pmap_t = torch.from_numpy(np.asarray(polmap).astype(np.int32)).to(dev)
-    step = max(1, _MAX_PLAN_SAMPLES // max(1, nf))
+    # blocks of whole time steps, so that each plan's rows are time-major
+    step = max(1, _MAX_PLAN_SAMPLES // max(1, nf) // max(1, nb)) * max(1, nb)
     for r0 in range(0, rows, step):
         r1 = min(rows, r0 + step)
         # NaN cells: the plan computes no gridding geometry, only the
         # degridder's (with the transform below)
         plan = _dev.GridPlan(r1 - r0, nf, nchan_g, (n_u, n_v), (np.nan, np.nan), support,
-                             sampling, tile=tile, device=dev)
+                             sampling, tile=tile, device=dev, n_baseline=max(1, nb))
         plan.cgk = cgk_dev
         _dev.call("av_plan_set_degrid_transform", plan._h, float(scale[0]), float(scale[1]),
                   float(offset[0]), float(offset[1]))

@@ -96,7 +96,8 @@ def main(argv=None) -> int:
     freq = torch.from_numpy(obs.freq).to(dev)
     c = args.cell_arcsec * synth.ARCSEC
     n_uv = (args.n, args.n)
-    plan = GridPlan(rows, nchan, nchan, n_uv, (-c, c), 7, 100, tile=tuple(args.tile), device=dev)
+    plan = GridPlan(rows, nchan, nchan, n_uv, (-c, c), 7, 100, tile=tuple(args.tile), device=dev,
+                    n_baseline=obs.uvw.shape[1])
     plan.set_uvw(uvw, freq)
     v2, w2, f2 = vis.reshape(-1, npol), w.reshape(-1, npol), flag.reshape(-1, npol)
 

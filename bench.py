@@ -199,7 +199,8 @@ def small_workload_leg(dev, reps=10):
                                    do_psf=True)
     iref, psfref = O.normalise(gref, swref), O.normalise(pref, pswref)
 
-    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev)
+    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev,
+                    n_baseline=nb)
     uvw_t = torch.from_numpy(obs.uvw.reshape(rows, 3)).to(dev)
     f_t = torch.from_numpy(obs.freq).to(dev)
     v = torch.from_numpy(vis.reshape(rows * nchan, npol)).to(dev, torch.complex64)
@@ -662,7 +663,8 @@ def row_major_cycle(dev, reps=3):
     V = rows * nchan * npol
     freq = torch.from_numpy(obs.freq).to(dev)
     vis2, w2, fl2 = (t.reshape(rows * nchan, npol) for t in (vis, w, flag))
-    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev)
+    plan = GridPlan(rows, nchan, nchan, n_uv, cell, 7, 100, tile=(16, 16), device=dev,
+                    n_baseline=obs.uvw.shape[1])
     plan.set_uvw(uvw, freq)
     grid = torch.empty((nchan, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
     sumwt = torch.empty((nchan, npol), dtype=torch.float64, device=dev)
@@ -872,6 +874,8 @@ def main():
     ap.add_argument("--f32-products", action="store_true",
                     help="grid and degrid FP32 16x16 tiles with FP32 products (16x16x4 "
                          "kernels) instead of split-FP16 operands")
+    ap.add_argument("--row-order-off", action="store_true",
+                    help="bin the rows in row order instead of baseline by baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -918,7 +922,8 @@ def main():
     w2 = w.reshape(rows * nchan, npol)
     fl2 = flag.reshape(rows * nchan, npol)
     plan = GridPlan(rows, nchan, n_img, n_uv, cell, 7, 100, tile=tuple(args.tile),
-                    unit_cap=args.unit_cap, device=dev)
+                    unit_cap=args.unit_cap, device=dev,
+                    n_baseline=1 if args.row_order_off else obs.uvw.shape[1])
     grid = torch.empty((n_img, npol) + tuple(n_uv), dtype=torch.complex64, device=dev)
     sumwt = torch.empty((n_img, npol), dtype=torch.float64, device=dev)
     model = torch.empty((rows * nchan, npol), dtype=torch.complex64, device=dev)
@@ -1081,6 +1086,7 @@ def main():
                                f"10% flagged",
                    "mode": args.mode, "tile": list(args.tile), "visibilities": V,
                    "compensated": bool(args.compensated), "subtile": bool(args.subtile),
+                   "row_order": "row" if args.row_order_off else "baseline",
                    "grid_kernel": names["grid"], "degrid_kernel": names["degrid"]},
         "roofline": roofline,
         "cpu_baseline": cpu,

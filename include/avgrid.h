@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 10
+#define AV_ABI_VERSION 11
 
 const char *av_last_error(void);
 int av_version(void);
@@ -147,6 +147,15 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
  * Takes effect at the next av_plan_set_uvw. */
 int av_plan_set_degrid_transform(av_plan *plan, double scale_u, double scale_v,
                                  double offset_u, double offset_v);
+
+/* Rows are time-major with n_baseline baselines per time step, as the
+ * reference's [time][baseline] loops visit them (standard_grid.py:310-319,
+ * standard_degrid.py:416-417); 1 (the default) = no known order.  n_baseline
+ * must divide n_rows.  The binning then writes each chunk's records baseline
+ * by baseline, which makes its record stores contiguous; the records of a
+ * bin are the same, in another order within a chunk.  Takes effect at the
+ * next av_plan_set_uvw. */
+int av_plan_set_row_order(av_plan *plan, int32_t n_baseline);
 
 /* Counts of the last av_plan_set_uvw (synchronises `stream`):
  * out[0] samples, out[1] grid-valid, out[2] degrid-valid, out[3] records,
