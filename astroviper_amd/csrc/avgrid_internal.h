@@ -164,6 +164,14 @@ struct PlanDev {
   uint32_t *rec;                    // [max_records][3]
   // work units: x = bin, y = first record, z = count, w = flags
   uint4 *gunits, *dunits;
+  // pair units of the split-FP16 degridder (16 x 16 tiles, support <= 7):
+  // tiles 2j and 2j + 1 along v of one image channel share a unit, pair p =
+  // (tile row, j, image channel) channel-minor; a unit is {bin of tile 2j,
+  // first index into the pair's primary records (those of tile 2j, then
+  // those of tile 2j + 1), count, bin of tile 2j + 1 or kNoBin}
+  int64_t n_pairs, max_punits;      // 0 when the plan has no pair units
+  uint32_t *punit_start;            // [n_pairs + 1]
+  uint4 *punits;                    // [max_punits]
   // validity bitmasks, one bit per sample: grid-valid, degrid-valid; bit
   // (chan, row) is bit row % 64 of word chan * mask_words_per_chan + row / 64
   // (chan 0 when rows carry their own channel)
@@ -193,6 +201,7 @@ struct PlanDev {
   uint32_t *counters;               // scratch for statistics
 };
 
+constexpr uint32_t kNoBin = 0xFFFFFFFFu;  // a pair unit's missing second tile
 constexpr uint32_t kUnitMulti = 1u;  // bin split over several units
 constexpr uint32_t kUnitFirst = 2u;  // first unit of its bin
 

@@ -773,15 +773,19 @@ struct DegridH3Lds {
 
 __host__ __device__ constexpr int h3_swz(int n) { return n ^ ((n >> 3) & 7); }
 
-// per-(wave, key) counts of a sort pass, then the three pass totals
-constexpr int kH3CntWords = 16;
+// per-(wave, key) counts of a sort pass, then the pass's slot count, window
+// starts and padding runs
+constexpr int kH3CntWords = 32;
+// slots a sort pass may add to its records: up to 15 of padding before each
+// window after the first
+constexpr int kH3Pad = 48;
 
-template <int NPOL, int NB, int NBW>
+template <int NPOL, int NB, int NBW, bool PAIR>
 __host__ __device__ inline DegridH3Lds degrid_h3_layout(int ov, int support, int tu, int tv) {
   DegridH3Lds L;
   L.halo = support / 2 + 1;
   L.ru = tu + 2 * L.halo;
-  L.rv = tv + 2 * L.halo;
+  L.rv = (PAIR ? 2 * tv : tv) + 2 * L.halo;
   L.rvp = NB * 16 / (2 * NPOL);
   L.nbn = NB * 16;
   int o = 0;
@@ -793,9 +797,9 @@ __host__ __device__ inline DegridH3Lds degrid_h3_layout(int ov, int support, int
   L.ahi = take(3 * L.nbn * 16);
   L.alo = take(3 * L.nbn * 16);
   L.tab = take(WTab(ov, support).size() * 4);
-  L.sgeo = take((kSortCap + 16) * 4);
-  L.sitem = take((kSortCap + 16) * 4);
-  L.sflag = take((kSortCap + 16) * 4);
+  L.sgeo = take((kSortCap + kH3Pad) * 4);
+  L.sitem = take((kSortCap + kH3Pad) * 4);
+  L.sflag = take((kSortCap + kH3Pad) * 4);
   L.scnt = take(kH3CntWords * 4);
   L.red = take(kDegridWaves * 4);
   L.total = o;
@@ -811,8 +815,19 @@ __host__ __device__ constexpr bool degrid_h3_fits(int tu, int tv, int support) {
          3 * (NB * 16 / (2 * NPOL)) * NPOL <= 2 * kDegridThreads;
 }
 
+// Pair units (four polarisations): the 32 + 2 halo columns of two tiles fit
+// the 20 column blocks, and the four 16-column windows starting at columns
+// 0, 8, 16 and 24 hold every footprint: a degrid centre lies in pair columns
+// -1 .. 32 and a record takes window min(max(av, 0) / 8, 3), whose columns
+// cover its footprint av + 1 .. av + S when S <= 7.
 template <int NPOL, int NB, int NBW>
-__global__ __launch_bounds__(kDegridThreads, 6)
+__host__ __device__ constexpr bool degrid_h3_pair_fits(int tu, int tv, int support) {
+  return NPOL == 4 && NB == 20 && NBW == 8 && degrid_h3_fits<NPOL, NB>(tu, tv, support) &&
+         support <= 7 && 2 * tv + 2 * (support / 2 + 1) <= NB * 16 / (2 * NPOL);
+}
+
+template <int NPOL, int NB, int NBW, bool PAIR>
+__global__ __launch_bounds__(kDegridThreads, PAIR ? 4 : 6)
 void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
                  const int32_t *__restrict__ pol_map, const uint8_t *__restrict__ flag,
                  float *__restrict__ vis_out, int mode) {
@@ -820,20 +835,28 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
   constexpr int C = 2 * NPOL;
   constexpr int CPB = 16 / C;  // cells per 16-entry block
   constexpr bool kHalves = NBW < NB;
-  constexpr int KB1 = NB - NBW;  // first block of the second half's window
+  // windows of NBW blocks, KSTEP blocks (8 cells) apart: two halves of one
+  // tile, or four quarters of a pair of tiles
+  constexpr int NWIN = PAIR ? 4 : (kHalves ? 2 : 1);
+  constexpr int KSTEP = 4;
   constexpr int HK = kSortCap / kDegridThreads;
-  // the second window starts on a multiple of 4 blocks, so a block's swizzle
-  // depends only on its index within the window (mod 4)
-  static_assert(!kHalves || KB1 % 4 == 0, "window start");
-  constexpr int nkeys = kHalves ? 3 : 2;
+  // windows start on multiples of 4 blocks, so a block's swizzle depends only
+  // on its index within the window (mod 4)
+  static_assert(!kHalves || (NPOL == 4 && (NWIN - 1) * KSTEP + NBW == NB), "windows");
+  static_assert(!PAIR || kHalves, "pair units use windows");
+  constexpr int nkeys = NWIN + 1;  // the windows, then records without geometry
   constexpr int NVW = HK * kDegridWaves;  // (record slot, wave) pairs of a pass
-  static_assert(NVW * nkeys + 3 <= kH3CntWords, "sort counts");
+  // per-(slot, wave, key) bases, then the slot count, the window starts and
+  // the padding runs (start, length) before windows 1 .. NWIN - 1
+  constexpr int XC = NVW * nkeys;
+  static_assert(XC + 1 + 3 * (NWIN - 1) <= kH3CntWords, "sort counts");
+  static_assert(16 * (NWIN - 1) <= kH3Pad, "padding slots");
   extern __shared__ __align__(16) char smem[];
   const uint32_t u = xcd_grouped_block(blockIdx.x, kDegridXcdGroup);
-  if (u >= P.dunit_start[P.n_bins]) return;
+  if (u >= (PAIR ? P.punit_start[P.n_pairs] : P.dunit_start[P.n_bins])) return;
   const int S = P.support, sc = S / 2, ov = P.ov;
   const int Tu = P.tile_u, Tv = P.tile_v;
-  const DegridH3Lds L = degrid_h3_layout<NPOL, NB, NBW>(ov, S, Tu, Tv);
+  const DegridH3Lds L = degrid_h3_layout<NPOL, NB, NBW, PAIR>(ov, S, Tu, Tv);
   float *tab = (float *)(smem + L.tab);
   _Float16 *ahi = (_Float16 *)(smem + L.ahi), *alo = (_Float16 *)(smem + L.alo);
   uint32_t *s_geo = (uint32_t *)(smem + L.sgeo);
@@ -849,14 +872,24 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
   const int li = lane & 15, lk = lane >> 4;
   const int half_v = Tv / 2;
 
-  const uint4 unit = P.dunits[u];
+  const uint4 unit = PAIR ? P.punits[u] : P.dunits[u];
   const int64_t b = unit.x;
+  // the unit's records by index: record slots directly, or for a pair unit
+  // indices into the primary records of tile 2j (n0 of them, from slot s0)
+  // and then those of tile 2j + 1 (from slot s1), whose degrid centres are
+  // shifted by Tv columns into the pair's region
   const uint32_t first = unit.y, end = unit.y + unit.z;
+  uint32_t s0 = 0, n0 = 0, s1 = 0;
+  if constexpr (PAIR) {
+    s0 = P.bin_start[b];
+    n0 = P.cnt_prim[b];
+    s1 = unit.w != kNoBin ? P.bin_start[unit.w] : 0u;
+  }
   auto key_of = [&](uint32_t g) {
     if (g == kInvalidGeo) return nkeys - 1;
-    if constexpr (kHalves) {
+    if constexpr (NWIN > 1) {
       const int av = (int)((g >> 16) & 0xFFu) - kPosBias;
-      return av >= half_v ? 1 : 0;
+      return min(max(av, 0) / half_v, NWIN - 1);
     }
     return 0;
   };
@@ -867,15 +900,18 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
     if (NPOL == 2) return *(const uint16_t *)(flag + (size_t)item * 2);
     return flag[item];
   };
+  auto load_record = [&](uint32_t i, uint32_t &geo, uint32_t &item, uint32_t &fb) {
+    const uint32_t slot = PAIR ? (i < n0 ? s0 + i : s1 + (i - n0)) : i;
+    const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)slot);
+    geo = rw.z;
+    if (PAIR && i >= n0 && geo != kInvalidGeo) geo += (uint32_t)Tv << 16;
+    item = rw.x;
+    fb = load_flags(rw.x);
+  };
   uint32_t pg[HK], pi[HK], pf[HK];
 #pragma unroll
-  for (int k = 0; k < HK; ++k) {
-    const uint32_t r = min(first + tid + k * kDegridThreads, end - 1);
-    const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
-    pg[k] = rw.z;
-    pi[k] = rw.x;
-    pf[k] = load_flags(rw.x);
-  }
+  for (int k = 0; k < HK; ++k)
+    load_record(min(first + tid + k * kDegridThreads, end - 1), pg[k], pi[k], pf[k]);
 
   const WTab W(ov, S);
   {
@@ -963,15 +999,10 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
     const int n = (int)min((uint32_t)kSortCap, end - c0);
     if (c0 != first) {  // later passes of a large unit
 #pragma unroll
-      for (int k = 0; k < HK; ++k) {
-        const uint32_t r = c0 + min(tid + k * kDegridThreads, n - 1);
-        const RecWords rw = *(const RecWords *)(P.rec + 3 * (size_t)r);
-        pg[k] = rw.z;
-        pi[k] = rw.x;
-        pf[k] = load_flags(rw.x);
-      }
+      for (int k = 0; k < HK; ++k)
+        load_record(c0 + min(tid + k * kDegridThreads, n - 1), pg[k], pi[k], pf[k]);
     }
-    // ---- stable counting sort of the pass's records by v-half, in LDS: each
+    // ---- stable counting sort of the pass's records by window, in LDS: each
     // wave counts its keys by ballot, one thread turns the (slot, wave, key)
     // counts into bases, and a record's slot is its base plus the number of
     // lower lanes of its wave with the same key (no LDS atomics)
@@ -990,37 +1021,47 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
     }
     __syncthreads();
     if (tid == 0) {
-      // key 0's records, then key 1's from the group boundary b1 (slots [h0,
-      // b1) hold no record), then key 2's; within a key in pass order
-      uint32_t cnt[NVW * nkeys];
+      // the windows' records in window order, each window from a group
+      // boundary (the slots of the padding before it hold no record), then
+      // the records without geometry; within a key in pass order
+      uint32_t cnt[XC];
 #pragma unroll
-      for (int i = 0; i < NVW * nkeys; ++i) cnt[i] = s_cnt[i];
-      uint32_t h0 = 0;
-#pragma unroll
-      for (int w = 0; w < NVW; ++w) h0 += cnt[w * nkeys];
-      const uint32_t b1 = kHalves ? ((h0 + 15u) & ~15u) : h0;
+      for (int i = 0; i < XC; ++i) cnt[i] = s_cnt[i];
       uint32_t run = 0;
 #pragma unroll
       for (int kk = 0; kk < nkeys; ++kk) {
-        if (kk == 1) run += b1 - h0;
+        if (kk >= 1 && kk < NWIN) {
+          const uint32_t r16 = (run + 15u) & ~15u;
+          s_cnt[XC + kk] = r16;                        // window kk's first slot
+          s_cnt[XC + NWIN + 2 * (kk - 1)] = run;       // its padding
+          s_cnt[XC + NWIN + 2 * (kk - 1) + 1] = r16 - run;
+          run = r16;
+        }
 #pragma unroll
         for (int w = 0; w < NVW; ++w) {
           s_cnt[w * nkeys + kk] = run;
           run += cnt[w * nkeys + kk];
         }
       }
-      s_cnt[NVW * nkeys] = b1;
-      s_cnt[NVW * nkeys + 1] = (uint32_t)n + (b1 - h0);
-      s_cnt[NVW * nkeys + 2] = h0;
+      s_cnt[XC] = run;  // slots of the pass, padding included
     }
     __syncthreads();
-    const int b1 = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys]);
-    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys + 1]);
-    const int h0 = __builtin_amdgcn_readfirstlane((int)s_cnt[NVW * nkeys + 2]);
-    if (tid < b1 - h0) {
-      s_geo[h0 + tid] = kInvalidGeo;
-      s_item[h0 + tid] = 0u;
-      s_flag[h0 + tid] = 0u;
+    const int np = __builtin_amdgcn_readfirstlane((int)s_cnt[XC]);
+    int wstart[NWIN];  // first slot of each window (window 0: slot 0)
+    wstart[0] = 0;
+#pragma unroll
+    for (int kk = 1; kk < NWIN; ++kk)
+      wstart[kk] = __builtin_amdgcn_readfirstlane((int)s_cnt[XC + kk]);
+    if constexpr (NWIN > 1) {
+      if (tid < 16 * (NWIN - 1)) {
+        const int gi = tid >> 4, j = tid & 15;
+        const uint32_t ps = s_cnt[XC + NWIN + 2 * gi], pl = s_cnt[XC + NWIN + 2 * gi + 1];
+        if ((uint32_t)j < pl) {
+          s_geo[ps + j] = kInvalidGeo;
+          s_item[ps + j] = 0u;
+          s_flag[ps + j] = 0u;
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < HK; ++k)
@@ -1044,7 +1085,9 @@ void k_degrid_h3(PlanDev P, const float *__restrict__ grid, int npol_g,
       const int r = g * 16 + li;
       const uint32_t geo = r < np ? s_geo[r] : kInvalidGeo;
       const uint32_t item = r < np ? s_item[r] : 0u;
-      const int kb0 = (kHalves && g * 16 >= b1) ? KB1 : 0;
+      int kb0 = 0;  // first block of the group's window
+#pragma unroll
+      for (int kk = 1; kk < NWIN; ++kk) kb0 += g * 16 >= wstart[kk] ? KSTEP : 0;
       const uint32_t fbits = polbad | (r < np ? s_flag[r] : 0u);
       int au, ou, avv, ovv;
       unpack_geo(geo_or_far(geo), au, ou, avv, ovv);
@@ -1327,27 +1370,33 @@ static int launch_degrid_finish(const PlanDev &d, void *vis_out, int mode, hipSt
   return AV_OK;
 }
 
-// The split-FP16 degridder, when the plan's tiles suit it and its LDS fits.
-template <int NPOL, int NB, int NBW>
+// The split-FP16 degridder, when the plan's tiles suit it and its LDS fits;
+// with PAIR over the plan's pair units.
+template <int NPOL, int NB, int NBW, bool PAIR>
 static int try_degrid_h3(av_plan *plan, const void *grid, int npol_g, const int32_t *pol_map,
                          const uint8_t *flag, const double *cgk, void *vis_out, int mode,
                          hipStream_t s, bool &done) {
   PlanDev &d = plan->d;
   if (!degrid_h3_fits<NPOL, NB>(d.tile_u, d.tile_v, d.support)) return AV_OK;
-  const DegridH3Lds L = degrid_h3_layout<NPOL, NB, NBW>(d.ov, d.support, d.tile_u, d.tile_v);
+  if (PAIR && (d.n_pairs == 0 || !degrid_h3_pair_fits<NPOL, NB, NBW>(d.tile_u, d.tile_v,
+                                                                      d.support)))
+    return AV_OK;
+  const DegridH3Lds L =
+      degrid_h3_layout<NPOL, NB, NBW, PAIR>(d.ov, d.support, d.tile_u, d.tile_v);
   const DeviceProps &prop = device_props();
   if (prop.error) return hip_fail(prop.error, "av_degrid: device properties");
   if ((size_t)L.total > prop.lds_per_block) return AV_OK;
   done = true;
   {
-    const int rc_lds =
-        ensure_lds_limit<k_degrid_h3<NPOL, NB, NBW>>(L.total, "av_degrid: hipFuncSetAttribute");
+    const int rc_lds = ensure_lds_limit<k_degrid_h3<NPOL, NB, NBW, PAIR>>(
+        L.total, "av_degrid: hipFuncSetAttribute");
     if (rc_lds) return rc_lds;
   }
   int rc = prepare_tables(d, cgk, s);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_degrid_h3<NPOL, NB, NBW>),
-                     dim3((unsigned)xcd_grouped_grid(d.max_dunits, kDegridXcdGroup)),
+  hipLaunchKernelGGL((k_degrid_h3<NPOL, NB, NBW, PAIR>),
+                     dim3((unsigned)xcd_grouped_grid(PAIR ? d.max_punits : d.max_dunits,
+                                                     kDegridXcdGroup)),
                      dim3(kDegridThreads), L.total, s, d, (const float *)grid, npol_g, pol_map,
                      flag, (float *)vis_out, mode);
   AV_LAUNCH_CHECK("k_degrid_h3");
@@ -1366,18 +1415,22 @@ static int dispatch_degrid(av_plan *plan, const void *grid, int npol_g, const in
       bool done = false;
       int rc = AV_OK;
       if (NPOL == 4 && nb == 12 && degrid_halves_fit<R, NPOL, 12, 8>(plan->d.tile_u, plan->d.tile_v,
-                                                                     plan->d.support))
-        rc = try_degrid_h3<4, 12, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
-                                     done);
-      else if (nb <= 4)
-        rc = try_degrid_h3<NPOL, 4, 4>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
-                                       done);
+                                                                     plan->d.support)) {
+        // pairs of tiles per unit where the plan has them, else one tile
+        rc = try_degrid_h3<4, 20, 8, true>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode,
+                                           s, done);
+        if (!done && !rc)
+          rc = try_degrid_h3<4, 12, 8, false>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,
+                                              mode, s, done);
+      } else if (nb <= 4)
+        rc = try_degrid_h3<NPOL, 4, 4, false>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,
+                                              mode, s, done);
       else if (nb <= 8)
-        rc = try_degrid_h3<NPOL, 8, 8>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode, s,
-                                       done);
+        rc = try_degrid_h3<NPOL, 8, 8, false>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,
+                                              mode, s, done);
       else if (nb <= 12)
-        rc = try_degrid_h3<NPOL, 12, 12>(plan, grid, npol_g, pol_map, flag, cgk, vis_out, mode,
-                                         s, done);
+        rc = try_degrid_h3<NPOL, 12, 12, false>(plan, grid, npol_g, pol_map, flag, cgk, vis_out,
+                                                mode, s, done);
       if (done || rc) return rc;
     }
   }

@@ -580,6 +580,44 @@ __global__ void k_units(PlanDev P) {
   }
 }
 
+// Pair units: the bins of tiles 2j and 2j + 1 (along v) of pair p, and the
+// primary records of both.
+__device__ inline void pair_bins(const PlanDev &P, int64_t p, int64_t &b0, bool &has1,
+                                 uint32_t &n) {
+  const int ntv2 = (P.ntv + 1) / 2;
+  const int64_t tp = p / P.n_img_chan;
+  const int ic = (int)(p - tp * P.n_img_chan);
+  const int tu = (int)(tp / ntv2), tv = 2 * (int)(tp - (int64_t)tu * ntv2);
+  b0 = ((int64_t)tu * P.ntv + tv) * P.n_img_chan + ic;
+  has1 = tv + 1 < P.ntv;
+  n = P.cnt_prim[b0] + (has1 ? P.cnt_prim[b0 + P.n_img_chan] : 0u);
+}
+
+__global__ void k_pair_counts(PlanDev P, uint32_t *npu) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n_pairs) return;
+  int64_t b0;
+  bool has1;
+  uint32_t n;
+  pair_bins(P, p, b0, has1, n);
+  const uint32_t cap = (uint32_t)P.unit_cap;
+  npu[p] = (n + cap - 1) / cap;
+}
+
+__global__ void k_pair_units(PlanDev P) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n_pairs) return;
+  int64_t b0;
+  bool has1;
+  uint32_t n;
+  pair_bins(P, p, b0, has1, n);
+  const uint32_t cap = (uint32_t)P.unit_cap;
+  const uint32_t k0 = P.punit_start[p], nu = P.punit_start[p + 1] - k0;
+  const uint32_t b1 = has1 ? (uint32_t)(b0 + P.n_img_chan) : kNoBin;
+  for (uint32_t k = 0; k < nu; ++k)
+    P.punits[k0 + k] = make_uint4((uint32_t)b0, k * cap, min(cap, n - k * cap), b1);
+}
+
 // per-call tables: the FP32 kernel table; for every kernel offset, the FP64
 // sum of the support taps of one axis (norm = nrm[ou] * nrm[ov]); and the
 // zero-padded weight table by (tap, offset) in FP32 and FP64.  Offsets past
@@ -705,6 +743,9 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   d.max_gunits = d.n_bins + d.max_records / unit_cap + 1;
   // a sample has at most two primary records (grid + separate degrid)
   d.max_dunits = d.n_bins + 2 * n_items / unit_cap + 1;
+  const bool pairs = tile_u == 16 && tile_v == 16 && support <= 7;
+  d.n_pairs = pairs ? (int64_t)d.ntu * ((d.ntv + 1) / 2) * n_img_chan : 0;
+  d.max_punits = pairs ? d.n_pairs + 2 * n_items / unit_cap + 1 : 0;
   d.row_chan = nullptr;
   d.order_nb = 1;
   d.uvw = nullptr;
@@ -738,7 +779,8 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
          o_curp = take(4 * nb), o_curd = take(4 * nb), o_gus = take(4 * (nb + 1)),
          o_dus = take(4 * (nb + 1)), o_scan = take(4 * scan_blocks),
          o_rec = take(12 * std::max<int64_t>(d.max_records, 1)), o_gun = take(sizeof(uint4) * d.max_gunits),
-         o_dun = take(sizeof(uint4) * d.max_dunits),
+         o_dun = take(sizeof(uint4) * d.max_dunits), o_pus = take(4 * (d.n_pairs + 1)),
+         o_pun = take(sizeof(uint4) * d.max_punits),
          o_gm = take(8 * mask_words), o_dm = take(8 * mask_words),
          o_chh = take(4 * chunk_cells), o_chd = take(4 * chunk_cells), o_rh = take(4 * row_cells),
          o_tab = take(sizeof(float) * oversampling * (support / 2 + 1)),
@@ -764,6 +806,8 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   d.rec = (uint32_t *)(c + o_rec);
   d.gunits = (uint4 *)(c + o_gun);
   d.dunits = (uint4 *)(c + o_dun);
+  d.punit_start = d.n_pairs ? (uint32_t *)(c + o_pus) : nullptr;
+  d.punits = d.n_pairs ? (uint4 *)(c + o_pun) : nullptr;
   d.gmask = (unsigned long long *)(c + o_gm);
   d.dmask = (unsigned long long *)(c + o_dm);
   d.n_words = mask_words - 1;
@@ -916,6 +960,15 @@ int av_plan_set_uvw(av_plan *plan, const double *uvw, const double *freq,
   if ((rc = scan_exclusive_u32(d.cur_dup, d.dunit_start, d.n_bins, d.scan_tmp, s))) return rc;
   hipLaunchKernelGGL(k_units, dim3(bb), dim3(256), 0, s, d);
   AV_LAUNCH_CHECK("k_units");
+  if (d.n_pairs > 0) {
+    // the bins' cursors are free again: the pairs' unit counts
+    const unsigned pb = (unsigned)((d.n_pairs + 255) / 256);
+    hipLaunchKernelGGL(k_pair_counts, dim3(pb), dim3(256), 0, s, d, d.cur_prim);
+    AV_LAUNCH_CHECK("k_pair_counts");
+    if ((rc = scan_exclusive_u32(d.cur_prim, d.punit_start, d.n_pairs, d.scan_tmp, s))) return rc;
+    hipLaunchKernelGGL(k_pair_units, dim3(pb), dim3(256), 0, s, d);
+    AV_LAUNCH_CHECK("k_pair_units");
+  }
   return AV_OK;
 }
 

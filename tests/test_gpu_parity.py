@@ -352,13 +352,17 @@ def test_degrid_pol_map_and_constant_grid(gpu, oracle, tile):
     assert np.all(m32[np.abs(m32) > 0] == np.float32(2.0))
 
 
-@pytest.mark.parametrize("support,cap,prec", [(7, 0, 32), (7, 64, 32), (6, 0, 32), (5, 0, 32),
-                                              (7, 0, 64)])
-def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap, prec):
-    """Four polarisations, 16 x 16 tiles: supports 6 and 7 split each unit's
-    records into two v-halves computing 8 of the 12 column blocks; support 5
-    (11 blocks) runs all blocks.  Dense units take several sort passes of
-    256 records (cap 0) or are cut into 64-record units."""
+@pytest.mark.parametrize("support,cap,prec,n", [(7, 0, 32, 256), (7, 64, 32, 256), (6, 0, 32, 256),
+                                                (5, 0, 32, 256), (7, 0, 64, 256),
+                                                (7, 0, 32, 240), (7, 64, 32, 240)])
+def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap, prec, n):
+    """Four polarisations, 16 x 16 tiles.  FP32 with supports 6 and 7 runs
+    pair units (two v-neighbouring tiles, four 16-column windows of the 20
+    column blocks; a 240-cell grid leaves the last tile of each row without
+    a partner), FP64 splits each tile's records into two v-halves computing
+    8 of the 12 column blocks, and support 5 (11 blocks) runs all blocks.
+    Dense units take several sort passes of 256 records (cap 0) or are cut
+    into 64-record units."""
     from astroviper_amd import synth
     arr = synth.ArrayConfig(n_pol=4, seed=3)
     obs = synth.make_observation(arr, synth.SkyConfig())
@@ -366,7 +370,7 @@ def test_degrid_v_halves_match_oracle(gpu, oracle, support, cap, prec):
     nt, nb, nchan = uvw.shape[0], uvw.shape[1], freq.size
     maxuv = np.max(np.hypot(uvw[..., 0], uvw[..., 1]))
     c = 0.45 * C / freq[-1] / maxuv
-    n_uv, cell = (256, 256), (-c, c)
+    n_uv, cell = (n, n), (-c * 256 / n, c * 256 / n)
     rng = np.random.default_rng(support + cap)
     flag = rng.random((nt, nb, nchan, 4)) < 0.1
     grid = rng.standard_normal((nchan, 4) + n_uv) + 1j * rng.standard_normal((nchan, 4) + n_uv)
