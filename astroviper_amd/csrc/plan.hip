@@ -33,19 +33,38 @@ struct ItemGeo {
   int lu, lv, fu, fv;  // degrid centre and kernel offsets
 };
 
-// Geometry of the sample of row `row` at channel `chan` (the row's own
-// channel when rows carry one).
-__device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, ItemGeo &g) {
+// The per-channel constants of the geometry: loaded once per channel where
+// a loop visits many rows of one channel (the chunked binning passes),
+// instead of once per sample behind the sample's own loads.
+struct ChanGeo {
+  int chan, ic, ch;  // channel, image channel, channel the tables are read at
+  bool ok;           // both in range
+  double freq, gsu, gsv;
+};
+
+__device__ inline ChanGeo chan_geo(const PlanDev &P, int chan) {
+  ChanGeo c;
+  c.chan = chan;
+  c.ok = chan >= 0 && chan < P.n_chan;
+  c.ch = c.ok ? chan : 0;
+  c.ic = P.chan_map[c.ch];
+  c.ok = c.ok && c.ic >= 0 && c.ic < P.n_img_chan;
+  c.freq = P.freq[c.ch];
+  c.gsu = P.gscale_u[c.ch];
+  c.gsv = P.gscale_v[c.ch];
+  return c;
+}
+
+// Geometry of the sample of row `row` at channel c, whose u and v are uu
+// and vv.
+__device__ inline void sample_geometry_at(const PlanDev &P, int64_t row, const ChanGeo &c,
+                                          double uu, double vv, ItemGeo &g) {
   g.row = row;
-  g.chan = P.row_chan ? P.row_chan[row] : chan;
-  bool chan_ok = g.chan >= 0 && g.chan < P.n_chan;
-  int ch = chan_ok ? g.chan : 0;
-  g.ic = P.chan_map[ch];
-  chan_ok = chan_ok && g.ic >= 0 && g.ic < P.n_img_chan;
-  double uu = P.uvw[g.row * 3 + 0];
-  double vv = P.uvw[g.row * 3 + 1];
+  g.chan = c.chan;
+  g.ic = c.ic;
+  const bool chan_ok = c.ok;
   const int sc = P.support / 2;
-  g.dvalid = degrid_geometry(uu, vv, P.freq[ch], P.dscale_u, P.dscale_v, P.doff_u, P.doff_v,
+  g.dvalid = degrid_geometry(uu, vv, c.freq, P.dscale_u, P.dscale_v, P.doff_u, P.doff_v,
                              P.n_u, P.n_v, P.support, P.ov, g.lu, g.lv, g.fu, g.fv) &&
              chan_ok;
   if (P.geometry == AV_GEOM_DEGRID) {
@@ -55,8 +74,8 @@ __device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, 
     g.ou = g.fu;
     g.ov = g.fv;
   } else {
-    g.gvalid = grid_geometry(uu, vv, P.gscale_u[ch], P.gscale_v[ch], P.n_u, P.n_v, sc, P.ov,
-                             g.iu, g.iv, g.ou, g.ov) &&
+    g.gvalid = grid_geometry(uu, vv, c.gsu, c.gsv, P.n_u, P.n_v, sc, P.ov, g.iu, g.iv, g.ou,
+                             g.ov) &&
                chan_ok;
     if (P.conjugate) {
       // the second scatter of do_imaging_weight (standard_grid.py:330-340,
@@ -64,12 +83,19 @@ __device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, 
       // kernel offsets and bounds test, no degrid record
       g.dvalid = false;
       if (g.gvalid) {
-        const double u = uu * P.gscale_u[ch], v = vv * P.gscale_v[ch];
+        const double u = uu * c.gsu, v = vv * c.gsv;
         g.iu = (int)(-u + (double)(P.n_u / 2) + 0.5);
         g.iv = (int)(-v + (double)(P.n_v / 2) + 0.5);
       }
     }
   }
+}
+
+// Geometry of the sample of row `row` at channel `chan` (the row's own
+// channel when rows carry one).
+__device__ inline void sample_geometry(const PlanDev &P, int64_t row, int chan, ItemGeo &g) {
+  const ChanGeo c = chan_geo(P, P.row_chan ? P.row_chan[row] : chan);
+  sample_geometry_at(P, row, c, P.uvw[row * 3 + 0], P.uvw[row * 3 + 1], g);
 }
 
 __device__ inline int64_t bin_of(const PlanDev &P, int tu, int tv, int ic) {
@@ -276,11 +302,12 @@ __global__ __launch_bounds__(kChunkThreads) void k_count_chunks(PlanDev P) {
   const int ntiles = P.ntu * P.ntv;
   uint32_t *hd = h + ntiles;
   for (int i = threadIdx.x; i < 2 * ntiles; i += blockDim.x) h[i] = 0u;
+  const ChanGeo cg = chan_geo(P, chan);
   __syncthreads();
   for (int64_t row = r0 + threadIdx.x; row < r0 + P.rows_per_chunk; row += blockDim.x) {
     ItemGeo g;
     g.gvalid = g.dvalid = false;
-    if (row < P.n_rows) sample_geometry(P, row, chan, g);
+    if (row < P.n_rows) sample_geometry_at(P, row, cg, P.uvw[row * 3 + 0], P.uvw[row * 3 + 1], g);
     write_masks(P, row, chan, g.gvalid, g.dvalid);
     if (!(g.gvalid || g.dvalid)) continue;
     const RecPlan rp = record_plan(P, g);
@@ -399,6 +426,7 @@ __global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P) {
   const uint32_t nt = (uint32_t)((r1 - 1) / nb - t_lo + 1);
   const uint32_t n_pos = nt * (uint32_t)nb;
   const uint64_t nt_mul = ((1ull << 32) + nt - 1) / nt;  // floor(p * nt_mul / 2^32) - p / nt in {0, 1}
+  const ChanGeo cg = chan_geo(P, chan);
   const uint32_t lane = __lane_id();
   for (uint32_t p0 = threadIdx.x - lane; p0 < n_pos; p0 += blockDim.x) {
     const uint32_t p = p0 + lane;
@@ -411,7 +439,7 @@ __global__ __launch_bounds__(kChunkThreads) void k_scatter_chunks(PlanDev P) {
     }
     ItemGeo g;
     g.gvalid = g.dvalid = false;
-    if (row >= 0) sample_geometry(P, row, chan, g);
+    if (row >= 0) sample_geometry_at(P, row, cg, P.uvw[row * 3 + 0], P.uvw[row * 3 + 1], g);
     const RecPlan rp = record_plan(P, g);
     const int64_t item = row * P.n_chan + chan;
     const uint32_t slot = run_slots(cp, rp.gprim ? rp.tu * P.ntv + rp.tv : -1);
