@@ -89,8 +89,11 @@ def test_noalloc_matches_plan_path(gpu, oracle, prec):
             w = w.astype(np.float32).astype(np.float64)
         g1, s1, g2, s2 = grid_both_ways(gpu, lib, obs, vis, w, flag, n_uv, cell, prec, cube)
         # record order inside a bin comes from atomic cursors, so sums may
-        # differ in their last bits between any two runs of either path
-        tol = 1e-12 if prec == 64 else 1e-6
+        # differ in their last bits between any two runs of either path: in
+        # FP32 a continuum bin here sums thousands of records, and two orders
+        # of one sum differ by up to ~1e-6 of the grid maximum (measured
+        # 1.04e-6); a stale or wrong plan would differ by O(1)
+        tol = 1e-12 if prec == 64 else 1e-5
         assert rel(g2, g1) < tol, (seed, prec)
         assert rel(s2, s1) < 1e-14, (seed, prec)
         # and both are the oracle's
