@@ -197,6 +197,22 @@ def test_row_order_binning(gpu, oracle, c2_subset, mode):
     b["plan"].close()
 
 
+def test_row_order_arguments(gpu):
+    """av_plan_set_row_order takes n_baseline >= 1 dividing n_rows; other
+    values are refused with an error and leave the plan usable."""
+    import ctypes
+    from astroviper_amd import _lib
+    from astroviper_amd.device import GridPlan
+    plan = GridPlan(6000, 2, 2, (256, 256), (-1e-5, 1e-5), 7, 100, tile=(16, 16), device=gpu)
+    lib = _lib.lib()
+    for bad in (0, -3, 7):  # 7 does not divide 6000
+        assert lib.av_plan_set_row_order(plan._h, bad) != 0
+        assert b"n_baseline" in lib.av_last_error()
+    assert lib.av_plan_set_row_order(plan._h, 300) == 0
+    assert lib.av_plan_set_row_order(ctypes.c_void_p(), 300) != 0
+    plan.close()
+
+
 # ------------------------------------------- 4096^2 precision variants
 @pytest.fixture(scope="module")
 def c5_subset(oracle):
