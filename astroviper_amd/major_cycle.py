@@ -33,6 +33,38 @@ import torch
 from . import device as D
 from .deconvolvers import hogbom as H
 
+_SIGNS = {}
+
+
+def model_to_uv(model: torch.Tensor, cdt: torch.dtype) -> torch.Tensor:
+    """fftshift(fft2(ifftshift(model))) over the last two axes, in the
+    complex dtype `cdt`: fft_lm_to_uv as predict_model_visibilities applies
+    it (imager.py:96-99, fft.py:5-18).
+
+    When both axes are multiples of 4 the two shifts by N/2 are sign flips:
+    with S[i, j] = (-1)^(i + j), fftshift(fft2(ifftshift(x))) = S * fft2(S * x),
+    exactly in exact arithmetic (the shift theorem; (-1)^(N0/2 + N1/2) = 1).
+    So the image is cast once, flipped in place, transformed and flipped in
+    place: two passes of the cube instead of the four of two rolls."""
+    ny, nx = model.shape[-2:]
+    ax = (-2, -1)
+    if ny % 4 or nx % 4:
+        m = model.to(cdt)
+        return torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(m, dim=ax), dim=ax), dim=ax)
+    key = (ny, nx, model.device, cdt)
+    s = _SIGNS.get(key)
+    if s is None:
+        rdt = torch.float32 if cdt == torch.complex64 else torch.float64
+        i = torch.arange(ny, device=model.device).reshape(-1, 1)
+        j = torch.arange(nx, device=model.device).reshape(1, -1)
+        s = (1 - 2 * ((i + j) % 2)).to(rdt)
+        _SIGNS[key] = s
+    x = model.to(cdt)
+    x.mul_(s)
+    y = torch.fft.fft2(x, dim=ax)
+    y.mul_(s)
+    return y
+
 
 class ResidentImager:
     """Device-resident state of one measurement set for major cycles.
@@ -102,11 +134,8 @@ class ResidentImager:
         degrid it into the model visibilities (added to them when
         incremental)."""
         m = torch.as_tensor(model_image).to(self.dev)
-        m = m.to(torch.complex128 if self.cdt == torch.complex128 else torch.complex64)
-        ax = (-2, -1)
-        grid = torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(m, dim=ax), dim=ax), dim=ax)
-        self.plan.degrid(grid.to(self.cdt).contiguous(), self.flag, self.model_vis,
-                         incremental=incremental)
+        grid = model_to_uv(m, self.cdt)
+        self.plan.degrid(grid.contiguous(), self.flag, self.model_vis, incremental=incremental)
 
     def compute_residual_visibilities(self) -> None:
         """imager.compute_residual_visibilities (imager.py:82-85):

@@ -636,7 +636,8 @@ def row_mosaic(dev, n_time=100, cpu_s=5.0):
 def row_major_cycle(dev, reps=3):
     """One major cycle of imager.py:108-342 on the benchmark workload (64
     antennas x 1000 times x 64 channels x 4 pols, 2048 x 2048 cube, FP32
-    gridding, FP64 FFTs) with every visibility-sized array and the binning
+    gridding and prediction, FP64 FFTs for the dirty image) with every
+    visibility-sized array and the binning
     plan resident on the GPU: grid the residual visibilities and form the
     dirty image (grid2image_spheroid_ms4), FFT a model image to the uv plane
     and degrid it (predict_model_visibilities, imager.py:88-105), and form
@@ -649,6 +650,7 @@ def row_major_cycle(dev, reps=3):
 
     from astroviper_amd import synth
     from astroviper_amd.device import GridPlan, dirty_image
+    from astroviper_amd.major_cycle import model_to_uv
 
     cfg = synth.CONFIG2
     arr = cfg["array"]
@@ -687,9 +689,7 @@ def row_major_cycle(dev, reps=3):
         e[1].record()
         img = dirty_image(grid, sumwt)
         e[2].record()
-        ax = (-2, -1)
-        uvm = torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(model, dim=ax), dim=ax),
-                                 dim=ax).to(torch.complex64)
+        uvm = model_to_uv(model, torch.complex64)  # as ResidentImager predicts in FP32
         plan.degrid(uvm, fl2, model_vis)
         e[3].record()
         torch.sub(vis2, model_vis, out=resid)
@@ -733,7 +733,7 @@ def row_major_cycle(dev, reps=3):
     return {"row": "major_cycle",
             "workload": f"one major cycle without CLEAN, {V} vis resident (64 antennas x 1000 "
                         f"times x {nchan} chan x {npol} pol), {n_uv[0]}x{n_uv[1]} cube, FP32 "
-                        f"gridding, FP64 FFTs",
+                        f"gridding and prediction, FP64 FFTs for the dirty image",
             "value": round(V / (tot / 1e3) / 1e6, 1), "unit": "M visibilities/sec",
             "ms": {**parts, "cycle": round(tot, 3)},
             "roofline": _roof(b_cycle, parts["grid"] + parts["predict"], "grid + degrid"),

@@ -119,3 +119,20 @@ def test_major_cycles_match_oracle_loop(gpu, oracle):
     assert len(log) == 2 and log[1]["peak_before"] < log[0]["peak_before"]
     assert rel(model.cpu().numpy(), m_ref) < 1e-5
     assert rel(resid.cpu().numpy(), r_img) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(3, 64, 128), (2, 60, 66), (1, 62, 64)])
+@pytest.mark.parametrize("cdt", ["complex64", "complex128"])
+def test_model_to_uv_matches_shifted_fft(gpu, shape, cdt):
+    """model_to_uv (the sign-flip form S * fft2(S * x) when both axes are
+    multiples of 4, the two shifts otherwise) against NumPy's
+    fftshift(fft2(ifftshift(x))), the reference's fft_lm_to_uv (fft.py:5-18),
+    in FP64 on the CPU."""
+    from astroviper_amd.major_cycle import model_to_uv
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape)
+    ax = (-2, -1)
+    ref = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(x, axes=ax), axes=ax), axes=ax)
+    got = model_to_uv(torch.from_numpy(x).to(gpu), getattr(torch, cdt)).cpu().numpy()
+    assert got.dtype == np.dtype(cdt)
+    assert rel(got, ref) < (1e-6 if cdt == "complex64" else 1e-13)
