@@ -8,6 +8,10 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL32 = 1e-4
+# two FP32 runs whose records reach a bin in different orders (atomic
+# cursors): their tile sums differ by FP32 rounding, which for bins of
+# thousands of records reaches ~1e-6 of the grid maximum
+FP32_ORDER = 1e-5
 
 
 def rel(a, b):
@@ -84,15 +88,17 @@ def test_graph_replays_match_eager(gpu):
         # binning fills each bin through atomic cursors, so the order of the
         # records in a bin (and with it the order of the FP32 tile sums and
         # the FP64 weight sums) can differ from run to run: grids and model
-        # visibilities agree to FP32 rounding, sums of weights to FP64
+        # visibilities agree to FP32 rounding of sums of thousands of
+        # records (up to ~1e-6 of the maximum; FP32_ORDER), sums of weights
+        # to FP64
         for g, r in zip(bg.grids, ref_g):
             assert torch.isfinite(g).all(), k
-            assert rel(g.cpu().numpy(), r.cpu().numpy()) < 1e-6, k
+            assert rel(g.cpu().numpy(), r.cpu().numpy()) < FP32_ORDER, k
         for w, r in zip(bg.sumwts, ref_w):
             assert rel(w.cpu().numpy(), r.cpu().numpy()) < 1e-12, k
         for m, r in zip(bg.models, ref_m):
             assert torch.isfinite(m).all(), k
-            assert rel(m.cpu().numpy(), r.cpu().numpy()) < 1e-6, k
+            assert rel(m.cpu().numpy(), r.cpu().numpy()) < FP32_ORDER, k
     bg.close()
 
 
@@ -143,9 +149,9 @@ def test_row_order_does_not_change_results(gpu, n_uv):
     (st0, g0, w0, m0), (st1, g1, w1, m1) = res
     assert st0 == st1
     assert st0["records"] > st0["grid_valid"] > 0
-    assert rel(g1, g0) < 1e-6
+    assert rel(g1, g0) < FP32_ORDER
     assert rel(w1, w0) < 1e-12
-    assert rel(m1, m0[perm.cpu().numpy()]) < 1e-6
+    assert rel(m1, m0[perm.cpu().numpy()]) < FP32_ORDER
 
 
 def test_concurrent_sets_match_sequential(gpu):
@@ -164,8 +170,8 @@ def test_concurrent_sets_match_sequential(gpu):
         bg.close()
     (g0, w0, m0), (g1, w1, m1) = out
     for a, b in zip(g0, g1):
-        assert rel(b, a) < 1e-6
+        assert rel(b, a) < FP32_ORDER
     for a, b in zip(w0, w1):
         assert rel(b, a) < 1e-12
     for a, b in zip(m0, m1):
-        assert rel(b, a) < 1e-6
+        assert rel(b, a) < FP32_ORDER
