@@ -243,6 +243,7 @@ def small_workload_leg(dev, reps=10):
     # the GPU degrids its own FP32 grid; the oracle model is from the FP64 grid
     mm = m.cpu().numpy().reshape(mref.shape)
     plan.close()
+    drop_in = drop_in_leg(obs, vis, w, flag, cell, n_uv, gref, iref, mref)
     return {"workload": f"27 antennas x {nt} times x {nchan} chan x {npol} pol = {V} vis, "
                         f"{n_uv[0]}x{n_uv[1]} cube, 5\" cells, 10% flagged, weights U(0.5,1.5)",
             "cpu_fp64_one_core": {"value": round(V / float(np.median(tcpu)) / 1e6, 3),
@@ -252,7 +253,69 @@ def small_workload_leg(dev, reps=10):
             "max_rel_err_vs_fp64_oracle": {
                 "grid": rel(g.cpu().numpy(), gref), "sumwt": rel(sw.cpu().numpy(), swref),
                 "dirty": img_err(img, iref), "psf": img_err(psf, psfref),
-                "degrid_from_own_grid": rel(mm, mref)}}
+                "degrid_from_own_grid": rel(mm, mref)},
+            "drop_in": drop_in}
+
+
+def drop_in_leg(obs, vis, w, flag, cell, n_uv, gref, iref, mref, reps=5):
+    """The reference entry points end to end on the 27-antenna workload:
+    grid2image_spheroid_ms4 (standard_grid.py:607-708) and
+    degrid_spheroid_ms4 (standard_degrid.py:488-608) from an MSv4-shaped
+    dataset of host arrays to host results (copies to the device, a plan per
+    call, FP64 gridding and FFT as the complex128 data select, copies back),
+    median of `reps` after one warm-up call, beside the FP64 oracle doing the
+    same work on one host core (gridding + normalisation, degridding)."""
+    from astroviper_amd.msv4 import make_visibility_dataset
+    from astroviper_amd.standard_degrid import degrid_spheroid_ms4
+    from astroviper_amd.standard_grid import grid2image_spheroid_ms4
+    from oracle import oracle as O
+
+    nt, nb, nchan, npol = w.shape
+    pix = np.array(cell)
+
+    def ds():
+        return make_visibility_dataset(vis.copy(), obs.uvw.copy(), w.copy(), flag.copy(), obs.freq)
+
+    def t_grid():
+        d = ds()
+        img = np.zeros((nchan, npol) + tuple(n_uv))
+        t0 = time.perf_counter()
+        grid2image_spheroid_ms4(d, img, pix, 7, 100, dopsf=False, column="VISIBILITY",
+                                chan_mode="cube")
+        return time.perf_counter() - t0, img
+
+    def t_degrid():
+        d = ds()
+        t0 = time.perf_counter()
+        degrid_spheroid_ms4(d, gref, pix, 7, 100)
+        return time.perf_counter() - t0, np.asarray(d["VISIBILITY_MODEL"].data)
+
+    t_grid()
+    t_degrid()
+    tg, img = zip(*[t_grid() for _ in range(reps)])
+    td, mod = zip(*[t_degrid() for _ in range(reps)])
+    w0 = np.where(flag, 0.0, w)
+    tcg, tcd = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        g, s = O.standard_grid(vis, obs.uvw, w0, obs.freq, n_uv, cell, chan_mode="cube")
+        O.normalise(g, s)
+        tcg.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        O.degrid(obs.uvw, flag, obs.freq, gref, cell)
+        tcd.append(time.perf_counter() - t0)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    return {"what": "host dataset in, host image / model visibilities out: copies, binning, "
+                    "FP64 gridding or degridding, FFT (gridding), one plan per call",
+            "grid2image_spheroid_ms4_ms": round(1e3 * float(np.median(tg)), 3),
+            "degrid_spheroid_ms4_ms": round(1e3 * float(np.median(td)), 3),
+            "cpu_fp64_one_core_grid_image_ms": round(1e3 * float(np.median(tcg)), 3),
+            "cpu_fp64_one_core_degrid_ms": round(1e3 * float(np.median(tcd)), 3),
+            "max_rel_err_vs_fp64_oracle": {"dirty": rel(img[-1], iref),
+                                           "model": rel(mod[-1].reshape(mref.shape), mref)}}
 
 
 # --------------------------------------------------------------------------
