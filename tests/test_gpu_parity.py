@@ -563,6 +563,49 @@ def test_degrid_skips_out_of_range_pol_and_zero_norm(gpu, oracle, prec):
         plan.close()
 
 
+def test_pair_degrid_pol_map_and_zero_norm(gpu, oracle):
+    """Four polarisations on 16 x 16 tiles in FP32 take the pair-unit
+    degridder: polmap entries outside a two-plane grid (3 and -1) give 0
+    when overwriting and leave the value when incremental, the others match
+    the oracle's dgrid_numba (standard_degrid.py:419-420), and a kernel table
+    that vanishes everywhere adds nothing (:482)."""
+    from astroviper_amd.device import GridPlan, degrid_kernel_name
+    uvw, freq, vis, w, flag, n_uv, cell = c1_data(dense=True)
+    rows, nchan = uvw.shape[0] * uvw.shape[1], freq.size
+    assert degrid_kernel_name((16, 16), 4) == "k_degrid_h3"
+    rng = np.random.default_rng(23)
+    grid = rng.standard_normal((1, 2) + n_uv) + 1j * rng.standard_normal((1, 2) + n_uv)
+    grid = as_prec(32, grid)
+    plan = GridPlan(rows, nchan, 1, n_uv, cell, tile=(16, 16), device=gpu)
+    plan.set_uvw(torch.from_numpy(uvw.reshape(rows, 3)).to(gpu), torch.from_numpy(freq).to(gpu))
+    gd = torch.from_numpy(grid).to(gpu, torch.complex64)
+    pm = torch.tensor([0, 3, 1, -1], dtype=torch.int32)
+    m = plan.degrid(gd, n_pol=4, pol_map=pm).cpu().numpy()
+    vals = np.zeros((uvw.shape[0], uvw.shape[1], nchan, 4), dtype=np.complex128)
+    sc = np.array([-cell[0] * n_uv[0], -cell[1] * n_uv[1], 0.0])
+    of = np.array([n_uv[0] / 2, n_uv[1] / 2, 0.0])
+    conv = oracle.create_prolate_spheroidal_kernel_1D(100, 7)
+    oracle.dgrid(uvw, vals, np.zeros(vals.shape, bool), sc, of, grid, freq, 7, 100, conv,
+                 np.zeros(nchan, dtype=np.int64), np.array([0, 3, 1, -1], dtype=np.int64))
+    vals = vals.reshape(-1, 4)
+    assert np.all(vals[:, 1] == 0) and np.all(vals[:, 3] == 0)
+    assert np.all(m[:, 1] == 0) and np.all(m[:, 3] == 0)
+    for p in (0, 2):
+        assert rel(m[:, p], vals[:, p]) < TOL32
+    start = torch.full((rows * nchan, 4), 1.5 + 0.5j, dtype=torch.complex64, device=gpu)
+    m2 = plan.degrid(gd, vis_out=start, n_pol=4, pol_map=pm, incremental=True).cpu().numpy()
+    for p in (1, 3):
+        assert np.all(m2[:, p] == np.complex64(1.5 + 0.5j))
+    for p in (0, 2):
+        assert rel(m2[:, p] - np.complex64(1.5 + 0.5j), vals[:, p]) < TOL32
+    keep = plan.cgk
+    plan.cgk = torch.zeros_like(keep)
+    m3 = plan.degrid(gd, n_pol=4, pol_map=pm).cpu().numpy()
+    assert np.all(m3 == 0) and not np.isnan(m3).any()
+    plan.cgk = keep
+    plan.close()
+
+
 def test_plain_gridder_sumwt_reduced_per_wavefront(gpu, oracle):
     """The plain gridder adds each wavefront's sum of weights per
     polarisation with one atomic; the sums still equal the oracle's, for
