@@ -197,6 +197,36 @@ def test_row_order_binning(gpu, oracle, c2_subset, mode):
     b["plan"].close()
 
 
+def test_row_order_hint_that_does_not_match(gpu, oracle, c2_subset):
+    """A baseline count that divides the rows but is not the rows' real one
+    (1600 for 2016 baselines) only changes the order in which a chunk's rows
+    are visited: every row is still visited once, so the counts and records
+    are those of row order and the FP64 grid agrees to rounding."""
+    d, refs = c2_subset
+    from astroviper_amd.device import GridPlan
+    nt, nb, nchan, npol = d["w"].shape
+    rows = nt * nb
+    assert nb != 1600 and rows % 1600 == 0
+    uvw = torch.from_numpy(np.ascontiguousarray(d["uvw"].reshape(rows, 3))).to(gpu)
+    freq = torch.from_numpy(d["freq"]).to(gpu)
+    v = torch.from_numpy(d["vis"].reshape(rows * nchan, npol)).to(gpu)
+    w = torch.from_numpy(d["w"].reshape(rows * nchan, npol)).to(gpu)
+    fl = torch.from_numpy(d["flag"].reshape(rows * nchan, npol).astype(np.uint8)).to(gpu)
+    out = []
+    for hint in (1600, 1):
+        plan = GridPlan(rows, nchan, nchan, d["n_uv"], d["cell"], 7, 100, tile=(16, 16),
+                        device=gpu, n_baseline=hint)
+        plan.set_uvw(uvw, freq)
+        g, sw = plan.grid(v, w, fl)
+        torch.cuda.synchronize()
+        out.append((plan.stats(), g.cpu().numpy(), sw.cpu().numpy()))
+        plan.close()
+    (s0, g0, w0), (s1, g1, w1) = out
+    assert s0 == s1
+    assert rel(g0, g1) < 1e-14 and rel(w0, w1) < 1e-14
+    assert rel(g0, refs["cube"]["grid"]) < TOL64
+
+
 def test_row_order_arguments(gpu):
     """av_plan_set_row_order takes n_baseline >= 1 dividing n_rows; other
     values are refused with an error and leave the plan usable."""
