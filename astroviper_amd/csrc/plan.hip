@@ -350,10 +350,13 @@ __global__ void k_chunk_totals(PlanDev P) {
 }
 
 // Chunked pass 3 (after the bin offsets): the first slot of every chunk's
-// run in every bin.  The chunks of a bin take their runs in chunk order, so
-// the records of a bin are in sample order from chunk to chunk.  The first
-// primary slot overwrites the primary counts (chunk_hist), the first
-// duplicate slot the duplicate counts (chunk_dup).
+// run in every bin.  The first primary slot overwrites the primary counts
+// (chunk_hist), the first duplicate slot the duplicate counts (chunk_dup).
+// When image channels gather several channels (continuum), a bin takes its
+// runs row-chunk by row-chunk, channel by channel within one: the channels
+// of a row, which share cache lines, then lie a run apart in the bin
+// instead of a whole channel's records apart.  Otherwise (one channel per
+// image channel) a bin takes its channel's runs in row order.
 __global__ void k_chunk_bases(PlanDev P) {
   const int ntiles = P.ntu * P.ntv;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,15 +366,22 @@ __global__ void k_chunk_bases(PlanDev P) {
   const int64_t b = (int64_t)t * P.n_img_chan + ic;
   uint32_t sp = P.bin_start[b];
   uint32_t sd = sp + P.cnt_prim[b];
-  for (int c = 0; c < P.n_chan; ++c) {
-    if (P.chan_map[c] != ic) continue;
-    for (int k = 0; k < P.chunks_per_chan; ++k) {
-      const int64_t i = ((int64_t)c * P.chunks_per_chan + k) * ntiles + t;
-      const uint32_t np = P.chunk_hist[i], nd = P.chunk_dup[i];
-      P.chunk_hist[i] = sp;
-      P.chunk_dup[i] = sd;
-      sp += np;
-      sd += nd;
+  auto take = [&](int c, int k) {
+    const int64_t i = ((int64_t)c * P.chunks_per_chan + k) * ntiles + t;
+    const uint32_t np = P.chunk_hist[i], nd = P.chunk_dup[i];
+    P.chunk_hist[i] = sp;
+    P.chunk_dup[i] = sd;
+    sp += np;
+    sd += nd;
+  };
+  if (P.n_img_chan < P.n_chan) {
+    for (int k = 0; k < P.chunks_per_chan; ++k)
+      for (int c = 0; c < P.n_chan; ++c)
+        if (P.chan_map[c] == ic) take(c, k);
+  } else {
+    for (int c = 0; c < P.n_chan; ++c) {
+      if (P.chan_map[c] != ic) continue;
+      for (int k = 0; k < P.chunks_per_chan; ++k) take(c, k);
     }
   }
 }
