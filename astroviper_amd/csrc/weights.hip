@@ -4,9 +4,10 @@
 //
 // Nearest-pixel gridding of the data weights (and their conjugate pixels)
 // into a float64 density grid per channel, and the per-sample Briggs
-// reweighting from that grid.  One thread per sample, channel-major so that
-// a wavefront's samples share a channel: the sum of weights of a wavefront is
-// reduced first and added with one atomic.  Grid cells take FP64 atomics
+// reweighting from that grid.  One thread per sample, in blocks of time
+// steps and channel by channel within a block, so that a wavefront's
+// samples share a channel: the sum of weights of a wavefront is reduced
+// first and added with one atomic.  Grid cells take FP64 atomics
 // (one sample adds to two cells); the sums therefore depend on the order of
 // arrival in their last bits only.
 #include "avgrid_internal.h"
@@ -23,16 +24,27 @@ __device__ inline bool nearest_pixel(double x, int n, int &i) {
   return i >= 0 && i < n;
 }
 
-// Sample i of a launch: channel-major, then baseline, then time (rows are
-// time-major, n_baseline per time step), so that a wavefront holds
-// consecutive time samples of one baseline, which usually fall in the same
-// pixel; one thread per (row, channel).
-__device__ inline void sample_of(int64_t i, int64_t n_rows, int64_t nb, int &ch, int64_t &row) {
+// Sample i of a launch (one thread per (row, channel); rows are time-major,
+// n_baseline per time step): blocks of kTimeBlock time steps, within a block
+// channel by channel, within a channel baseline by baseline with time
+// fastest.  A wavefront then holds consecutive time samples of one baseline,
+// which usually fall in the same pixel, and a block's uvw rows (and the
+// lines of neighbouring channels' weights) are read by all its channels
+// while they are still in cache: taking whole channels in turn read them
+// from memory once per channel (grid pass 12.9 -> 9.3 ms, reweighting
+// 7.6 -> 3.5 ms on 64 channels x 2016 baselines x 1000 times).
+constexpr int64_t kTimeBlock = 16;
+__device__ inline void sample_of(int64_t i, int64_t n_rows, int64_t nb, int nch, int &ch,
+                                 int64_t &row) {
   const int64_t nt = n_rows / nb;
-  ch = (int)(i / n_rows);
-  const int64_t rem = i - (int64_t)ch * n_rows;
-  const int64_t b = rem / nt;
-  row = (rem - b * nt) * nb + b;
+  const int64_t full = kTimeBlock * nb * nch;  // samples of a full block
+  const int64_t k = i / full;
+  const int64_t j = i - k * full;
+  const int64_t bt = min(kTimeBlock, nt - k * kTimeBlock);  // time steps of this block (the last may be short)
+  ch = (int)(j / (bt * nb));
+  const int64_t rem = j - (int64_t)ch * bt * nb;
+  const int64_t b = rem / bt;
+  row = (k * kTimeBlock + (rem - b * bt)) * nb + b;
 }
 
 // Sum v over the runs of equal keys in lane order (the wavefront's lanes
@@ -64,7 +76,7 @@ __global__ __launch_bounds__(256) void k_grid_imaging_weights(
   const int64_t total = n_rows * n_chan;
   int ch = n_chan - 1;
   int64_t row = 0;
-  if (i < total) sample_of(i, n_rows, n_baseline, ch, row);
+  if (i < total) sample_of(i, n_rows, n_baseline, n_chan, ch, row);
   double add = 0.0;
   // this sample's pixel and conjugate pixel (-1: none), and its weight
   int64_t pix = -1, cpix = -1;
@@ -123,7 +135,7 @@ __global__ __launch_bounds__(256) void k_degrid_imaging_weights(
   if (i >= total) return;
   int ch;
   int64_t row;
-  sample_of(i, n_rows, n_baseline, ch, row);
+  sample_of(i, n_rows, n_baseline, n_chan, ch, row);
   const int64_t k = row * n_chan + ch;
   const double su = -(freq[ch] * cell_u * (double)n_u) / kLightSpeed;
   const double sv = -(freq[ch] * cell_v * (double)n_v) / kLightSpeed;
