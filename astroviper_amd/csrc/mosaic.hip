@@ -24,6 +24,14 @@
 // tile, so no atomics.  The tile then goes to HBM once, split bins adding
 // theirs with atomics.
 //
+// The taps of a record that count (inside the tile and inside the
+// convolution function) form a rectangle, which staging works out once per
+// record and polarisation, one record per lane, with its first kernel-value
+// index and cell.  The wavefronts then walk only that rectangle: a tap is a
+// quotient by the rectangle's width and two 24-bit multiply-adds from
+// wave-uniform bases, with no bounds tests, and taps outside the tile
+// (most of a record near a tile edge) cost nothing.
+//
 // The kernel sum of a record (the reference's `norm`, over all its taps)
 // depends only on its convolution function and the kernel offsets of its
 // first tap, so it is tabulated per call, summed in the reference's order
@@ -35,6 +43,9 @@
 namespace av {
 
 constexpr int kMosaicStage = 64;
+// units per XCD group: a tile's channels are consecutive units, so a group
+// of 64 keeps a tile's units (and the samples they share) on one XCD's L2
+constexpr uint32_t kMosaicGroup = 64;
 
 template <typename R>
 struct MosaicArgs {
@@ -50,14 +61,20 @@ struct MosaicArgs {
   int du_lo, ndu, dv_lo, ndv;
 };
 
-// per staged record: centre in tile coordinates, cf row/column of tap 0,
-// and per polarisation the cf base, supports and weighted data
+// per staged record and polarisation: the h x w rectangle of its taps that
+// count (n = h * w, 0 for a record with nothing to add here).  Tap t of it
+// is row q = t / w, column p = t - q * w; its kernel value is element
+// cfoff + q * ov_u * cv_n + p * ov_v of the convolution functions and its
+// cell is cell0 + q * tile_v + p.
 template <typename R, int NPOL>
 struct MosaicStaged {
-  int cu, cv, cfu0, cfv0;
-  int base[NPOL], su[NPOL], sv[NPOL];
+  int n[NPOL], w[NPOL], cfoff[NPOL], cell0[NPOL];
+  float rw[NPOL];  // 1 / w rounded up: (int)(t * rw) = t / w for t < 2^16
   R dre[NPOL], dim[NPOL];
 };
+
+// floor(x / d) for d > 0
+__device__ inline int floor_div(int x, int d) { return x >= 0 ? x / d : -((-x + d - 1) / d); }
 
 // The kernel-sum table: one thread per (convolution function, first-tap
 // offsets), summing the in-range taps as the reference does.
@@ -96,7 +113,7 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
   St *st = (St *)(smem + (size_t)T * TV * NPOL * sizeof(Cplx2<R>));
   const int tid = threadIdx.x, lane = tid & 63;
   const int wp = __builtin_amdgcn_readfirstlane(tid >> 6);  // this wavefront's polarisation
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = xcd_grouped_block(blockIdx.x, kMosaicGroup);
   const uint32_t n_units = P.gunit_start[P.n_bins];
   if (u >= n_units) return;
   const uint4 unit = P.gunits[u];
@@ -113,32 +130,19 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
   for (int i = tid; i < T * TV * NPOL; i += 64 * NPOL) tile[i] = Cplx2<R>{(R)0, (R)0};
   double swl = 0.0;  // this lane's records' sum of weights, this wavefront's polarisation
   const int ccu = A.cu_n / 2, ccv = A.cv_n / 2;
-  const size_t cfsize = (size_t)A.cu_n * A.cv_n;
+  const int cfsize = A.cu_n * A.cv_n;  // the host bounds ncf * cfsize below 2^31
+  const int cfrow = A.ov_u * A.cv_n;   // elements between taps of one column
   const Cplx2<R> *cfall = (const Cplx2<R> *)A.cfpg;
-
-  // tap t of staged record s for this wavefront's polarisation: its kernel
-  // value's index (always in bounds) and whether the tap counts
-  auto tap = [&](const St &s, int t, size_t &idx, bool &ok, int &lu, int &lv) {
-    const int su = s.su[wp], sv = max(s.sv[wp], 1);
-    int q = (int)((float)t * (1.0f / (float)sv));  // t / sv, then one correction
-    q = q * sv > t ? q - 1 : ((q + 1) * sv <= t ? q + 1 : q);
-    const int iu = q - su / 2, iv = t - q * sv - sv / 2;
-    const int cfu = A.ov_u * iu + s.cfu0, cfv = A.ov_v * iv + s.cfv0;
-    ok = t < su * s.sv[wp] && cfu >= 0 && cfu < A.cu_n && cfv >= 0 && cfv < A.cv_n;
-    idx = (size_t)s.base[wp] * cfsize +
-          (size_t)min(max(cfu, 0), A.cu_n - 1) * A.cv_n + min(max(cfv, 0), A.cv_n - 1);
-    lu = s.cu + iu;
-    lv = s.cv + iv;
-  };
 
   for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
     __syncthreads();  // the previous batch's staged records are consumed
     // ---- stage this lane's record: every wavefront forms the record's
-    // geometry and its own polarisation's data (the first also stores the
-    // geometry), so the four chains of loads run side by side
+    // geometry and its own polarisation's rectangle and data, so the four
+    // chains of loads run side by side
     {
       St &s = st[lane];
-      int cu = 0, cv = 0, cfu0 = 0, cfv0 = 0, pbase = 0, psu = 0, psv = 0;
+      int pn = 0, pw = 1, pcfoff = 0, pcell0 = 0;
+      float prw = 1.0f;
       R pdre = (R)0, pdim = (R)0;
       const uint32_t r = base + lane;
       if (r < first + cnt) {
@@ -162,10 +166,9 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
         const double uu = P.uvw[row * 3] * P.gscale_u[ch] + (double)(P.n_u / 2);
         const double vv = P.uvw[row * 3 + 1] * P.gscale_v[ch] + (double)(P.n_v / 2);
         const int gcu = u0 + au, gcv = v0 + avv;
-        cu = au;
-        cv = avv;
-        cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
-        cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
+        const int cu = au, cv = avv;
+        const int cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
+        const int cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
         const bool prim = r < prim_end;
         const int du = min(max(cfu0 - ccu - A.du_lo, 0), A.ndu - 1);
         const int dv = min(max(cfv0 - ccv - A.dv_lo, 0), A.ndv - 1);
@@ -184,9 +187,21 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
         // mosaic_grid.py:412: skip NaN or exactly-zero data (and, here,
         // convolution functions outside the table)
         if (cf_ok && !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
-          pbase = idx;
-          psu = wsu;
-          psv = wsv;
+          // taps iu in [-wsu / 2, wsu - wsu / 2) whose kernel row
+          // ov_u * iu + cfu0 lies in [0, cu_n) and whose cell cu + iu lies
+          // in the tile; the same along v
+          const int scu = wsu / 2, scv = wsv / 2;
+          const int iu_lo = max(max(-scu, -floor_div(cfu0, A.ov_u)), -cu);
+          const int iu_hi = min(min(wsu - scu - 1, floor_div(A.cu_n - 1 - cfu0, A.ov_u)), T - 1 - cu);
+          const int iv_lo = max(max(-scv, -floor_div(cfv0, A.ov_v)), -cv);
+          const int iv_hi = min(min(wsv - scv - 1, floor_div(A.cv_n - 1 - cfv0, A.ov_v)), TV - 1 - cv);
+          if (iu_hi >= iu_lo && iv_hi >= iv_lo) {
+            pw = iv_hi - iv_lo + 1;
+            pn = (iu_hi - iu_lo + 1) * pw;
+            pcfoff = idx * cfsize + (A.ov_u * iu_lo + cfu0) * A.cv_n + A.ov_v * iv_lo + cfv0;
+            pcell0 = (cu + iu_lo) * TV + cv + iv_lo;
+            prw = (1.0f / (float)pw) * (1.0f + 0x1p-20f);
+          }
           pdre = dre;
           pdim = dim;
           if (prim) {
@@ -195,15 +210,11 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
           }
         }
       }
-      if (wp == 0) {
-        s.cu = cu;
-        s.cv = cv;
-        s.cfu0 = cfu0;
-        s.cfv0 = cfv0;
-      }
-      s.base[wp] = pbase;
-      s.su[wp] = psu;
-      s.sv[wp] = psv;
+      s.n[wp] = pn;
+      s.w[wp] = pw;
+      s.cfoff[wp] = pcfoff;
+      s.cell0[wp] = pcell0;
+      s.rw[wp] = prw;
       s.dre[wp] = pdre;
       s.dim[wp] = pdim;
     }
@@ -214,20 +225,22 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
     Cplx2<R> *ptile = tile + (size_t)wp * T * TV;
     for (int k = 0; k < nst; ++k) {
       const St &s = st[k];
+      // every lane reads the same record: its values are wave-uniform
+      const int n = __builtin_amdgcn_readfirstlane(s.n[wp]);
+      if (n == 0) continue;
+      const int w = __builtin_amdgcn_readfirstlane(s.w[wp]);
+      const int cell0 = __builtin_amdgcn_readfirstlane(s.cell0[wp]);
+      const float rw = __builtin_bit_cast(
+          float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, s.rw[wp])));
+      const Cplx2<R> *cf = cfall + __builtin_amdgcn_readfirstlane(s.cfoff[wp]);
       const R dre = s.dre[wp], dim = s.dim[wp];
-      if (dre == (R)0 && dim == (R)0) continue;  // uniform: every lane reads the same record
-      const int ntaps = s.su[wp] * s.sv[wp];
-      for (int t = lane; t < ntaps; t += 64) {
-        size_t idx;
-        bool ok;
-        int lu, lv;
-        tap(s, t, idx, ok, lu, lv);
-        if (ok && lu >= 0 && lu < T && lv >= 0 && lv < TV) {
-          const Cplx2<R> c = cfall[idx];
-          Cplx2<R> &cell = ptile[lu * TV + lv];
-          cell.re += c.re * dre - c.im * dim;
-          cell.im += c.re * dim + c.im * dre;
-        }
+      for (int t = lane; t < n; t += 64) {
+        const int q = (int)((float)t * rw);
+        const int pc = t - (int)__umul24(q, w);
+        const Cplx2<R> c = cf[__umul24(q, cfrow) + __umul24(pc, A.ov_v)];
+        Cplx2<R> &cell = ptile[cell0 + (int)__umul24(q, TV) + pc];
+        cell.re += c.re * dre - c.im * dim;
+        cell.im += c.re * dim + c.im * dre;
       }
     }
   }
@@ -280,6 +293,12 @@ static int launch_mosaic(av_plan *plan, MosaicArgs<R> A, double *sumwt, int mode
   // (first-tap offsets: floor(x * ov + 0.5) for x in (-1/2, 1/2], within
   // [-ov / 2 - 1, ov / 2 + 1])
   const int64_t ncf = (int64_t)A.ncb * A.ncc * A.ncp;
+  if (ncf * A.cu_n * A.cv_n >= (int64_t)1 << 31)
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: convolution functions of 2^31 elements or more");
+  if ((int64_t)A.ov_u * A.cv_n >= 1 << 24 || A.ov_v >= 1 << 24)
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: ov_u * cv_n must be below 2^24");
+  if (d.tile_u * d.tile_v > 1 << 16)
+    return fail(AV_ERR_UNSUPPORTED, "av_mosaic_grid: tile of more than 2^16 cells");
   A.du_lo = -(A.ov_u / 2) - 1;
   A.ndu = A.ov_u + 3;
   A.dv_lo = -(A.ov_v / 2) - 1;
@@ -292,7 +311,7 @@ static int launch_mosaic(av_plan *plan, MosaicArgs<R> A, double *sumwt, int mode
                      A, (int)ncf);
   int rc = hipGetLastError() == hipSuccess ? AV_OK : fail(AV_ERR_HIP, "k_mosaic_norms launch");
   if (rc == AV_OK) {
-    hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)d.max_gunits), dim3(64 * NPOL),
+    hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)xcd_grouped_grid(d.max_gunits, kMosaicGroup)), dim3(64 * NPOL),
                        lds, s, d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
     if (hipGetLastError() != hipSuccess) rc = fail(AV_ERR_HIP, "k_mosaic_grid launch");
   }

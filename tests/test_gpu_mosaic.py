@@ -98,3 +98,47 @@ def test_mosaic_with_standard_kernel_is_the_standard_grid(gpu, oracle):
     # (centre +- 3); at 0.40 of the cell limit every sample passes both
     assert rel(g, gs) < 1e-12
     np.testing.assert_allclose(s, ss, rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_mosaic_taps_clipped_by_function_and_tile(gpu, oracle, prec):
+    """Supports of 1, 2, 4, 6 and 10 taps (even and odd, unequal per axis)
+    and convolution functions narrower than the oversampled footprint, so
+    that taps fall off the function's edges and are skipped (as the oracle
+    skips them), with 16 x 16 or 32 x 32 tiles cutting each footprint and a
+    share of the data exactly zero."""
+    from astroviper_amd.mosaic_grid import mosaic_grid_jit
+    d = problem(seed=11)
+    rng = np.random.default_rng(11)
+    ov = (3, 5)
+    ws = rng.choice(np.array([1, 2, 4, 6, 10]), size=(3, 2, 2, 2))
+    ws[0, 0, 0] = (1, 10)
+    ws[1, 1, 1] = (10, 2)
+    # a footprint of s taps spans ov * s kernel rows; these hold about 2/3
+    cu_n, cv_n = 2 * ov[0] * 10 // 3 + 1, 2 * ov[1] * 10 // 3
+    # real parts dominant, so the sums of weights (over Re(norm^2)) do not cancel
+    cf = (rng.uniform(0.5, 1.5, size=(3, 2, 2, cu_n, cv_n))
+          + 0.2j * rng.normal(size=(3, 2, 2, cu_n, cv_n)))
+    pg = np.exp(1j * rng.uniform(-0.3, 0.3, size=(2, cu_n, cv_n)))
+    nt, nb, nchan, npol = d["w"].shape
+    vis, w = d["vis"].copy(), d["w"].copy()
+    w[rng.random(w.shape) < 0.1] = 0.0
+    if prec == 32:
+        vis = vis.astype(np.complex64).astype(np.complex128)
+        w = w.astype(np.float32).astype(np.float64)
+    for smax in (int(ws.max()), 6):  # tiles of 32 (support 21) and of 16 (support 13)
+        wsm = np.minimum(ws, smax)
+        args = (d["uvw"], d["freq"], np.arange(nchan), np.arange(npol), d["cfbm"], d["cfcm"],
+                d["cfpm"], w, cf, np.array(d["n_uv"]), d["cell"], wsm, np.array(ov), pg)
+        gref = np.zeros((nchan, npol) + d["n_uv"], dtype=np.complex128)
+        sref = np.zeros((nchan, npol))
+        oracle.mosaic_grid(gref, sref, False, vis, *args)
+        assert np.count_nonzero(gref) > 1000
+        g = np.zeros((nchan, npol) + d["n_uv"],
+                     dtype=np.complex64 if prec == 32 else np.complex128)
+        s = np.zeros((nchan, npol))
+        mosaic_grid_jit(g, s, False, vis, *args)
+        e = rel(g, gref)
+        print(f"mosaic clipped taps smax={smax} FP{prec}: grid {e:.2e}", flush=True)
+        assert e < (1e-5 if prec == 32 else 1e-12)
+        np.testing.assert_allclose(s, sref, rtol=1e-5 if prec == 32 else 1e-12)
