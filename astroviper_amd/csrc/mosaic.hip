@@ -135,6 +135,7 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
   const Cplx2<R> *cfall = (const Cplx2<R> *)A.cfpg;
 
   for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
+    uint64_t live;  // the staged records with taps for this wavefront's polarisation
     __syncthreads();  // the previous batch's staged records are consumed
     // ---- stage this lane's record: every wavefront forms the record's
     // geometry and its own polarisation's rectangle and data, so the four
@@ -210,6 +211,7 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
           }
         }
       }
+      live = __ballot(pn > 0);
       s.n[wp] = pn;
       s.w[wp] = pw;
       s.cfoff[wp] = pcfoff;
@@ -221,13 +223,11 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
     __syncthreads();
     // ---- the staged records, one after another; lanes over the taps of
     // this wavefront's polarisation, v fastest (t = iu' * sv + iv')
-    const int nst = (int)min((uint32_t)kMosaicStage, first + cnt - base);
     Cplx2<R> *ptile = tile + (size_t)wp * T * TV;
-    for (int k = 0; k < nst; ++k) {
-      const St &s = st[k];
+    for (; live; live &= live - 1) {
+      const St &s = st[__builtin_ctzll(live)];
       // every lane reads the same record: its values are wave-uniform
       const int n = __builtin_amdgcn_readfirstlane(s.n[wp]);
-      if (n == 0) continue;
       const int w = __builtin_amdgcn_readfirstlane(s.w[wp]);
       const int cell0 = __builtin_amdgcn_readfirstlane(s.cell0[wp]);
       const float rw = __builtin_bit_cast(
@@ -237,10 +237,13 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
       for (int t = lane; t < n; t += 64) {
         const int q = (int)((float)t * rw);
         const int pc = t - (int)__umul24(q, w);
-        const Cplx2<R> c = cf[__umul24(q, cfrow) + __umul24(pc, A.ov_v)];
+        // an unsigned offset: no sign extension before the 64-bit address
+        const uint32_t off = (uint32_t)__umul24(q, cfrow) + (uint32_t)__umul24(pc, A.ov_v);
+        const Cplx2<R> c = cf[off];
         Cplx2<R> &cell = ptile[cell0 + (int)__umul24(q, TV) + pc];
-        cell.re += c.re * dre - c.im * dim;
-        cell.im += c.re * dim + c.im * dre;
+        // the product first, then the cell, as the reference adds them
+        cell.re += fma(c.re, dre, -(c.im * dim));
+        cell.im += fma(c.re, dim, c.im * dre);
       }
     }
   }
