@@ -159,7 +159,8 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
           vre = A.vis[((size_t)item * NPOL + wp) * 2];
           vim = A.vis[((size_t)item * NPOL + wp) * 2 + 1];
         }
-        const int cfb = A.cf_baseline_map[(int)(row % A.n_baseline)];
+        // a 32-bit remainder: row < 2^32, as item = row * n_chan + ch is 32-bit
+        const int cfb = A.cf_baseline_map[(uint32_t)row % (uint32_t)A.n_baseline];
         const int cfc = A.cf_chan_map[ch];
         const int idx = (cfb * A.ncc + cfc) * A.ncp + A.cf_pol_map[wp];
         // the reference's offsets at this call's oversampling, from the
