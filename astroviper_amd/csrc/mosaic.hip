@@ -5,9 +5,12 @@
 // convolution function (already multiplied by the field's phase gradient)
 // chosen by its baseline, channel and polarisation, over a support that the
 // same three indices choose.  The samples come binned by a plan whose
-// support is 2 * max_support + 1: its bounds test is then the reference's
-// (u_centre +- max_support inside the grid) and every footprint lies inside
-// the tiles the plan records the sample in.
+// support is at least max_support = max(weight_support): every footprint
+// then lies inside the tiles the plan records the sample in, and the plan's
+// own bounds test keeps every sample that the reference's test (u_centre +-
+// max_support inside the grid, mosaic_grid.py:379-383) keeps.  That test is
+// applied here as records are staged, with max_support reduced from the
+// table on the device.
 //
 // One workgroup per work unit, one wavefront per polarisation.  The unit's
 // tile, all its polarisations (polarisation-major, so the lanes of a
@@ -59,7 +62,20 @@ struct MosaicArgs {
   // offsets cu_n / 2 + du_lo + du and cv_n / 2 + dv_lo + dv
   const double *norms;
   int du_lo, ndu, dv_lo, ndv;
+  const int *max_support = nullptr;  // max(weight_support), one value on the device
 };
+
+// max(weight_support): the reference's max_support_center
+// (mosaic_grid.py:345), by one workgroup of 256 threads
+__global__ __launch_bounds__(256) void k_mosaic_max_support(const int32_t *ws, int n, int *out) {
+  __shared__ int red[4];
+  int m = INT_MIN;
+  for (int i = threadIdx.x; i < n; i += 256) m = max(m, ws[i]);
+  for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = max(max(red[0], red[1]), max(red[2], red[3]));
+}
 
 // per staged record and polarisation: the h x w rectangle of its taps that
 // count (n = h * w, 0 for a record with nothing to add here).  Tap t of it
@@ -133,6 +149,7 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
   const int cfsize = A.cu_n * A.cv_n;  // the host bounds ncf * cfsize below 2^31
   const int cfrow = A.ov_u * A.cv_n;   // elements between taps of one column
   const Cplx2<R> *cfall = (const Cplx2<R> *)A.cfpg;
+  const int msup = *A.max_support;
 
   for (uint32_t base = first; base < first + cnt; base += kMosaicStage) {
     uint64_t live;  // the staged records with taps for this wavefront's polarisation
@@ -169,6 +186,9 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
         const double vv = P.uvw[row * 3 + 1] * P.gscale_v[ch] + (double)(P.n_v / 2);
         const int gcu = u0 + au, gcv = v0 + avv;
         const int cu = au, cv = avv;
+        // the reference's bounds test (mosaic_grid.py:379-383), written
+        // without sums that could overflow for any table value
+        const bool inside = gcu >= msup && P.n_u - gcu > msup && gcv >= msup && P.n_v - gcv > msup;
         const int cfu0 = (int)floor(((double)gcu - uu) * (double)A.ov_u + 0.5) + ccu;
         const int cfv0 = (int)floor(((double)gcv - vv) * (double)A.ov_v + 0.5) + ccv;
         const bool prim = r < prim_end;
@@ -188,7 +208,7 @@ __global__ __launch_bounds__(64 * NPOL) void k_mosaic_grid(PlanDev P, MosaicArgs
         }
         // mosaic_grid.py:412: skip NaN or exactly-zero data (and, here,
         // convolution functions outside the table)
-        if (cf_ok && !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
+        if (inside && cf_ok && !(isnan(dre) || isnan(dim)) && !(dre == (R)0 && dim == (R)0)) {
           // taps iu in [-wsu / 2, wsu - wsu / 2) whose kernel row
           // ov_u * iu + cfu0 lies in [0, cu_n) and whose cell cu + iu lies
           // in the tile; the same along v
@@ -308,15 +328,25 @@ static int launch_mosaic(av_plan *plan, MosaicArgs<R> A, double *sumwt, int mode
   A.dv_lo = -(A.ov_v / 2) - 1;
   A.ndv = A.ov_v + 3;
   const int64_t entries = ncf * A.ndu * A.ndv;
+  // one stream-ordered block: the kernel sums, then max(weight_support)
   void *norms = nullptr;
-  AV_HIP(hipMallocAsync(&norms, (size_t)entries * 2 * sizeof(double), s));
+  AV_HIP(hipMallocAsync(&norms, (size_t)entries * 2 * sizeof(double) + 16, s));
   A.norms = (const double *)norms;
+  int *msup = (int *)((char *)norms + (size_t)entries * 2 * sizeof(double));
+  A.max_support = msup;
   hipLaunchKernelGGL((k_mosaic_norms<R>), dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, s,
                      A, (int)ncf);
   int rc = hipGetLastError() == hipSuccess ? AV_OK : fail(AV_ERR_HIP, "k_mosaic_norms launch");
   if (rc == AV_OK) {
-    hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>), dim3((unsigned)xcd_grouped_grid(d.max_gunits, kMosaicGroup)), dim3(64 * NPOL),
-                       lds, s, d, A, (mode & AV_GRID_PSF) ? 1 : 0, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(k_mosaic_max_support, dim3(1), dim3(256), 0, s, A.weight_support,
+                       (int)(2 * ncf), msup);
+    if (hipGetLastError() != hipSuccess) rc = fail(AV_ERR_HIP, "k_mosaic_max_support launch");
+  }
+  if (rc == AV_OK) {
+    hipLaunchKernelGGL((k_mosaic_grid<R, NPOL>),
+                       dim3((unsigned)xcd_grouped_grid(d.max_gunits, kMosaicGroup)),
+                       dim3(64 * NPOL), lds, s, d, A, (mode & AV_GRID_PSF) ? 1 : 0,
+                       accumulate ? 1 : 0);
     if (hipGetLastError() != hipSuccess) rc = fail(AV_ERR_HIP, "k_mosaic_grid launch");
   }
   const hipError_t fe = hipFreeAsync(norms, s);

@@ -724,7 +724,7 @@ int plan_layout(int64_t n_rows, int32_t n_chan, int32_t n_img_chan, int32_t n_u,
   if (n_rows < 0 || n_chan < 1 || n_img_chan < 1 || n_u < 1 || n_v < 1)
     return fail(AV_ERR_INVALID, "av_plan_create: sizes must be positive");
   // plans bin footprints up to kMaxPlanSupport wide (the mosaic gridder's
-  // 2 * max_support + 1); av_grid and av_degrid take supports up to
+  // max_support); av_grid and av_degrid take supports up to
   // kMaxSupport
   if (support < 2 || support > kMaxPlanSupport)
     return fail(AV_ERR_UNSUPPORTED, "av_plan_create: support must be in [2, 63]");

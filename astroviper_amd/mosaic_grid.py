@@ -25,6 +25,15 @@ def _tile_for(support: int) -> int:
     raise NotImplementedError(f"convolution support {support} exceeds the widest tile")
 
 
+def _plan_shape(max_support: int):
+    """The binning plan's support and tile for a mosaic call.  The plan
+    records each sample in the tiles its footprint (at most max_support taps
+    per axis) touches, and its bounds test at that support keeps every sample
+    the reference keeps; the gridder applies the reference's own test
+    (centre +- max_support inside the grid, mosaic_grid.py:379-383)."""
+    return max(2, max_support), _tile_for(2 * max_support + 1)
+
+
 def mosaic_grid_jit(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map, pol_map,
                     cf_baseline_map, cf_chan_map, cf_pol_map, imaging_weight, conv_kernel, n_uv,
                     delta_lm, weight_support, oversampling, phase_gradient):
@@ -43,8 +52,7 @@ def mosaic_grid_jit(grid, sum_weight, do_psf, vis_data, uvw, freq_chan, chan_map
     dev = _dev.require_gpu()
     ws = np.ascontiguousarray(weight_support, dtype=np.int32)
     max_support = int(np.max(ws))
-    support = 2 * max_support + 1  # the plan's bounds test is then the reference's
-    tile = _tile_for(support)
+    support, tile = _plan_shape(max_support)
     ov = (int(oversampling[0]), int(oversampling[1]))
     ck = np.asarray(conv_kernel)
     ncb, ncc, ncp, cu_n, cv_n = ck.shape

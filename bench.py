@@ -594,7 +594,7 @@ def row_mosaic(dev, n_time=100, cpu_s=5.0):
 
     from astroviper_amd import _lib, synth
     from astroviper_amd import device as D
-    from astroviper_amd.mosaic_grid import _tile_for
+    from astroviper_amd.mosaic_grid import _plan_shape
 
     cfg = synth.CONFIG2
     arr = synth.ArrayConfig(**{**cfg["array"].__dict__, "n_time": n_time})
@@ -615,8 +615,7 @@ def row_mosaic(dev, n_time=100, cpu_s=5.0):
     cfcm = (np.arange(nchan) * ncc // nchan).astype(np.int32)
     cfpm = np.zeros(npol, dtype=np.int32)
     smax = int(ws.max())
-    support = 2 * smax + 1
-    tile = _tile_for(support)
+    support, tile = _plan_shape(smax)
     w_eff = torch.where(flag.bool(), torch.zeros_like(w), w)  # flagged samples weigh 0
     freq = torch.from_numpy(obs.freq).to(dev)
 

@@ -211,8 +211,10 @@ int av_grid_noalloc(const double *uvw, int64_t n_rows, const double *freq,
 
 /* mosaic_grid_jit, core/imaging/imaging_utils/mosaic_grid.py:306-486: the
  * mosaic / A-projection gridder.  Every sample of the plan (created with
- * AV_GEOM_REFERENCE, support = 2 * max(weight_support) + 1, so that its
- * bounds test is the reference's, and tiles at least that wide) is
+ * AV_GEOM_REFERENCE, support >= max(weight_support), so that each footprint
+ * lies in the tiles the plan records, and tiles at least that wide) whose
+ * centre passes the reference's bounds test (centre +- max(weight_support)
+ * inside the grid, mosaic_grid.py:379-383, applied by the call) is
  * convolved with the complex convolution function cfpg[cfb][cfc][cfp]
  * [cu_n][cv_n] (cfpg holds ncb x ncc x ncp of them) (conv_kernel * phase_gradient[0], mosaic_grid.py:359) of
  * cfb = cf_baseline_map[row % n_baseline], cfc = cf_chan_map[chan],

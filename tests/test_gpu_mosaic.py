@@ -142,3 +142,41 @@ def test_mosaic_taps_clipped_by_function_and_tile(gpu, oracle, prec):
         print(f"mosaic clipped taps smax={smax} FP{prec}: grid {e:.2e}", flush=True)
         assert e < (1e-5 if prec == 32 else 1e-12)
         np.testing.assert_allclose(s, sref, rtol=1e-5 if prec == 32 else 1e-12)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_mosaic_bounds_test_at_the_grid_edges(gpu, oracle, prec):
+    """Cells large enough that the longest baselines reach the grid's edges:
+    the binning plan (at the footprint's support) keeps samples whose centre
+    lies within max_support of an edge, and the gridder must drop them as the
+    reference's test (centre +- max_support inside the grid) does."""
+    from astroviper_amd.mosaic_grid import mosaic_grid_jit
+    d = problem(seed=13)
+    nt, nb, nchan, npol = d["w"].shape
+    cell = tuple(x * 0.52 / 0.40 for x in d["cell"])  # |u| up to 0.52 of the half-grid span
+    vis, w = d["vis"], d["w"]
+    if prec == 32:
+        vis = vis.astype(np.complex64).astype(np.complex128)
+        w = w.astype(np.float32).astype(np.float64)
+    args = (d["uvw"], d["freq"], np.arange(nchan), np.arange(npol), d["cfbm"], d["cfcm"],
+            d["cfpm"], w, d["cf"], np.array(d["n_uv"]), cell, d["ws"], np.array(d["ov"]), d["pg"])
+    # the reference's test drops samples that a test at max_support / 2 keeps
+    n = d["n_uv"][0]
+    ms = int(d["ws"].max())
+    f = d["freq"][None, None, :]
+    pos = np.abs(np.stack([d["uvw"][..., 0:1] * f * cell[0] * n / C,
+                           d["uvw"][..., 1:2] * f * cell[1] * n / C]))
+    between = np.logical_and(pos.max(axis=0) > n / 2 - ms - 1, pos.max(axis=0) < n / 2 - ms // 2 - 2)
+    assert np.count_nonzero(between) > 50
+    for psf in (False, True):
+        gref = np.zeros((nchan, npol) + d["n_uv"], dtype=np.complex128)
+        sref = np.zeros((nchan, npol))
+        oracle.mosaic_grid(gref, sref, psf, vis, *args)
+        g = np.zeros((nchan, npol) + d["n_uv"],
+                     dtype=np.complex64 if prec == 32 else np.complex128)
+        s = np.zeros((nchan, npol))
+        mosaic_grid_jit(g, s, psf, vis, *args)
+        e = rel(g, gref)
+        print(f"mosaic edges FP{prec} psf={psf}: grid {e:.2e}", flush=True)
+        assert e < (1e-5 if prec == 32 else 1e-12)
+        np.testing.assert_allclose(s, sref, rtol=1e-6 if prec == 32 else 1e-12)
