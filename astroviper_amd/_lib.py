@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libavgrid.so")
 
-AV_ABI_VERSION = 11  # must equal AV_ABI_VERSION of include/avgrid.h
+AV_ABI_VERSION = 12  # must equal AV_ABI_VERSION of include/avgrid.h
 
 AV_OK = 0
 AV_ERR_INVALID = -1
@@ -74,6 +74,8 @@ SIGNATURES = {
     "av_shift_taper": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I32, _D,
                                   _I32, _I32, _I32, _I32, _P]),
     "av_normalise": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _I32, _P]),
+    "av_ifftshift_hermitian": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _P]),
+    "av_normalise_real": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P, _I32, _P]),
     "av_hogbom_work_bytes": (_c.c_int, [_P]),
     "av_hogbom_maximg": (_c.c_int, [_P, _P, _I32, _I32, _I32, _P, _P]),
     "av_hogbom_clean": (_c.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,

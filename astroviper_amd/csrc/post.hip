@@ -48,6 +48,60 @@ __global__ void k_normalise(const R *fft, const double *sumwt, const double *tu,
   }
 }
 
+// The half spectrum whose complex-to-real inverse FFT is the shifted real
+// image: for even n_u and n_v, with X = ifftshift(in),
+//   real(fftshift(ifft2(X))) = irfft2(Y),
+//   Y[ku][kv] = (-1)^(ku + kv) * (X[ku][kv] + conj(X[-ku][-kv])) / 2,
+// kv in [0, n_v / 2], indices mod n.  The bracket is X's Hermitian part
+// (whose inverse transform is the real part of X's), and the sign moves the
+// result by half a period on each axis, which is fftshift for even n.  Y is
+// exactly Hermitian on its self-conjugate columns, as the complex-to-real
+// transform assumes.  out is complex FP64 [n_planes][n_u][n_v / 2 + 1].
+template <typename R>
+__global__ void k_ifftshift_hermitian(const R *in, double *out, int64_t n_planes, int n_u,
+                                      int n_v) {
+  const int nh = n_v / 2 + 1;
+  const int64_t hplane = (int64_t)n_u * nh, plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * hplane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / hplane;
+    const int64_t c = i - p * hplane;
+    const int ku = (int)(c / nh), kv = (int)(c - (int64_t)ku * nh);
+    // X[k] = in[(k + n / 2) mod n]; X[-k] = in[(n / 2 - k) mod n]
+    int su = ku + n_u / 2;
+    su -= su >= n_u ? n_u : 0;
+    int sv = kv + n_v / 2;
+    sv -= sv >= n_v ? n_v : 0;
+    int mu = n_u / 2 - ku;
+    mu += mu < 0 ? n_u : 0;
+    int mv = n_v / 2 - kv;
+    mv += mv < 0 ? n_v : 0;
+    const R *a = in + (p * plane + (int64_t)su * n_v + sv) * 2;
+    const R *b = in + (p * plane + (int64_t)mu * n_v + mv) * 2;
+    const double sgn = ((ku + kv) & 1) ? -0.5 : 0.5;
+    out[i * 2] = sgn * ((double)a[0] + (double)b[0]);
+    out[i * 2 + 1] = sgn * ((double)a[1] - (double)b[1]);
+  }
+}
+
+// The real image of k_ifftshift_hermitian's transform (already shifted):
+// image / corrTerm * nx * ny / sumwt, left to right, in FP64, as k_normalise.
+template <typename O>
+__global__ void k_normalise_real(const double *img, const double *sumwt, const double *tu,
+                                 const double *tv, int64_t n_planes, int n_u, int n_v, O *out) {
+  const int64_t plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / plane;
+    const int64_t c = i - p * plane;
+    const int ku = (int)(c / n_v), kv = (int)(c - (int64_t)ku * n_v);
+    const double corr = tu[ku] * tv[kv];
+    out[i] = (O)(img[i] / corr * (double)n_u * (double)n_v / sumwt[p]);
+  }
+}
+
 // out[k] = scale * in[(k + shift) % n] (per axis), optionally divided by the
 // taper at the source or the destination index, real or complex in/out.
 template <typename R, typename O, bool IN_REAL, bool OUT_REAL>
@@ -161,6 +215,49 @@ int av_shift_taper(const void *in, void *out, int64_t n_planes, int32_t n_u, int
   }
 #undef AV_ST
   AV_LAUNCH_CHECK("k_shift_taper");
+  return AV_OK;
+}
+
+int av_ifftshift_hermitian(const void *in, void *out, int64_t n_planes, int32_t n_u,
+                           int32_t n_v, int32_t dtype, void *stream) {
+  if (!in || !out || n_planes < 0 || n_u < 2 || n_v < 2)
+    return fail(AV_ERR_INVALID, "av_ifftshift_hermitian: bad arguments");
+  if ((n_u | n_v) & 1)
+    return fail(AV_ERR_UNSUPPORTED, "av_ifftshift_hermitian: n_u and n_v must be even");
+  const int64_t total = n_planes * n_u * (int64_t)(n_v / 2 + 1);
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g(grid_blocks(total)), b(256);
+  if (dtype == AV_F32)
+    hipLaunchKernelGGL(k_ifftshift_hermitian<float>, g, b, 0, s, (const float *)in,
+                       (double *)out, n_planes, n_u, n_v);
+  else if (dtype == AV_F64)
+    hipLaunchKernelGGL(k_ifftshift_hermitian<double>, g, b, 0, s, (const double *)in,
+                       (double *)out, n_planes, n_u, n_v);
+  else
+    return fail(AV_ERR_INVALID, "av_ifftshift_hermitian: bad dtype");
+  AV_LAUNCH_CHECK("k_ifftshift_hermitian");
+  return AV_OK;
+}
+
+int av_normalise_real(const double *image, const double *sumwt, const double *taper_u,
+                      const double *taper_v, int64_t n_planes, int32_t n_u, int32_t n_v,
+                      void *out, int32_t out_dtype, void *stream) {
+  if (!image || !sumwt || !taper_u || !taper_v || !out || n_planes < 0 || n_u < 1 || n_v < 1)
+    return fail(AV_ERR_INVALID, "av_normalise_real: bad arguments");
+  const int64_t total = n_planes * n_u * n_v;
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g(grid_blocks(total)), b(256);
+  if (out_dtype == AV_F64)
+    hipLaunchKernelGGL(k_normalise_real<double>, g, b, 0, s, image, sumwt, taper_u, taper_v,
+                       n_planes, n_u, n_v, (double *)out);
+  else if (out_dtype == AV_F32)
+    hipLaunchKernelGGL(k_normalise_real<float>, g, b, 0, s, image, sumwt, taper_u, taper_v,
+                       n_planes, n_u, n_v, (float *)out);
+  else
+    return fail(AV_ERR_INVALID, "av_normalise_real: bad dtype");
+  AV_LAUNCH_CHECK("k_normalise_real");
   return AV_OK;
 }
 

@@ -358,21 +358,40 @@ def dirty_image(grid: torch.Tensor, sumwt: torch.Tensor, support: int = 7,
     The FFT runs in `fft_dtype` (FP64 by default, also for an FP32 grid: the
     taper correction divides by down to grdsf(1)^2 = 1.6e-5 at the image
     edge and would magnify FP32 FFT rounding to ~1e-3 of the peak).
+
+    FP64 transforms of even-sized planes go through a complex-to-real FFT:
+    only the real part of the inverse FFT is kept, and it is the transform of
+    the grid's Hermitian half spectrum, which av_ifftshift_hermitian folds
+    (with both shifts) in one pass; the FFT then does half the work and the
+    passes around it move half the bytes.  Other cases transform the whole
+    complex planes (`work`, if given, holds them).
     """
     code = _real_code(grid.dtype)
     fcode = _real_code(fft_dtype)
     n_u, n_v = grid.shape[-2:]
     planes = grid.numel() // (n_u * n_v)
     g = grid.contiguous()
+    tu = ps_correction_1d(n_u, grid.device)
+    tv = tu if n_v == n_u else ps_correction_1d(n_v, grid.device)
+    sw = sumwt.contiguous().to(torch.float64)
+    if fft_dtype == torch.complex128 and n_u % 2 == 0 and n_v % 2 == 0:
+        half = torch.empty(g.shape[:-1] + (n_v // 2 + 1,), dtype=torch.complex128,
+                           device=g.device)
+        call("av_ifftshift_hermitian", _ptr(g), _ptr(half), planes, n_u, n_v, code, _stream())
+        img = torch.fft.irfft2(half, s=(n_u, n_v), dim=(-2, -1))
+        del half
+        out = img if out_dtype == torch.float64 else torch.empty(grid.shape, dtype=out_dtype,
+                                                                 device=grid.device)
+        call("av_normalise_real", _ptr(img), _ptr(sw), _ptr(tu), _ptr(tv), planes, n_u, n_v,
+             _ptr(out), _real_code(out_dtype), _stream())
+        return out
     if work is None:
         work = torch.empty(g.shape, dtype=fft_dtype, device=g.device)
     call("av_ifftshift", _ptr(g), _ptr(work), planes, n_u, n_v, code, fcode, _stream())
     torch.fft.ifft2(work, dim=(-2, -1), out=work)
-    tu = ps_correction_1d(n_u, grid.device)
-    tv = tu if n_v == n_u else ps_correction_1d(n_v, grid.device)
     out = torch.empty(grid.shape, dtype=out_dtype, device=grid.device)
-    call("av_normalise", _ptr(work), _ptr(sumwt.contiguous().to(torch.float64)), _ptr(tu),
-         _ptr(tv), planes, n_u, n_v, fcode, _ptr(out), _real_code(out_dtype), _stream())
+    call("av_normalise", _ptr(work), _ptr(sw), _ptr(tu), _ptr(tv), planes, n_u, n_v, fcode,
+         _ptr(out), _real_code(out_dtype), _stream())
     return out
 
 

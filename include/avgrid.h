@@ -65,7 +65,7 @@ extern "C" {
 
 /* ABI version: bumped whenever a signature changes; the bindings refuse a
  * library whose version differs from the one they were written for. */
-#define AV_ABI_VERSION 11
+#define AV_ABI_VERSION 12
 
 const char *av_last_error(void);
 int av_version(void);
@@ -285,6 +285,20 @@ int av_normalise(const void *fft_planes, const double *sumwt,
                  const double *taper_u, const double *taper_v,
                  int64_t n_planes, int32_t n_u, int32_t n_v, int32_t dtype,
                  void *out, int32_t out_dtype, void *stream);
+/* The same image through a complex-to-real FFT, for even n_u and n_v
+ * (standard_grid.py:693-708 keeps only the real part of the inverse FFT).
+ * av_ifftshift_hermitian: out[p][ku][kv], kv in [0, n_v/2], complex FP64,
+ * = (-1)^(ku+kv) * (X[ku][kv] + conj(X[-ku][-kv])) / 2 with X =
+ * ifftshift(in[p]) (in complex of dtype); its 2-D complex-to-real inverse
+ * FFT (backward normalisation) is real(fftshift(ifft2(X))).
+ * av_normalise_real: out[p] = image[p] / taper * n_u * n_v / sumwt[p] for
+ * that real FP64 image (already shifted); out float64 or float32. */
+int av_ifftshift_hermitian(const void *in, void *out, int64_t n_planes,
+                           int32_t n_u, int32_t n_v, int32_t dtype, void *stream);
+int av_normalise_real(const double *image, const double *sumwt,
+                      const double *taper_u, const double *taper_v,
+                      int64_t n_planes, int32_t n_u, int32_t n_v, void *out,
+                      int32_t out_dtype, void *stream);
 
 /* ------------------------------------------------------- Hogbom CLEAN --
  * core/imaging/deconvolvers/hogbom (the _hogbom_ext extension).  Images are
