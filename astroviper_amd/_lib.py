@@ -76,6 +76,7 @@ SIGNATURES = {
     "av_normalise": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _I32, _P]),
     "av_ifftshift_hermitian": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "av_normalise_real": (_c.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P, _I32, _P]),
+    "av_hermitian_expand": (_c.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P]),
     "av_hogbom_work_bytes": (_c.c_int, [_P]),
     "av_hogbom_maximg": (_c.c_int, [_P, _P, _I32, _I32, _I32, _P, _P]),
     "av_hogbom_clean": (_c.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,

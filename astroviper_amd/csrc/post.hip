@@ -102,6 +102,36 @@ __global__ void k_normalise_real(const double *img, const double *sumwt, const d
   }
 }
 
+// The whole spectrum of a real image's FFT from its half [n_u][n_v / 2 + 1]
+// (a real-to-complex FFT's output): F[ku][kv] = conj(F[-ku][n_v - kv]) for
+// kv > n_v / 2, indices mod n.  With `sign`, each value is multiplied by
+// (-1)^(ku + kv), the output side of fftshift(fft2(ifftshift(x))) =
+// S * fft2(S * x) for axes that are multiples of 4.
+template <typename R>
+__global__ void k_hermitian_expand(const R *half, R *out, int64_t n_planes, int n_u, int n_v,
+                                   int sign) {
+  const int nh = n_v / 2 + 1;
+  const int64_t plane = (int64_t)n_u * n_v;
+  const int64_t total = n_planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / plane;
+    const int64_t c = i - p * plane;
+    const int ku = (int)(c / n_v), kv = (int)(c - (int64_t)ku * n_v);
+    const bool direct = kv < nh;
+    const int su = direct ? ku : (ku == 0 ? 0 : n_u - ku);
+    const int sv = direct ? kv : n_v - kv;
+    const R *src = half + ((p * n_u + su) * nh + sv) * 2;
+    R re = src[0], im = direct ? src[1] : -src[1];
+    if (sign && ((ku + kv) & 1)) {
+      re = -re;
+      im = -im;
+    }
+    out[i * 2] = re;
+    out[i * 2 + 1] = im;
+  }
+}
+
 // out[k] = scale * in[(k + shift) % n] (per axis), optionally divided by the
 // taper at the source or the destination index, real or complex in/out.
 template <typename R, typename O, bool IN_REAL, bool OUT_REAL>
@@ -258,6 +288,28 @@ int av_normalise_real(const double *image, const double *sumwt, const double *ta
   else
     return fail(AV_ERR_INVALID, "av_normalise_real: bad dtype");
   AV_LAUNCH_CHECK("k_normalise_real");
+  return AV_OK;
+}
+
+int av_hermitian_expand(const void *half, void *out, int64_t n_planes, int32_t n_u, int32_t n_v,
+                        int32_t sign, int32_t dtype, void *stream) {
+  if (!half || !out || n_planes < 0 || n_u < 1 || n_v < 1)
+    return fail(AV_ERR_INVALID, "av_hermitian_expand: bad arguments");
+  if (sign && ((n_u | n_v) & 3))
+    return fail(AV_ERR_INVALID, "av_hermitian_expand: sign needs axes that are multiples of 4");
+  const int64_t total = n_planes * n_u * (int64_t)n_v;
+  if (total == 0) return AV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g(grid_blocks(total)), b(256);
+  if (dtype == AV_F32)
+    hipLaunchKernelGGL(k_hermitian_expand<float>, g, b, 0, s, (const float *)half, (float *)out,
+                       n_planes, n_u, n_v, sign);
+  else if (dtype == AV_F64)
+    hipLaunchKernelGGL(k_hermitian_expand<double>, g, b, 0, s, (const double *)half,
+                       (double *)out, n_planes, n_u, n_v, sign);
+  else
+    return fail(AV_ERR_INVALID, "av_hermitian_expand: bad dtype");
+  AV_LAUNCH_CHECK("k_hermitian_expand");
   return AV_OK;
 }
 

@@ -45,7 +45,13 @@ def model_to_uv(model: torch.Tensor, cdt: torch.dtype) -> torch.Tensor:
     with S[i, j] = (-1)^(i + j), fftshift(fft2(ifftshift(x))) = S * fft2(S * x),
     exactly in exact arithmetic (the shift theorem; (-1)^(N0/2 + N1/2) = 1).
     So the image is cast once, flipped in place, transformed and flipped in
-    place: two passes of the cube instead of the four of two rolls."""
+    place: two passes of the cube instead of the four of two rolls.
+
+    A real image (the model) goes through a real-to-complex FFT, which
+    computes half the spectrum for half the work: the flipped image is
+    formed in the real dtype in one pass, transformed, and
+    av_hermitian_expand writes the whole spectrum from the half
+    (F[-k] = conj(F[k])) with the output flip, in one pass."""
     ny, nx = model.shape[-2:]
     ax = (-2, -1)
     if ny % 4 or nx % 4:
@@ -59,6 +65,16 @@ def model_to_uv(model: torch.Tensor, cdt: torch.dtype) -> torch.Tensor:
         j = torch.arange(nx, device=model.device).reshape(1, -1)
         s = (1 - 2 * ((i + j) % 2)).to(rdt)
         _SIGNS[key] = s
+    if not model.is_complex():
+        rdt = s.dtype
+        x = torch.empty(model.shape, dtype=rdt, device=model.device)
+        torch.mul(model, s, out=x)
+        h = torch.fft.rfft2(x, dim=ax)
+        del x
+        y = torch.empty(model.shape, dtype=cdt, device=model.device)
+        D.call("av_hermitian_expand", D._ptr(h), D._ptr(y), model.numel() // (ny * nx), ny, nx,
+               1, D._real_code(cdt), D._stream())
+        return y
     x = model.to(cdt)
     x.mul_(s)
     y = torch.fft.fft2(x, dim=ax)

@@ -299,6 +299,15 @@ int av_normalise_real(const double *image, const double *sumwt,
                       const double *taper_u, const double *taper_v,
                       int64_t n_planes, int32_t n_u, int32_t n_v, void *out,
                       int32_t out_dtype, void *stream);
+/* fft_lm_to_uv of a real model image (fft.py:5-18, as
+ * predict_model_visibilities applies it, imager.py:96-99) through a
+ * real-to-complex FFT: out[p] [n_u][n_v] complex of dtype is the whole
+ * spectrum F, from its half [n_u][n_v/2 + 1] (F[ku][kv] = conj(F[-ku][n_v -
+ * kv]) past the half), times (-1)^(ku+kv) when sign != 0 (axes multiples of
+ * 4: then fftshift(fft2(ifftshift(x))) = S * fft2(S * x), S = (-1)^(i+j)). */
+int av_hermitian_expand(const void *half, void *out, int64_t n_planes,
+                        int32_t n_u, int32_t n_v, int32_t sign, int32_t dtype,
+                        void *stream);
 
 /* ------------------------------------------------------- Hogbom CLEAN --
  * core/imaging/deconvolvers/hogbom (the _hogbom_ext extension).  Images are

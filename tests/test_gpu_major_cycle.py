@@ -121,7 +121,7 @@ def test_major_cycles_match_oracle_loop(gpu, oracle):
     assert rel(resid.cpu().numpy(), r_img) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [(3, 64, 128), (2, 60, 66), (1, 62, 64)])
+@pytest.mark.parametrize("shape", [(3, 64, 128), (2, 2, 36, 20), (2, 60, 66), (1, 62, 64)])
 @pytest.mark.parametrize("cdt", ["complex64", "complex128"])
 def test_model_to_uv_matches_shifted_fft(gpu, shape, cdt):
     """model_to_uv (the sign-flip form S * fft2(S * x) when both axes are
@@ -135,4 +135,9 @@ def test_model_to_uv_matches_shifted_fft(gpu, shape, cdt):
     ref = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(x, axes=ax), axes=ax), axes=ax)
     got = model_to_uv(torch.from_numpy(x).to(gpu), getattr(torch, cdt)).cpu().numpy()
     assert got.dtype == np.dtype(cdt)
+    assert rel(got, ref) < (1e-6 if cdt == "complex64" else 1e-13)
+    # a complex image takes the complex FFT
+    z = x + 1j * rng.standard_normal(shape)
+    ref = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(z, axes=ax), axes=ax), axes=ax)
+    got = model_to_uv(torch.from_numpy(z).to(gpu), getattr(torch, cdt)).cpu().numpy()
     assert rel(got, ref) < (1e-6 if cdt == "complex64" else 1e-13)
